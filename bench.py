@@ -1,0 +1,157 @@
+"""ct x ct mult + relinearize throughput at N=2^16, L=24 (+2 aux primes, 40 digits) on one MI355X.
+
+A step is one RnsCkksContext::mult followed by one relinearize (the crate's Hadamard-product
+flow, context/ops.rs:63-70, 111-122) on a ciphertext pair already resident in HBM, with the
+relinearization key resident in NTT form.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode per-op|batched] [--batch B]
+                  [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "rns-ckks-rust_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def uniform(moduli, n, rs, polys):
+    out = np.empty((polys, len(moduli), n), dtype=np.uint64)
+    for l, q in enumerate(moduli):
+        out[:, l] = rs.integers(0, q, size=(polys, n), dtype=np.uint64)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="per-op", choices=["per-op"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path has no multi-GPU form here); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from rnsckks import capi as G
+    from rnsckks import params as PR
+
+    p = PR.config_per_op_gpu()
+    n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
+    log(f"[bench] context N={n} L={L} K={K}")
+    ctx = G.Context(p)
+    D = ctx.num_decomp_digits(L)
+    rs = np.random.default_rng(0)
+    lhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2))
+    rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2))
+    log(f"[bench] generating relin key: {D} digits x 2 x {L + K} limbs")
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    ksk = ctx.ksk_create(key, L - 1, K)
+    del key
+    quad = G.DeviceBuffer((3, L, n))
+    out = G.DeviceBuffer((2, L, n))
+
+    def step():
+        ctx.mult(lhs, rhs, L, 40, 40, quad)
+        ctx.relinearize(quad, L, 80, ksk, out)
+
+    for _ in range(args.warmup):
+        step()
+    G.synchronize()
+    log(f"[bench] warmup done; timing {args.steps} steps")
+    e0, e1 = G.Event(), G.Event()
+    G.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    e1.synchronize()
+    G.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_ms(e1)
+    ms_per_step = ms / args.steps
+    value = 1000.0 / ms_per_step
+    log(f"[bench] {ms_per_step:.3f} ms/op ({wall / args.steps * 1e3:.3f} ms wall)")
+
+    # per-kernel timing over a few more steps of the same workload (HIP events on the launch stream)
+    G.profile_enable(True)
+    prof_steps = 3
+    for _ in range(prof_steps):
+        step()
+    prof = G.profile_report()
+    G.profile_enable(False)
+    top = sorted(prof.items(), key=lambda kv: -kv[1][1])
+    for k, (cnt, tot) in top:
+        log(f"[bench]   {k:16s} launches={cnt:5d} total={tot:9.3f} ms avg={tot / cnt * 1e3:9.2f} us")
+    name, (cnt, tot) = top[0]
+    avg_s = tot / cnt / 1e3
+    if name == "digit_mac_rows":
+        alg = (3 * D + 2) * n * 8
+    elif name in ("ntt_fwd_cols", "ntt_fwd_rows", "ntt_inv_rows", "ntt_inv_cols"):
+        alg = None
+    else:
+        alg = None
+    roofline = None
+    if alg is not None:
+        achieved = alg / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, args.cpu_threads)
+
+    line = {
+        "metric": "ct×ct mult+relin ops/s at N=2^16 L=24, batched vs per-op GPU; bit-exact",
+        "value": round(value, 2), "unit": "ops/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 residues (FP64-exact modular arithmetic)", "data": "synthetic (seeded uniform residues)",
+        "config": {"workload": "per-op mult+relinearize, N=2^16, L=24 primes < 2^50, K=2 aux, D=40 digits",
+                   "mode": args.mode, "poly_degree": n, "limbs": L, "aux": K, "digits": D},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "kernels_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in top},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(p, threads):
+    """The oracle (C restatement of the crate's CPU path: u128 % arithmetic, 3 NTTs per
+    multiply_ntt, coefficient-form keys, per-digit mixed-radix lift, OpenMP over limbs and
+    coefficients) timed on one mult + relinearize of the same workload."""
+    from oracle import oracle as O
+    n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
+    O.set_threads(threads)
+    qb = O.Basis(n, p.primes, p.roots)
+    eb = O.Basis(n, p.primes + p.aux_primes, p.roots + p.aux_roots)
+    rs = np.random.default_rng(1)
+    D = O.num_decomp_digits(qb.total_modulus_bits())
+    lhs, rhs = uniform(p.primes, n, rs, 2), uniform(p.primes, n, rs, 2)
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    log(f"[bench] cpu baseline: 1 op on {threads} threads")
+    t = time.perf_counter()
+    quad = O.multiply_ciphertexts(qb, lhs, rhs)
+    O.relinearize(qb, eb, quad, key)
+    dt = time.perf_counter() - t
+    log(f"[bench] cpu baseline {dt:.2f} s/op")
+    return {"value": round(1.0 / dt, 5), "unit": "ops/s", "cores": threads, "kind": "port",
+            "sample": "1 mult+relinearize at N=2^16, L=24, K=2, D=40 (oracle, OpenMP)"}
+
+
+if __name__ == "__main__":
+    main()

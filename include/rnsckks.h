@@ -1,0 +1,159 @@
+/*
+ * rnsckks.h — C ABI of the MI355X (gfx950) RNS-CKKS arithmetic hot path.
+ *
+ * Drop-in boundary for the NTT, poly-multiply, Hadamard multiply-add, ct x ct multiply,
+ * relinearize / keyswitch, rescale and rotate calls of the Rust crate Akram275/rns-ckks-rust.
+ * Each entry point names the crate function it replaces (file:line in that crate).
+ *
+ * Conventions
+ *  - Residues are canonical uint64 in [0, q).  A polynomial with nq Q-limbs and np aux limbs is
+ *    uint64_t[nq + np][N] in DEVICE memory, limb-major: limb l < nq is reduced mod Q-prime l,
+ *    limb nq + k mod aux prime k (the eval-key basis Q[0..nq) || P of
+ *    src/rns/context/construction.rs:76-90).  A ciphertext is [c0][c1] (2 polys back to back),
+ *    a quadratic ciphertext [c0][c1][c2].  The ciphertext level is limbs - 1.
+ *  - All operations are out-of-place unless stated; outputs may alias inputs only where stated.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are asynchronous on it.
+ *  - Return value: RCK_OK (0) or a negative code; rck_last_error() returns the message of the
+ *    last failure on the calling thread.  Where the crate panics on a mismatch (assert!), these
+ *    calls return RCK_ERR_INVALID with the crate's assertion message.
+ *  - A context owns device scratch reused across calls: calls on one context must not overlap
+ *    in time (use one context per concurrently used stream).
+ */
+#ifndef RNSCKKS_H
+#define RNSCKKS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RCK_OK 0
+#define RCK_ERR_INVALID (-1)  /* argument / assertion failure (the crate panics or returns Err) */
+#define RCK_ERR_DEVICE (-2)   /* HIP runtime error */
+#define RCK_ERR_INTERNAL (-3) /* host allocation or other failure */
+
+typedef struct rck_context rck_context;
+typedef struct rck_ksk rck_ksk;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* rck_last_error(void);
+/* ABI version (major * 100 + minor). */
+int rck_version(void);
+
+/* ---- device memory, streams and timing (plumbing for callers without a GPU runtime) ---- */
+int rck_malloc(void** ptr, size_t bytes);
+int rck_free(void* ptr);
+int rck_memcpy_htod(void* dst, const void* src, size_t bytes);
+int rck_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int rck_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int rck_device_synchronize(void);
+int rck_stream_create(void** stream);
+int rck_stream_destroy(void* stream);
+int rck_stream_synchronize(void* stream);
+int rck_event_create(void** event);
+int rck_event_destroy(void* event);
+int rck_event_record(void* event, void* stream);
+int rck_event_synchronize(void* event);
+int rck_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---- context ----
+ * Replaces RnsCkksContext::new(RnsCkksParams) (src/rns/context/construction.rs:9-30) with its
+ * validation (src/rns/context/params.rs:48-73, src/rns/basis.rs:32-59, src/math/ntt.rs:17-60;
+ * same error messages).  primes/roots: the Q chain (RnsPrime{modulus, primitive_root}),
+ * aux_primes/aux_roots: the P chain (may be empty: Q-only keyswitching). */
+int rck_context_create(uint32_t poly_degree, const uint64_t* primes, const uint64_t* roots, uint32_t n_primes,
+                       const uint64_t* aux_primes, const uint64_t* aux_roots, uint32_t n_aux, uint32_t scale_bits,
+                       rck_context** out);
+int rck_context_destroy(rck_context* ctx);
+/* num_decomp_digits(total_modulus_bits(Q[0..limbs)), 30) (src/rns/keyswitching.rs:238-240) */
+int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digits);
+/* galois_element_for_rotation (src/rns/rotation.rs:14-26): 5^steps mod 2N, (5^-1)^|steps| if < 0 */
+uint64_t rck_galois_element_for_rotation(uint32_t poly_degree, int64_t steps);
+
+/* ---- L0: NttPlan::forward / inverse (src/math/ntt.rs:78-89) over every limb, in place ----
+ * Cyclic transform in natural order with w = g^((q-1)/N): X_k = sum_j x_j w^{jk}; inverse
+ * includes N^{-1}.  Bit-identical to the crate's radix-2 NTT. */
+int rck_ntt_forward(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, void* stream);
+int rck_ntt_inverse(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, void* stream);
+
+/* ---- L1/L2: ModularPolynomial / RnsPolynomial limb ops (src/poly/mod.rs, src/rns/polynomial.rs) ---- */
+/* RnsPolynomial::add / sub (polynomial.rs:99-119); out may alias a or b */
+int rck_poly_add(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq, uint32_t np,
+                 void* stream);
+int rck_poly_sub(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq, uint32_t np,
+                 void* stream);
+/* RnsPolynomial::multiply_ntt (polynomial.rs:121-136 -> poly/mod.rs:134-161): negacyclic product */
+int rck_poly_multiply_ntt(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq,
+                          uint32_t np, void* stream);
+/* RnsPolynomial::apply_galois_automorphism (polynomial.rs:138-145 -> poly/mod.rs:110-130) */
+int rck_poly_galois(rck_context* ctx, const uint64_t* a, uint64_t* out, uint32_t nq, uint32_t np, uint64_t galois,
+                    void* stream);
+/* pointwise (Hadamard) modular multiply-add out = a*b + c per coefficient (the pointwise step of
+ * poly/mod.rs:152-154 plus the accumulation of keyswitching.rs:182-187); c may be NULL; out may
+ * alias any input */
+int rck_poly_mul_add(rck_context* ctx, const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* out,
+                     uint32_t nq, uint32_t np, void* stream);
+
+/* ---- L3: RnsCkksContext scheme ops (src/rns/context/ops.rs, src/rns/context/rotation.rs) ---- */
+/* RnsCiphertext::add (src/rns/ciphertexts.rs:60-69) */
+int rck_ct_add(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint32_t limbs, uint32_t lhs_scale,
+               uint32_t rhs_scale, uint64_t* out, uint32_t* out_scale, void* stream);
+/* RnsCkksContext::mult (context/ops.rs:63-70 -> ops.rs:84-100): quadratic out, scale = sum */
+int rck_mult(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint32_t limbs, uint32_t lhs_scale,
+             uint32_t rhs_scale, uint64_t* out_quad, uint32_t* out_scale, void* stream);
+/* RnsCkksContext::mult_plain (context/ops.rs:20-30 -> ops.rs:54-66): scale := 2 * ct scale */
+int rck_mult_plain(rck_context* ctx, const uint64_t* ct, const uint64_t* plain, uint32_t limbs, uint32_t ct_scale,
+                   uint64_t* out, uint32_t* out_scale, void* stream);
+/* RnsCkksContext::rescale (context/ops.rs:32-37 -> ops.rs:68-82, 135-186): out has limbs-1 limbs,
+ * scale -= floor(log2 q_last) (saturating) */
+int rck_rescale(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale, uint64_t* out,
+                uint32_t* out_scale, void* stream);
+/* RnsCkksContext::rescale_quadratic (context/ops.rs:85-90 -> ops.rs:115-133) */
+int rck_rescale_quadratic(rck_context* ctx, const uint64_t* quad, uint32_t limbs, uint32_t scale, uint64_t* out,
+                          uint32_t* out_scale, void* stream);
+
+/* Keyswitching key upload: RnsKeySwitchingKey {ksk, decomp_bits=30, level, aux_level_count}
+ * (src/rns/keyswitching.rs:12-42) plus the RnsRotationKey galois element (src/rns/rotation.rs:8-12;
+ * pass 0 for a relinearization key).  key: uint64_t[digits][2][level+1+aux][N] in HOST memory,
+ * coefficient form (ksk[i].c0, ksk[i].c1).  Converted once to the NTT domain and kept in HBM. */
+int rck_ksk_create(rck_context* ctx, const uint64_t* key, uint32_t digits, uint32_t level, uint32_t aux_level_count,
+                   uint64_t galois_element, rck_ksk** out);
+/* same, from DEVICE memory */
+int rck_ksk_create_device(rck_context* ctx, const uint64_t* key, uint32_t digits, uint32_t level,
+                          uint32_t aux_level_count, uint64_t galois_element, rck_ksk** out);
+int rck_ksk_destroy(rck_ksk* key);
+
+/* RnsCkksContext::keyswitch (context/ops.rs:97-109 -> keyswitching.rs:130-157 Q-only,
+ * :159-194 hybrid): poly has `limbs` Q-limbs; out = [c0][c1] over the same limbs;
+ * *out_scale = limbs (the crate's keyswitch returns scale_bits = level count) */
+int rck_keyswitch(rck_context* ctx, const uint64_t* poly, uint32_t limbs, const rck_ksk* key, uint64_t* out,
+                  uint32_t* out_scale, void* stream);
+/* RnsCkksContext::relinearize (context/ops.rs:111-122): (c0 + ks.c0, c1 + ks.c1), scale kept */
+int rck_relinearize(rck_context* ctx, const uint64_t* quad, uint32_t limbs, uint32_t scale, const rck_ksk* key,
+                    uint64_t* out, uint32_t* out_scale, void* stream);
+/* RnsCkksContext::mult_relinearized (context/ops.rs:124-133): mult, relinearize, rescale;
+ * out has limbs-1 limbs */
+int rck_mult_relinearized(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint32_t limbs,
+                          uint32_t lhs_scale, uint32_t rhs_scale, const rck_ksk* key, uint64_t* out,
+                          uint32_t* out_scale, void* stream);
+/* RnsCiphertext::apply_galois_automorphism (ciphertexts.rs:71-78) */
+int rck_ct_galois(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint64_t galois, uint64_t* out,
+                  void* stream);
+/* RnsCkksContext::rotate / conjugate (context/rotation.rs:104-135) with the key's galois element:
+ * (sigma(c0) + ks.c0, ks.c1), scale kept; key level must equal the ciphertext level */
+int rck_rotate(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale, const rck_ksk* key,
+               uint64_t* out, uint32_t* out_scale, void* stream);
+
+/* ---- diagnostics (no counterpart in the crate) ----
+ * Per-kernel timing: while enabled, every kernel launch is bracketed by HIP events on the stream it
+ * is launched on.  rck_profile_report synchronizes the device and writes a JSON object
+ * {"kernel": [launches, total_ms], ...} for the launches since the previous report. */
+int rck_profile_enable(int on);
+int rck_profile_report(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RNSCKKS_H */

@@ -1,0 +1,546 @@
+// C ABI (include/rnsckks.h): argument checks with the crate's assertion messages, scale/level
+// metadata rules, and the per-op composition of the kernels.
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "../../include/rnsckks.h"
+#include "ops.hpp"
+
+namespace rck {
+Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32_t L, const uint64_t* p,
+                        const uint64_t* gp, uint32_t K, uint32_t scale_bits);
+}
+
+using namespace rck;
+
+struct rck_context {
+    Context* c;
+};
+
+struct rck_ksk {
+    rck_context* owner;
+    double* data;  // NTT form: [digits][2][limbs][N]
+    uint32_t digits, level, aux, limbs;
+    uint64_t galois;
+};
+
+template <typename F>
+static int guard(F&& f) {
+    try {
+        f();
+        return RCK_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return RCK_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return RCK_ERR_INTERNAL;
+    }
+}
+
+static hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+static Context& C(rck_context* ctx) {
+    RCK_REQUIRE(ctx && ctx->c, "null context");
+    return *ctx->c;
+}
+
+static void check_limbs(const Context& c, uint32_t nq, uint32_t np) {
+    RCK_REQUIRE(nq >= 1 && nq <= c.L, "limb count must be in 1..=" + std::to_string(c.L) + ", got " +
+                                          std::to_string(nq));
+    RCK_REQUIRE(np == 0 || np == c.K, "aux limb count must be 0 or the aux basis size");
+}
+
+static void check_ptr(const void* p, const char* what) { RCK_REQUIRE(p != nullptr, std::string("null ") + what); }
+
+// floor(log2 q) of the dropped prime (ops.rs:74)
+static uint32_t floor_log2(uint64_t q) { return 63 - __builtin_clzll(q); }
+
+extern "C" {
+
+const char* rck_last_error(void) { return last_error(); }
+int rck_version(void) { return 100; }
+
+int rck_malloc(void** ptr, size_t bytes) {
+    return guard([&] {
+        check_ptr(ptr, "pointer");
+        RCK_CHECK_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+int rck_free(void* ptr) {
+    return guard([&] { RCK_CHECK_HIP(hipFree(ptr)); });
+}
+int rck_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return guard([&] { RCK_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int rck_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return guard([&] { RCK_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int rck_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+    return guard([&] { RCK_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream))); });
+}
+int rck_device_synchronize(void) {
+    return guard([&] { RCK_CHECK_HIP(hipDeviceSynchronize()); });
+}
+int rck_stream_create(void** stream) {
+    return guard([&] {
+        check_ptr(stream, "stream");
+        hipStream_t s;
+        RCK_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+    });
+}
+int rck_stream_destroy(void* stream) {
+    return guard([&] { RCK_CHECK_HIP(hipStreamDestroy(S(stream))); });
+}
+int rck_stream_synchronize(void* stream) {
+    return guard([&] { RCK_CHECK_HIP(hipStreamSynchronize(S(stream))); });
+}
+int rck_event_create(void** event) {
+    return guard([&] {
+        hipEvent_t e;
+        RCK_CHECK_HIP(hipEventCreate(&e));
+        *event = e;
+    });
+}
+int rck_event_destroy(void* event) {
+    return guard([&] { RCK_CHECK_HIP(hipEventDestroy(static_cast<hipEvent_t>(event))); });
+}
+int rck_event_record(void* event, void* stream) {
+    return guard([&] { RCK_CHECK_HIP(hipEventRecord(static_cast<hipEvent_t>(event), S(stream))); });
+}
+int rck_event_synchronize(void* event) {
+    return guard([&] { RCK_CHECK_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event))); });
+}
+int rck_event_elapsed_ms(void* start, void* stop, float* ms) {
+    return guard([&] {
+        RCK_CHECK_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    });
+}
+
+int rck_context_create(uint32_t poly_degree, const uint64_t* primes, const uint64_t* roots, uint32_t n_primes,
+                       const uint64_t* aux_primes, const uint64_t* aux_roots, uint32_t n_aux, uint32_t scale_bits,
+                       rck_context** out) {
+    return guard([&] {
+        check_ptr(out, "output");
+        *out = nullptr;
+        RCK_REQUIRE(n_primes == 0 || (primes && roots), "null prime list");
+        RCK_REQUIRE(n_aux == 0 || (aux_primes && aux_roots), "null aux prime list");
+        Context* c = context_create(poly_degree, primes, roots, n_primes, aux_primes, aux_roots, n_aux, scale_bits);
+        *out = new rck_context{c};
+    });
+}
+
+int rck_context_destroy(rck_context* ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        delete ctx->c;
+        delete ctx;
+    });
+}
+
+int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digits) {
+    return guard([&] {
+        const Context& c = C(const_cast<rck_context*>(ctx));
+        RCK_REQUIRE(limbs >= 1 && limbs <= c.L, "bad limb count");
+        *digits = (c.qbits[limbs] + 29) / 30;
+    });
+}
+
+uint64_t rck_galois_element_for_rotation(uint32_t poly_degree, int64_t steps) {
+    const uint64_t m = 2ull * poly_degree;
+    uint64_t g = 5 % m, e = steps >= 0 ? (uint64_t)steps : (uint64_t)(-steps);
+    if (steps < 0) {
+        uint64_t inv;
+        if (!h_invmod(g, m, &inv)) return 0;
+        g = inv;
+    }
+    uint64_t r = 1 % m;
+    while (e) {
+        if (e & 1) r = (uint64_t)((unsigned __int128)r * g % m);
+        g = (uint64_t)((unsigned __int128)g * g % m);
+        e >>= 1;
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------ L0 cyclic NTT
+int rck_ntt_forward(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, nq, np);
+        check_ptr(data, "data");
+        const int limbs = nq + np;
+        const long long n = c.n;
+        PrimeMap pm{(int)nq, (int)c.L, 0};
+        double* t = static_cast<double*>(c.workspace(8, (size_t)limbs * n * 8));
+        launch_twist_in(c, {data, n, 0}, {t, n, 0}, limbs, pm, S(stream));
+        launch_fwd(c, kInF64, t, n, 0, t, n, 0, limbs, 1, pm, S(stream));
+        launch_brv_out(c, {t, n, 0}, {data, n, 0}, limbs, pm, S(stream));
+    });
+}
+
+int rck_ntt_inverse(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, nq, np);
+        check_ptr(data, "data");
+        const int limbs = nq + np;
+        const long long n = c.n;
+        PrimeMap pm{(int)nq, (int)c.L, 0};
+        double* t = static_cast<double*>(c.workspace(8, (size_t)2 * limbs * n * 8));
+        double* t2 = t + (size_t)limbs * n;
+        launch_brv_in(c, {data, n, 0}, {t, n, 0}, limbs, pm, S(stream));
+        launch_inv(c, t, n, 0, kOutF64, t2, n, 0, limbs, 1, pm, S(stream));
+        launch_twist_out(c, {t2, n, 0}, {data, n, 0}, limbs, pm, S(stream));
+    });
+}
+
+// ------------------------------------------------------------------ L1/L2 poly ops
+int rck_poly_add(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq, uint32_t np,
+                 void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, nq, np);
+        const long long n = c.n;
+        launch_addsub(c, {a, n, 0}, {b, n, 0}, {out, n, 0}, nq + np, 1, 0, PrimeMap{(int)nq, (int)c.L, 0}, S(stream));
+    });
+}
+
+int rck_poly_sub(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq, uint32_t np,
+                 void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, nq, np);
+        const long long n = c.n;
+        launch_addsub(c, {a, n, 0}, {b, n, 0}, {out, n, 0}, nq + np, 1, 1, PrimeMap{(int)nq, (int)c.L, 0}, S(stream));
+    });
+}
+
+// negacyclic product of `polys` polynomial pairs a[z] * b[z or 0] -> out[z] (NTT resident)
+static void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
+                     long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    const long long n = c.n, P = (long long)limbs * n;
+    const int bpolys = b_ps ? polys : 1;
+    double* t = static_cast<double*>(c.workspace(0, (size_t)(polys + bpolys) * P * 8));
+    double* tb = t + (size_t)polys * P;
+    launch_fwd(c, kInU64, a, n, a_ps, t, n, P, limbs, polys, pm, s);
+    launch_fwd(c, kInU64, b, n, b_ps, tb, n, P, limbs, bpolys, pm, s);
+    launch_mul_f64(c, {t, n, P}, {tb, n, b_ps ? P : 0}, {t, n, P}, limbs, polys, pm, s);
+    launch_inv(c, t, n, P, kOutU64, out, n, out_ps, limbs, polys, pm, s);
+}
+
+int rck_poly_multiply_ntt(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq,
+                          uint32_t np, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, nq, np);
+        check_ptr(a, "a");
+        check_ptr(b, "b");
+        check_ptr(out, "out");
+        const long long P = (long long)(nq + np) * c.n;
+        poly_mul(c, a, P, b, P, out, P, nq + np, 1, PrimeMap{(int)nq, (int)c.L, 0}, S(stream));
+    });
+}
+
+int rck_poly_galois(rck_context* ctx, const uint64_t* a, uint64_t* out, uint32_t nq, uint32_t np, uint64_t galois,
+                    void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, nq, np);
+        RCK_REQUIRE(a != out, "galois automorphism cannot run in place");
+        const long long n = c.n;
+        launch_galois(c, {a, n, 0}, {out, n, 0}, nq + np, 1, galois, PrimeMap{(int)nq, (int)c.L, 0}, S(stream));
+    });
+}
+
+int rck_poly_mul_add(rck_context* ctx, const uint64_t* a, const uint64_t* b, const uint64_t* cc, uint64_t* out,
+                     uint32_t nq, uint32_t np, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, nq, np);
+        const long long n = c.n;
+        launch_mul_add_u64(c, {a, n, 0}, {b, n, 0}, {cc, n, 0}, cc != nullptr, {out, n, 0}, nq + np, 1,
+                           PrimeMap{(int)nq, (int)c.L, 0}, S(stream));
+    });
+}
+
+// ------------------------------------------------------------------ L3 scheme ops
+int rck_ct_add(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint32_t limbs, uint32_t lhs_scale,
+               uint32_t rhs_scale, uint64_t* out, uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        RCK_REQUIRE(lhs_scale == rhs_scale, "ciphertexts must share the same scale to add");
+        const long long n = c.n, P = (long long)limbs * n;
+        launch_addsub(c, {lhs, n, P}, {rhs, n, P}, {out, n, P}, limbs, 2, 0, PrimeMap{(int)limbs, (int)c.L, 0},
+                      S(stream));
+        if (out_scale) *out_scale = lhs_scale;
+    });
+}
+
+static void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, uint64_t* out, hipStream_t s) {
+    const long long n = c.n, P = (long long)limbs * n;
+    PrimeMap pm{limbs, (int)c.L, 0};
+    double* x = static_cast<double*>(c.workspace(0, (size_t)4 * P * 8));
+    double* d = static_cast<double*>(c.workspace(1, (size_t)3 * P * 8));
+    launch_fwd(c, kInU64, lhs, n, P, x, n, P, limbs, 2, pm, s);
+    launch_fwd(c, kInU64, rhs, n, P, x + 2 * P, n, P, limbs, 2, pm, s);
+    launch_tensor(c, x, d, limbs, pm, s);
+    launch_inv(c, d, n, P, kOutU64, out, n, P, limbs, 3, pm, s);
+}
+
+int rck_mult(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint32_t limbs, uint32_t lhs_scale,
+             uint32_t rhs_scale, uint64_t* out_quad, uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        check_ptr(lhs, "lhs");
+        check_ptr(rhs, "rhs");
+        check_ptr(out_quad, "out");
+        RCK_REQUIRE(lhs_scale == rhs_scale, "ciphertexts must share the same scale to multiply");
+        mult_impl(c, lhs, rhs, limbs, out_quad, S(stream));
+        if (out_scale) *out_scale = lhs_scale + rhs_scale;
+    });
+}
+
+int rck_mult_plain(rck_context* ctx, const uint64_t* ct, const uint64_t* plain, uint32_t limbs, uint32_t ct_scale,
+                   uint64_t* out, uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        const long long P = (long long)limbs * c.n;
+        poly_mul(c, ct, P, plain, 0, out, P, limbs, 2, PrimeMap{(int)limbs, (int)c.L, 0}, S(stream));
+        if (out_scale) *out_scale = ct_scale + ct_scale;
+    });
+}
+
+static void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t* out, hipStream_t s) {
+    const long long n = c.n;
+    launch_rescale(c, {in, n, (long long)limbs * n}, {out, n, (long long)(limbs - 1) * n}, limbs, polys, s);
+}
+
+int rck_rescale(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale, uint64_t* out,
+                uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        RCK_REQUIRE(limbs > 1, "rescale requires at least two levels");
+        RCK_REQUIRE((const void*)ct != (const void*)out, "rescale cannot run in place");
+        rescale_impl(c, ct, limbs, 2, out, S(stream));
+        if (out_scale) {
+            uint32_t b = floor_log2(c.hp[limbs - 1].q);
+            *out_scale = scale > b ? scale - b : 0;
+        }
+    });
+}
+
+int rck_rescale_quadratic(rck_context* ctx, const uint64_t* quad, uint32_t limbs, uint32_t scale, uint64_t* out,
+                          uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        RCK_REQUIRE(limbs > 1, "quadratic rescale requires at least two levels");
+        RCK_REQUIRE((const void*)quad != (const void*)out, "rescale cannot run in place");
+        rescale_impl(c, quad, limbs, 3, out, S(stream));
+        if (out_scale) {
+            uint32_t b = floor_log2(c.hp[limbs - 1].q);
+            *out_scale = scale > b ? scale - b : 0;
+        }
+    });
+}
+
+// ------------------------------------------------------------------ keys
+static void ksk_create(rck_context* ctx, const uint64_t* key, bool on_device, uint32_t digits, uint32_t level,
+                       uint32_t aux, uint64_t galois, rck_ksk** out) {
+    Context& c = C(ctx);
+    check_ptr(out, "output");
+    check_ptr(key, "key");
+    *out = nullptr;
+    RCK_REQUIRE(level < c.L, "keyswitch key level out of range");
+    RCK_REQUIRE(aux == 0 || aux == c.K, "aux_level_count must be 0 or the context's aux basis size");
+    RCK_REQUIRE(digits >= 1, "keyswitch key must have digits");
+    const uint32_t limbs = level + 1 + aux;
+    const long long n = c.n, P = (long long)limbs * n;
+    rck_ksk* k = new rck_ksk{ctx, nullptr, digits, level, aux, limbs, galois};
+    try {
+        RCK_CHECK_HIP(hipMalloc(&k->data, (size_t)digits * 2 * P * 8));
+        uint64_t* stage = nullptr;
+        if (!on_device) RCK_CHECK_HIP(hipMalloc(&stage, (size_t)2 * P * 8));
+        PrimeMap pm{(int)(level + 1), (int)c.L, 0};
+        for (uint32_t d = 0; d < digits; d++) {
+            const uint64_t* src = key + (size_t)d * 2 * P;
+            if (!on_device) {
+                RCK_CHECK_HIP(hipMemcpy(stage, src, (size_t)2 * P * 8, hipMemcpyHostToDevice));
+                src = stage;
+            }
+            launch_fwd(c, kInU64, src, n, P, k->data + (size_t)d * 2 * P, n, P, limbs, 2, pm, nullptr);
+        }
+        RCK_CHECK_HIP(hipDeviceSynchronize());
+        if (stage) RCK_CHECK_HIP(hipFree(stage));
+    } catch (...) {
+        if (k->data) (void)hipFree(k->data);
+        delete k;
+        throw;
+    }
+    *out = k;
+}
+
+int rck_ksk_create(rck_context* ctx, const uint64_t* key, uint32_t digits, uint32_t level, uint32_t aux_level_count,
+                   uint64_t galois_element, rck_ksk** out) {
+    return guard([&] { ksk_create(ctx, key, false, digits, level, aux_level_count, galois_element, out); });
+}
+
+int rck_ksk_create_device(rck_context* ctx, const uint64_t* key, uint32_t digits, uint32_t level,
+                          uint32_t aux_level_count, uint64_t galois_element, rck_ksk** out) {
+    return guard([&] { ksk_create(ctx, key, true, digits, level, aux_level_count, galois_element, out); });
+}
+
+int rck_ksk_destroy(rck_ksk* key) {
+    return guard([&] {
+        if (!key) return;
+        RCK_CHECK_HIP(hipDeviceSynchronize());
+        if (key->data) RCK_CHECK_HIP(hipFree(key->data));
+        delete key;
+    });
+}
+
+// keyswitch of `poly` (limbs Q-limbs); adds add0 / add1 (nullable, limbs limbs each) to the two
+// outputs.  Mirrors keyswitch_rns (keyswitching.rs:130-157) and keyswitch_rns_hybrid (:159-194).
+static void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const rck_ksk* key, const uint64_t* add0,
+                           const uint64_t* add1, uint64_t* out, hipStream_t s) {
+    check_ptr(key, "key");
+    const long long n = c.n, P = (long long)limbs * n;
+    const uint32_t active = (c.qbits[limbs] + 29) / 30;
+    int levels;
+    if (key->aux == 0) {
+        RCK_REQUIRE((int)key->level == limbs - 1, "keyswitch key level must match the active modulus chain");
+        RCK_REQUIRE(key->digits >= active, "keyswitch key must cover the active modulus chain");
+        levels = key->digits;  // keyswitching.rs:138 decomposes into ksk.ksk.len() digits
+    } else {
+        RCK_REQUIRE((int)key->level == limbs - 1, "keyswitch key level must match the active Q chain");
+        RCK_REQUIRE(key->aux == c.K, "eval-key context must match the keyswitch key auxiliary basis");
+        RCK_REQUIRE(key->digits >= active, "keyswitch key must cover the active modulus chain");
+        RCK_REQUIRE(c.qbits[limbs] > 30, "hybrid keyswitching needs a ciphertext modulus above 2^30");
+        levels = active;  // keyswitching.rs:170
+    }
+    const int naux = key->aux;
+    const int elimbs = limbs + naux;
+    const long long EP = (long long)elimbs * n;
+    int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)levels * n * 4));
+    double* acc = static_cast<double*>(c.workspace(6, (size_t)2 * EP * 8));
+    launch_decompose(c, poly, limbs, levels, digits, s);
+    launch_digit_mac(c, digits, levels, key->data, limbs, naux, acc, s);
+    PrimeMap pm{limbs, (int)c.L, 0};
+    if (naux) {
+        uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)2 * EP * 8));
+        launch_inv(c, acc, n, EP, kOutU64, accu, n, EP, elimbs, 2, pm, s);
+        launch_mod_down(c, accu, limbs, add0, add1, out, P, s);
+    } else if (add0 || add1) {
+        uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)2 * P * 8));
+        launch_inv(c, acc, n, P, kOutU64, accu, n, P, limbs, 2, pm, s);
+        if (add0)
+            launch_addsub(c, {add0, n, 0}, {accu, n, 0}, {out, n, 0}, limbs, 1, 0, pm, s);
+        else
+            RCK_CHECK_HIP(hipMemcpyAsync(out, accu, P * 8, hipMemcpyDeviceToDevice, s));
+        if (add1)
+            launch_addsub(c, {add1, n, 0}, {accu + P, n, 0}, {out + P, n, 0}, limbs, 1, 0, pm, s);
+        else
+            RCK_CHECK_HIP(hipMemcpyAsync(out + P, accu + P, P * 8, hipMemcpyDeviceToDevice, s));
+    } else {
+        launch_inv(c, acc, n, P, kOutU64, out, n, P, limbs, 2, pm, s);
+    }
+}
+
+int rck_keyswitch(rck_context* ctx, const uint64_t* poly, uint32_t limbs, const rck_ksk* key, uint64_t* out,
+                  uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        check_ptr(poly, "poly");
+        check_ptr(out, "out");
+        keyswitch_impl(c, poly, limbs, key, nullptr, nullptr, out, S(stream));
+        if (out_scale) *out_scale = limbs;
+    });
+}
+
+int rck_relinearize(rck_context* ctx, const uint64_t* quad, uint32_t limbs, uint32_t scale, const rck_ksk* key,
+                    uint64_t* out, uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        check_ptr(quad, "quad");
+        check_ptr(out, "out");
+        const long long P = (long long)limbs * c.n;
+        keyswitch_impl(c, quad + 2 * P, limbs, key, quad, quad + P, out, S(stream));
+        if (out_scale) *out_scale = scale;
+    });
+}
+
+int rck_mult_relinearized(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint32_t limbs,
+                          uint32_t lhs_scale, uint32_t rhs_scale, const rck_ksk* key, uint64_t* out,
+                          uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        RCK_REQUIRE(lhs_scale == rhs_scale, "ciphertexts must share the same scale to multiply");
+        RCK_REQUIRE(limbs > 1, "rescale requires at least two levels");
+        const long long P = (long long)limbs * c.n;
+        uint64_t* quad = static_cast<uint64_t*>(c.workspace(2, (size_t)3 * P * 8));
+        uint64_t* rel = static_cast<uint64_t*>(c.workspace(3, (size_t)2 * P * 8));
+        mult_impl(c, lhs, rhs, limbs, quad, S(stream));
+        keyswitch_impl(c, quad + 2 * P, limbs, key, quad, quad + P, rel, S(stream));
+        rescale_impl(c, rel, limbs, 2, out, S(stream));
+        if (out_scale) {
+            uint32_t s = lhs_scale + rhs_scale, b = floor_log2(c.hp[limbs - 1].q);
+            *out_scale = s > b ? s - b : 0;
+        }
+    });
+}
+
+int rck_ct_galois(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint64_t galois, uint64_t* out,
+                  void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        RCK_REQUIRE(ct != out, "galois automorphism cannot run in place");
+        const long long n = c.n, P = (long long)limbs * n;
+        launch_galois(c, {ct, n, P}, {out, n, P}, limbs, 2, galois, PrimeMap{(int)limbs, (int)c.L, 0}, S(stream));
+    });
+}
+
+int rck_rotate(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale, const rck_ksk* key,
+               uint64_t* out, uint32_t* out_scale, void* stream) {
+    return guard([&] {
+        Context& c = C(ctx);
+        check_limbs(c, limbs, 0);
+        check_ptr(key, "key");
+        check_ptr(ct, "ct");
+        check_ptr(out, "out");
+        RCK_REQUIRE((int)key->level == (int)limbs - 1, "rotation key level must match the ciphertext level");
+        const long long n = c.n, P = (long long)limbs * n;
+        uint64_t* rot = static_cast<uint64_t*>(c.workspace(3, (size_t)2 * P * 8));
+        launch_galois(c, {ct, n, P}, {rot, n, P}, limbs, 2, key->galois, PrimeMap{(int)limbs, (int)c.L, 0},
+                      S(stream));
+        keyswitch_impl(c, rot + P, limbs, key, rot, nullptr, out, S(stream));
+        if (out_scale) *out_scale = scale;
+    });
+}
+
+int rck_profile_enable(int on) {
+    return guard([&] { prof_set(on != 0); });
+}
+
+int rck_profile_report(char* buf, size_t len) {
+    return guard([&] {
+        std::string r = prof_report();
+        RCK_REQUIRE(buf && len > r.size(), "report buffer too small");
+        std::memcpy(buf, r.c_str(), r.size() + 1);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// Device context: the prime chain (Q primes then P primes), per-prime NTT twiddle tables and
+// the basis-conversion constants the keyswitch kernels need, all resident in HBM.
+//
+// Mirrors RnsCkksContext / RnsContext / NttPlan (src/rns/context.rs:15-29,
+// src/rns/basis.rs:24-112, src/math/ntt.rs:4-60): a context is built once per parameter set;
+// the level of an operand is the length of its Q-prefix, and the eval-key basis of level l is
+// Q[0..l] followed by all of P (src/rns/context/construction.rs:72-90).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace rck {
+
+// Per-prime constants read by every kernel (one entry per prime of Q || P).
+struct DevPrime {
+    double q;          // modulus
+    double qinv;       // fl(1/q)
+    double ninv;       // N^{-1} mod q, centred
+    double pad0;
+    const double* fw;  // forward twiddles: fw[k] = psi^{bitrev_logN(k)}, centred, k < N
+    const double* iv;  // inverse twiddles: iv[k] = psi^{-bitrev_logN(k)}, centred
+    const double* tw;  // natural-order psi^{-j} (cyclic-mode pre-twist), centred
+    const double* twi; // natural-order psi^{j}  (cyclic-mode post-twist), centred
+};
+
+// Decomposition / basis-conversion constants (keyswitching.rs:196-322).
+constexpr int kMaxQ = 64;     // Q primes supported by the decomposition kernels
+constexpr int kMaxP = 4;      // aux primes supported by mod-down
+constexpr int kWords = 64;    // 32-bit words of the multiword scratch (2048 bits)
+
+struct HostPrime {
+    uint64_t q, g, root, root_inv, psi, psi_inv, n_inv;
+};
+
+struct Context {
+    uint32_t n = 0, logn = 0;
+    uint32_t L = 0, K = 0;  // number of Q and P primes
+    uint32_t scale_bits = 0;
+    std::vector<HostPrime> hp;  // [L + K]
+    DevPrime* d_primes = nullptr;
+    double* d_tables = nullptr;
+    // inv_q[i * (L+K) + j] = q_j^{-1} mod q_i (centred), for i != j over the whole chain
+    double* d_invq = nullptr;
+    // qwords[l * kWords + w]: 32-bit words of Q_l = q_0 ... q_{l-1}, l = 0..L
+    uint32_t* d_qwords = nullptr;
+    std::vector<std::vector<uint32_t>> qwords;  // host copy
+    std::vector<uint32_t> qbits;                // bits(Q_l)
+    // mod-down constants (aux basis P): pwords = words of P, halfp = floor(P/2)
+    uint32_t* d_pwords = nullptr;
+    double* d_pmodq = nullptr;     // [L] P mod q_i (centred)
+    double* d_pinvq = nullptr;     // [L] P^{-1} mod q_i (centred)
+    double* d_pmod_aux = nullptr;  // [K][L] p_k mod q_i (centred)
+    // scratch owned by the context, grown on demand
+    std::vector<void*> ws_ptr;
+    std::vector<size_t> ws_size;
+
+    uint32_t prime_count() const { return L + K; }
+    void* workspace(int slot, size_t bytes);
+    ~Context();
+};
+
+// errors
+void set_error(const std::string& msg);
+const char* last_error();
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define RCK_CHECK_HIP(expr)                                                                  \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            throw ::rck::Error{-2, std::string(#expr) + ": " + hipGetErrorString(e_)};     \
+        }                                                                                    \
+    } while (0)
+
+#define RCK_REQUIRE(cond, message)                          \
+    do {                                                    \
+        if (!(cond)) throw ::rck::Error{-1, (message)};     \
+    } while (0)
+
+// Per-kernel timing (diagnostics): when enabled, every launch is bracketed by HIP events on its
+// own stream; prof_report() sums the elapsed times per kernel name.
+bool prof_enabled();
+struct ProfScope {
+    hipStream_t s;
+    const char* name;
+    int slot = -1;
+    ProfScope(hipStream_t s_, const char* n_);
+    ~ProfScope();
+};
+void prof_set(bool on);
+std::string prof_report();
+
+// host modular helpers (exact, 128-bit)
+uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q);
+uint64_t h_powmod(uint64_t a, uint64_t e, uint64_t q);
+bool h_invmod(uint64_t a, uint64_t q, uint64_t* out);
+double h_centred(uint64_t x, uint64_t q);
+
+}  // namespace rck

@@ -1,0 +1,362 @@
+// NTT pass kernels and their launchers (see ntt.hpp for the transform and the layout).
+#include "ntt.hpp"
+
+namespace rck {
+
+template <int IN>
+__device__ __forceinline__ double load_in(const void* p, long long idx, double q) {
+    if constexpr (IN == kInU64) {
+        return load_centred(static_cast<const uint64_t*>(p)[idx], q);
+    } else if constexpr (IN == kInI32) {
+        return (double)static_cast<const int32_t*>(p)[idx];
+    } else {
+        return static_cast<const double*>(p)[idx];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Column pass, forward: stages 0..LOGR-1 on the columns x = r + N2 c.
+// Thread (tau, col): tid = tau * S + col, so consecutive lanes touch consecutive columns.
+// ---------------------------------------------------------------------------------------
+template <int LOGR, int IN>
+__global__ __launch_bounds__(256) void k_fwd_cols(Batch<const void> in, int in_elem, Batch<double> out, int logn,
+                                                  PrimeMap pm, const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[R * S];
+    const int N2 = 1 << (logn - LOGR);
+    const int tid = threadIdx.x, col = tid % S, tau = tid / S;
+    const int r = blockIdx.x * S + col;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    const char* src = static_cast<const char*>(in.base) +
+                      ((long long)z * in.ps + (long long)y * in.ls) * in_elem;
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = load_in<IN>(src, r + (long long)N2 * (tau + TPS * e), q);
+    fwd_phase_a(v, P.fw, 1, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[(tau + TPS * e) * S + col] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[(16 * tau + e) * S + col];
+    fwd_phase_b<LOGR>(v, P.fw, 1, tau, q, qinv);
+    double* dst = out.at(z, y);
+#pragma unroll
+    for (int e = 0; e < 16; e++) dst[r + (long long)N2 * (16 * tau + e)] = v[e];
+}
+
+// ---------------------------------------------------------------------------------------
+// Row pass, forward: stages log2(N1) .. logN-1 on the contiguous rows (in place).
+// Thread (row, tau): tid = row * TPS + tau.
+// ---------------------------------------------------------------------------------------
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_fwd_rows(Batch<double> data, int logn, PrimeMap pm,
+                                                  const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[S * (R + R / 16 + 1)];
+    const int N1 = 1 << (logn - LOGR);
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int b = blockIdx.x * S + row;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    double* p = data.at(z, y) + (long long)b * R;
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
+    fwd_phase_a(v, P.fw, N1 + b, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, tau + TPS * e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
+    fwd_phase_b<LOGR>(v, P.fw, N1 + b, tau, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) p[16 * tau + e] = v[e];
+}
+
+// Row pass, inverse (in place).
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, PrimeMap pm,
+                                                  const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[S * (R + R / 16 + 1)];
+    const int N1 = 1 << (logn - LOGR);
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int b = blockIdx.x * S + row;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    double* p = data.at(z, y) + (long long)b * R;
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = p[16 * tau + e];
+    inv_phase_b<LOGR>(v, P.iv, N1 + b, tau, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, 16 * tau + e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, tau + TPS * e)];
+    inv_phase_a(v, P.iv, N1 + b, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) p[tau + TPS * e] = v[e];
+}
+
+// Column pass, inverse, with the N^{-1} scaling; writes canonical u64 or reduced doubles.
+template <int LOGR, int OUT>
+__global__ __launch_bounds__(256) void k_inv_cols(Batch<const double> in, Batch<void> out, int out_elem, int logn,
+                                                  PrimeMap pm, const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[R * S];
+    const int N2 = 1 << (logn - LOGR);
+    const int tid = threadIdx.x, col = tid % S, tau = tid / S;
+    const int r = blockIdx.x * S + col;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    const double* src = in.at(z, y);
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = src[r + (long long)N2 * (16 * tau + e)];
+    inv_phase_b<LOGR>(v, P.iv, 1, tau, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[(16 * tau + e) * S + col] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
+    inv_phase_a(v, P.iv, 1, q, qinv);
+    char* dst = static_cast<char*>(out.base) + ((long long)z * out.ps + (long long)y * out.ls) * out_elem;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const long long idx = r + (long long)N2 * (tau + TPS * e);
+        double x = mulmod(v[e], P.ninv, q, qinv);
+        if constexpr (OUT == kOutU64) {
+            reinterpret_cast<uint64_t*>(dst)[idx] = f64_to_u64(canon(x, q, qinv));
+        } else {
+            reinterpret_cast<double*>(dst)[idx] = reduce(x, q, qinv);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Small rings (N <= 2^11): one workgroup per limb, every stage through LDS.
+// ---------------------------------------------------------------------------------------
+template <int IN>
+__global__ __launch_bounds__(256) void k_small_fwd(Batch<const void> in, int in_elem, Batch<double> out, int logn,
+                                                   PrimeMap pm, const DevPrime* __restrict__ primes) {
+    __shared__ double a[2048];
+    const int n = 1 << logn, y = blockIdx.y, z = blockIdx.z;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    const char* src = static_cast<const char*>(in.base) + ((long long)z * in.ps + (long long)y * in.ls) * in_elem;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = load_in<IN>(src, i, q);
+    __syncthreads();
+    for (int s = 0; s < logn; s++) {
+        const int t = n >> (s + 1);
+        for (int k = threadIdx.x; k < n / 2; k += blockDim.x) {
+            const int g = k / t, j = k % t, x = 2 * g * t + j;
+            double u = a[x], v = a[x + t];
+            ct_bfly(u, v, P.fw[(1 << s) + g], q, qinv);
+            a[x] = reduce(u, q, qinv);
+            a[x + t] = reduce(v, q, qinv);
+        }
+        __syncthreads();
+    }
+    double* dst = out.at(z, y);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = a[i];
+}
+
+template <int OUT>
+__global__ __launch_bounds__(256) void k_small_inv(Batch<const double> in, Batch<void> out, int out_elem, int logn,
+                                                   PrimeMap pm, const DevPrime* __restrict__ primes) {
+    __shared__ double a[2048];
+    const int n = 1 << logn, y = blockIdx.y, z = blockIdx.z;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    const double* src = in.at(z, y);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = src[i];
+    __syncthreads();
+    for (int s = logn - 1; s >= 0; s--) {
+        const int t = n >> (s + 1);
+        for (int k = threadIdx.x; k < n / 2; k += blockDim.x) {
+            const int g = k / t, j = k % t, x = 2 * g * t + j;
+            double u = a[x], v = a[x + t];
+            gs_bfly(u, v, P.iv[(1 << s) + g], q, qinv);
+            a[x] = reduce(u, q, qinv);
+            a[x + t] = reduce(v, q, qinv);
+        }
+        __syncthreads();
+    }
+    char* dst = static_cast<char*>(out.base) + ((long long)z * out.ps + (long long)y * out.ls) * out_elem;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double x = mulmod(a[i], P.ninv, q, qinv);
+        if constexpr (OUT == kOutU64)
+            reinterpret_cast<uint64_t*>(dst)[i] = f64_to_u64(canon(x, q, qinv));
+        else
+            reinterpret_cast<double*>(dst)[i] = reduce(x, q, qinv);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------
+static int elem_size(int kind) { return kind == kInI32 ? 4 : 8; }
+
+static void check_dims(const Context& c, int limbs, int polys) {
+    RCK_REQUIRE(c.logn >= 1 && c.logn <= 16, "ring degree must be in 2^1 .. 2^16");
+    RCK_REQUIRE(limbs > 0 && limbs <= 65535 && polys > 0 && polys <= 65535, "bad launch batch");
+}
+
+#define RCK_LAUNCH_CHECK() RCK_CHECK_HIP(hipGetLastError())
+
+template <int LOGR>
+static void fwd_cols_t(const Context& c, int in_kind, Batch<const void> in, Batch<double> out, int limbs, int polys,
+                       PrimeMap pm, hipStream_t s) {
+    constexpr int S = 256 / ((1 << LOGR) / 16);
+    const int n2 = 1 << (c.logn - LOGR);
+    dim3 grid(n2 / S, limbs, polys);
+    ProfScope ps(s, "ntt_fwd_cols");
+    if (in_kind == kInU64)
+        k_fwd_cols<LOGR, kInU64><<<grid, 256, 0, s>>>(in, 8, out, c.logn, pm, c.d_primes);
+    else if (in_kind == kInI32)
+        k_fwd_cols<LOGR, kInI32><<<grid, 256, 0, s>>>(in, 4, out, c.logn, pm, c.d_primes);
+    else
+        k_fwd_cols<LOGR, kInF64><<<grid, 256, 0, s>>>(in, 8, out, c.logn, pm, c.d_primes);
+    RCK_LAUNCH_CHECK();
+}
+
+template <int LOGR>
+static void fwd_rows_t(const Context& c, Batch<double> d, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    constexpr int S = 256 / ((1 << LOGR) / 16);
+    const int n1 = 1 << (c.logn - LOGR);
+    ProfScope ps(s, "ntt_fwd_rows");
+    k_fwd_rows<LOGR><<<dim3(n1 / S, limbs, polys), 256, 0, s>>>(d, c.logn, pm, c.d_primes);
+    RCK_LAUNCH_CHECK();
+}
+
+template <int LOGR>
+static void inv_rows_t(const Context& c, Batch<double> d, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    constexpr int S = 256 / ((1 << LOGR) / 16);
+    const int n1 = 1 << (c.logn - LOGR);
+    ProfScope ps(s, "ntt_inv_rows");
+    k_inv_rows<LOGR><<<dim3(n1 / S, limbs, polys), 256, 0, s>>>(d, c.logn, pm, c.d_primes);
+    RCK_LAUNCH_CHECK();
+}
+
+template <int LOGR>
+static void inv_cols_t(const Context& c, Batch<const double> in, int out_kind, Batch<void> out, int limbs, int polys,
+                       PrimeMap pm, hipStream_t s) {
+    constexpr int S = 256 / ((1 << LOGR) / 16);
+    const int n2 = 1 << (c.logn - LOGR);
+    dim3 grid(n2 / S, limbs, polys);
+    ProfScope ps(s, "ntt_inv_cols");
+    if (out_kind == kOutU64)
+        k_inv_cols<LOGR, kOutU64><<<grid, 256, 0, s>>>(in, out, 8, c.logn, pm, c.d_primes);
+    else
+        k_inv_cols<LOGR, kOutF64><<<grid, 256, 0, s>>>(in, out, 8, c.logn, pm, c.d_primes);
+    RCK_LAUNCH_CHECK();
+}
+
+static bool small_ring(const Context& c) { return c.logn <= 11; }
+
+void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
+                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    check_dims(c, limbs, polys);
+    RCK_REQUIRE(!small_ring(c), "two-pass NTT needs N >= 2^12");
+    Batch<const void> bi{in, in_ls, in_ps};
+    Batch<double> bo{out, out_ls, out_ps};
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    switch (n1) {
+        case 6: fwd_cols_t<6>(c, in_kind, bi, bo, limbs, polys, pm, s); break;
+        case 7: fwd_cols_t<7>(c, in_kind, bi, bo, limbs, polys, pm, s); break;
+        case 8: fwd_cols_t<8>(c, in_kind, bi, bo, limbs, polys, pm, s); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+}
+
+void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
+                     hipStream_t s) {
+    check_dims(c, limbs, polys);
+    Batch<double> b{data, ls, ps};
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    switch (n2) {
+        case 6: fwd_rows_t<6>(c, b, limbs, polys, pm, s); break;
+        case 7: fwd_rows_t<7>(c, b, limbs, polys, pm, s); break;
+        case 8: fwd_rows_t<8>(c, b, limbs, polys, pm, s); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+}
+
+void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
+                     hipStream_t s) {
+    check_dims(c, limbs, polys);
+    Batch<double> b{data, ls, ps};
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    switch (n2) {
+        case 6: inv_rows_t<6>(c, b, limbs, polys, pm, s); break;
+        case 7: inv_rows_t<7>(c, b, limbs, polys, pm, s); break;
+        case 8: inv_rows_t<8>(c, b, limbs, polys, pm, s); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+}
+
+void launch_inv_cols(const Context& c, const double* in, long long in_ls, long long in_ps, int out_kind, void* out,
+                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    check_dims(c, limbs, polys);
+    Batch<const double> bi{in, in_ls, in_ps};
+    Batch<void> bo{out, out_ls, out_ps};
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    switch (n1) {
+        case 6: inv_cols_t<6>(c, bi, out_kind, bo, limbs, polys, pm, s); break;
+        case 7: inv_cols_t<7>(c, bi, out_kind, bo, limbs, polys, pm, s); break;
+        case 8: inv_cols_t<8>(c, bi, out_kind, bo, limbs, polys, pm, s); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+}
+
+void launch_fwd(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
+                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    check_dims(c, limbs, polys);
+    if (small_ring(c)) {
+        Batch<const void> bi{in, in_ls, in_ps};
+        Batch<double> bo{out, out_ls, out_ps};
+        dim3 grid(1, limbs, polys);
+        ProfScope ps(s, "ntt_small_fwd");
+        if (in_kind == kInU64)
+            k_small_fwd<kInU64><<<grid, 256, 0, s>>>(bi, 8, bo, c.logn, pm, c.d_primes);
+        else if (in_kind == kInI32)
+            k_small_fwd<kInI32><<<grid, 256, 0, s>>>(bi, 4, bo, c.logn, pm, c.d_primes);
+        else
+            k_small_fwd<kInF64><<<grid, 256, 0, s>>>(bi, 8, bo, c.logn, pm, c.d_primes);
+        RCK_LAUNCH_CHECK();
+        return;
+    }
+    launch_fwd_cols(c, in_kind, in, in_ls, in_ps, out, out_ls, out_ps, limbs, polys, pm, s);
+    launch_fwd_rows(c, out, out_ls, out_ps, limbs, polys, pm, s);
+}
+
+void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, int out_kind, void* out,
+                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    check_dims(c, limbs, polys);
+    if (small_ring(c)) {
+        Batch<const double> bi{in, in_ls, in_ps};
+        Batch<void> bo{out, out_ls, out_ps};
+        dim3 grid(1, limbs, polys);
+        ProfScope ps(s, "ntt_small_inv");
+        if (out_kind == kOutU64)
+            k_small_inv<kOutU64><<<grid, 256, 0, s>>>(bi, bo, 8, c.logn, pm, c.d_primes);
+        else
+            k_small_inv<kOutF64><<<grid, 256, 0, s>>>(bi, bo, 8, c.logn, pm, c.d_primes);
+        RCK_LAUNCH_CHECK();
+        return;
+    }
+    launch_inv_rows(c, in, in_ls, in_ps, limbs, polys, pm, s);
+    launch_inv_cols(c, in, in_ls, in_ps, out_kind, out, out_ls, out_ps, limbs, polys, pm, s);
+}
+
+}  // namespace rck

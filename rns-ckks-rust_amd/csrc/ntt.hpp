@@ -1,0 +1,178 @@
+// Negacyclic NTT over every RNS limb, as LDS-staged radix-16 register passes.
+//
+// Transform: the merged (psi-twisted) Cooley-Tukey NTT with bit-reversed twiddles
+//   fw[k] = psi^{bitrev_logN(k)};  output slot i holds a(psi^{2 bitrev(i) + 1}).
+// A pointwise product of two such transforms, followed by the Gentleman-Sande inverse with
+// iv[k] = psi^{-bitrev(k)} and N^{-1}, is the negacyclic product in Z_q[X]/(X^N + 1) — the value
+// ModularPolynomial::multiply_ntt (src/poly/mod.rs:134-161) computes with a psi twist, a
+// cyclic NTT, pointwise product, inverse NTT and an untwist.  The product is unique, so the
+// two agree exactly.
+//
+// Layout: N = N1 * N2 split into two passes over a limb held limb-major in HBM.
+//   column pass: the first log2(N1) stages are N2 independent N1-point transforms on the
+//                columns x = r + N2*c (consecutive r are adjacent: coalesced loads);
+//   row pass:    the remaining log2(N2) stages are N1 independent N2-point transforms on the
+//                contiguous rows b*N2 .. b*N2+N2-1, with twiddles fw[(N1 + b) << s + i].
+// Each pass keeps 16 elements per thread in registers (radix-16: 4 stages per register phase)
+// and exchanges once through LDS between its two phases.  256 threads per workgroup; a
+// workgroup owns 4096 elements = 32 KiB of LDS.
+#pragma once
+#include "context.hpp"
+#include "modarith.hpp"
+
+namespace rck {
+
+// Prime index of limb l of an operand with nq Q-limbs: Q primes first, then the aux primes.
+__device__ __forceinline__ int prime_of(int l, int nq, int qmax) { return l < nq ? l : qmax + (l - nq); }
+
+// Cooley-Tukey butterfly (forward): (u, v) -> (u + v w, u - v w)
+__device__ __forceinline__ void ct_bfly(double& a, double& b, double w, double q, double qinv) {
+    double t = mulmod(b, w, q, qinv);
+    double u = a;
+    a = u + t;
+    b = u - t;
+}
+
+// Gentleman-Sande butterfly (inverse): (u, v) -> (u + v, (u - v) w)
+__device__ __forceinline__ void gs_bfly(double& a, double& b, double w, double q, double qinv) {
+    double u = a, v = b;
+    a = u + v;
+    b = mulmod(u - v, w, q, qinv);
+}
+
+// LDS index of a row tile with one pad word per 16 (conflict-free for the x = 16 tau + e reads).
+template <int R>
+__device__ __forceinline__ int row_lds(int row, int x) {
+    return row * (R + R / 16 + 1) + x + (x >> 4);
+}
+
+template <int E>
+__device__ __forceinline__ void reduce_all(double (&v)[E], double q, double qinv) {
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = reduce(v[e], q, qinv);
+}
+
+// Phase A of a 2^LOGR-point sub-transform: the thread holds x = tau + (R/16) e, e < 16, and
+// runs stages 0..3 (distance bits LOGR-1 .. LOGR-4).  Twiddle of stage s, group i:
+// tbl[(base << s) + i].  Inputs |v| <= 0.51q; outputs reduced.
+__device__ __forceinline__ void fwd_phase_a(double (&v)[16], const double* __restrict__ tbl, int base, double q,
+                                            double qinv) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int half = 8 >> s;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if ((e & half) == 0) {
+                double w = tbl[(base << s) + (e >> (4 - s))];
+                ct_bfly(v[e], v[e + half], w, q, qinv);
+            }
+        }
+    }
+    reduce_all(v, q, qinv);
+}
+
+// Phase B: the thread holds x = 16 tau + e and runs stages 4..LOGR-1.
+template <int LOGR>
+__device__ __forceinline__ void fwd_phase_b(double (&v)[16], const double* __restrict__ tbl, int base, int tau,
+                                            double q, double qinv) {
+#pragma unroll
+    for (int s = 4; s < LOGR; s++) {
+        const int half = (1 << LOGR) >> (s + 1);
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if ((e & half) == 0) {
+                double w = tbl[(base << s) + ((16 * tau + e) >> (LOGR - s))];
+                ct_bfly(v[e], v[e + half], w, q, qinv);
+            }
+        }
+    }
+    reduce_all(v, q, qinv);
+}
+
+// Inverse phase B: stages LOGR-1 .. 4 (reduce every two stages: the sums double each stage).
+template <int LOGR>
+__device__ __forceinline__ void inv_phase_b(double (&v)[16], const double* __restrict__ tbl, int base, int tau,
+                                            double q, double qinv) {
+    int cnt = 0;
+#pragma unroll
+    for (int s = LOGR - 1; s >= 4; s--) {
+        const int half = (1 << LOGR) >> (s + 1);
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if ((e & half) == 0) {
+                double w = tbl[(base << s) + ((16 * tau + e) >> (LOGR - s))];
+                gs_bfly(v[e], v[e + half], w, q, qinv);
+            }
+        }
+        if (++cnt == 2) {
+            reduce_all(v, q, qinv);
+            cnt = 0;
+        }
+    }
+    if (cnt) reduce_all(v, q, qinv);
+}
+
+// Inverse phase A: stages 3 .. 0 on x = tau + (R/16) e.
+__device__ __forceinline__ void inv_phase_a(double (&v)[16], const double* __restrict__ tbl, int base, double q,
+                                            double qinv) {
+#pragma unroll
+    for (int s = 3; s >= 0; s--) {
+        const int half = 8 >> s;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if ((e & half) == 0) {
+                double w = tbl[(base << s) + (e >> (4 - s))];
+                gs_bfly(v[e], v[e + half], w, q, qinv);
+            }
+        }
+        if (s == 2 || s == 0) reduce_all(v, q, qinv);
+    }
+}
+
+// Operand addressing for a batch of limbs: element (z, y, idx) at base + z*ps + y*ls + idx.
+template <typename T>
+struct Batch {
+    T* base;
+    long long ls;  // limb stride (elements)
+    long long ps;  // poly stride (elements)
+    __device__ __forceinline__ T* at(int z, int y) const { return base + (long long)z * ps + (long long)y * ls; }
+};
+
+struct PrimeMap {
+    int nq;    // Q-limbs in the operand
+    int qmax;  // Q primes in the context (aux primes start there)
+    int yoff;  // limb index of blockIdx.y == 0
+    __device__ __forceinline__ int operator()(int y) const { return prime_of(y + yoff, nq, qmax); }
+};
+
+enum InKind { kInU64 = 0, kInI32 = 1, kInF64 = 2 };
+enum OutKind { kOutU64 = 0, kOutF64 = 1 };
+
+// Sizes of the two passes for a ring degree 2^logn (2^12 .. 2^16).
+inline void split_logn(int logn, int* n1, int* n2) {
+    *n1 = logn / 2;
+    *n2 = logn - *n1;
+}
+
+// ---------------------------------------------------------------------------------------
+// Host launchers (ntt.hip)
+// ---------------------------------------------------------------------------------------
+struct Ctx;  // fwd decl not used
+
+// forward transform of a batch (coefficient form -> NTT form, doubles, reduced)
+void launch_fwd(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
+                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+// forward row pass only / column pass only (used by fused pipelines)
+void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
+                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
+                     hipStream_t s);
+// inverse transform (NTT form doubles -> coefficient form u64 canonical, or reduced doubles)
+void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, int out_kind, void* out,
+                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
+                     hipStream_t s);
+void launch_inv_cols(const Context& c, const double* in, long long in_ls, long long in_ps, int out_kind, void* out,
+                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+
+}  // namespace rck

@@ -1,0 +1,40 @@
+// Launchers of the elementwise kernels (ops.hip) and the keyswitch pipeline (keyswitch.hip).
+#pragma once
+#include "ntt.hpp"
+
+namespace rck {
+
+void launch_addsub(const Context& c, Batch<const uint64_t> a, Batch<const uint64_t> b, Batch<uint64_t> out, int limbs,
+                   int polys, int sub, PrimeMap pm, hipStream_t s);
+void launch_galois(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs, int polys,
+                   uint64_t galois, PrimeMap pm, hipStream_t s);
+void launch_rescale(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs_in, int polys,
+                    hipStream_t s);
+void launch_mul_add_u64(const Context& c, Batch<const uint64_t> a, Batch<const uint64_t> b, Batch<const uint64_t> cc,
+                        bool has_c, Batch<uint64_t> out, int limbs, int polys, PrimeMap pm, hipStream_t s);
+void launch_mul_f64(const Context& c, Batch<const double> a, Batch<const double> b, Batch<double> out, int limbs,
+                    int polys, PrimeMap pm, hipStream_t s);
+void launch_tensor(const Context& c, const double* x, double* d, int limbs, PrimeMap pm, hipStream_t s);
+void launch_twist_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
+                     hipStream_t s);
+void launch_brv_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
+                    hipStream_t s);
+void launch_brv_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
+                   hipStream_t s);
+void launch_twist_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
+                      hipStream_t s);
+
+// keyswitch pipeline
+// Balanced base-2^30 decomposition of a coefficient-form polynomial with nq Q-limbs into
+// `levels` signed digits per coefficient: digits[i * N + c] (keyswitching.rs:196-236, 381-467).
+void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s);
+// acc_{0,1}[j] = sum_i NTT_j(digit_i) * key[i][{0,1}][j] for every eval prime j (Q-prefix nq, then
+// naux aux primes), NTT form.  key: [levels][2][nq + naux][N] NTT-form doubles.
+void launch_digit_mac(const Context& c, const int32_t* digits, int levels, const double* key, int nq, int naux,
+                      double* acc, hipStream_t s);
+// mod_down_eval_to_q (keyswitching.rs:278-322) on coefficient-form accumulators over Q-prefix
+// nq || P (2 polys, u64), optionally adding add0 / add1 (nq limbs each) to the two results.
+void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64_t* add0, const uint64_t* add1,
+                     uint64_t* out, long long out_ps, hipStream_t s);
+
+}  // namespace rck

@@ -1,0 +1,321 @@
+"""ctypes binding of librnsckks.so (include/rnsckks.h).
+
+Thin by design: device buffers, the context, keyswitching keys and one Python method per C
+entry point.  Every call goes straight to the HIP library; there is no CPU fallback — a
+missing or unloadable library raises at import of this module's first use.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librnsckks.so")
+_lib = None
+
+vp = ctypes.c_void_p
+u32, u64, i64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64
+u32p, u64p = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)
+
+EXPORTS = {
+    "rck_last_error": (ctypes.c_char_p, []),
+    "rck_version": (ctypes.c_int, []),
+    "rck_malloc": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
+    "rck_free": (ctypes.c_int, [vp]),
+    "rck_memcpy_htod": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    "rck_memcpy_dtoh": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    "rck_memcpy_dtod": (ctypes.c_int, [vp, vp, ctypes.c_size_t, vp]),
+    "rck_device_synchronize": (ctypes.c_int, []),
+    "rck_stream_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "rck_stream_destroy": (ctypes.c_int, [vp]),
+    "rck_stream_synchronize": (ctypes.c_int, [vp]),
+    "rck_event_create": (ctypes.c_int, [ctypes.POINTER(vp)]),
+    "rck_event_destroy": (ctypes.c_int, [vp]),
+    "rck_event_record": (ctypes.c_int, [vp, vp]),
+    "rck_event_synchronize": (ctypes.c_int, [vp]),
+    "rck_event_elapsed_ms": (ctypes.c_int, [vp, vp, ctypes.POINTER(ctypes.c_float)]),
+    "rck_context_create": (ctypes.c_int, [u32, u64p, u64p, u32, u64p, u64p, u32, u32, ctypes.POINTER(vp)]),
+    "rck_context_destroy": (ctypes.c_int, [vp]),
+    "rck_num_decomp_digits": (ctypes.c_int, [vp, u32, u32p]),
+    "rck_galois_element_for_rotation": (u64, [u32, i64]),
+    "rck_ntt_forward": (ctypes.c_int, [vp, vp, u32, u32, vp]),
+    "rck_ntt_inverse": (ctypes.c_int, [vp, vp, u32, u32, vp]),
+    "rck_poly_add": (ctypes.c_int, [vp, vp, vp, vp, u32, u32, vp]),
+    "rck_poly_sub": (ctypes.c_int, [vp, vp, vp, vp, u32, u32, vp]),
+    "rck_poly_multiply_ntt": (ctypes.c_int, [vp, vp, vp, vp, u32, u32, vp]),
+    "rck_poly_galois": (ctypes.c_int, [vp, vp, vp, u32, u32, u64, vp]),
+    "rck_poly_mul_add": (ctypes.c_int, [vp, vp, vp, vp, vp, u32, u32, vp]),
+    "rck_ct_add": (ctypes.c_int, [vp, vp, vp, u32, u32, u32, vp, u32p, vp]),
+    "rck_mult": (ctypes.c_int, [vp, vp, vp, u32, u32, u32, vp, u32p, vp]),
+    "rck_mult_plain": (ctypes.c_int, [vp, vp, vp, u32, u32, vp, u32p, vp]),
+    "rck_rescale": (ctypes.c_int, [vp, vp, u32, u32, vp, u32p, vp]),
+    "rck_rescale_quadratic": (ctypes.c_int, [vp, vp, u32, u32, vp, u32p, vp]),
+    "rck_ksk_create": (ctypes.c_int, [vp, vp, u32, u32, u32, u64, ctypes.POINTER(vp)]),
+    "rck_ksk_create_device": (ctypes.c_int, [vp, vp, u32, u32, u32, u64, ctypes.POINTER(vp)]),
+    "rck_ksk_destroy": (ctypes.c_int, [vp]),
+    "rck_keyswitch": (ctypes.c_int, [vp, vp, u32, vp, vp, u32p, vp]),
+    "rck_relinearize": (ctypes.c_int, [vp, vp, u32, u32, vp, vp, u32p, vp]),
+    "rck_mult_relinearized": (ctypes.c_int, [vp, vp, vp, u32, u32, u32, vp, vp, u32p, vp]),
+    "rck_ct_galois": (ctypes.c_int, [vp, vp, u32, u64, vp, vp]),
+    "rck_rotate": (ctypes.c_int, [vp, vp, u32, u32, vp, vp, u32p, vp]),
+    "rck_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "rck_profile_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+}
+
+
+class RckError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+        self.msg = msg
+
+
+def lib():
+    """Load librnsckks.so (raises if it was not built: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C rns-ckks-rust_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RckError(rc, lib().rck_last_error().decode())
+
+
+def _u64arr(x):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+    return a, a.ctypes.data_as(u64p)
+
+
+# ------------------------------------------------------------------ device memory
+class DeviceBuffer:
+    """A device allocation with a numpy shape/dtype for transfers."""
+
+    def __init__(self, shape, dtype=np.uint64):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = vp()
+        _check(lib().rck_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.shape, arr.dtype)
+        _check(lib().rck_memcpy_htod(b.ptr, arr.ctypes.data, b.nbytes))
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        _check(lib().rck_memcpy_htod(self.ptr, arr.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        synchronize()
+        _check(lib().rck_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def offset(self, elems):
+        return self.ptr + int(elems) * self.dtype.itemsize
+
+    def free(self):
+        if self.ptr:
+            _check(lib().rck_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.rck_free(self.ptr)
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(lib().rck_device_synchronize())
+
+
+class Event:
+    def __init__(self):
+        p = vp()
+        _check(lib().rck_event_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        _check(lib().rck_event_record(self.ptr, stream))
+
+    def synchronize(self):
+        _check(lib().rck_event_synchronize(self.ptr))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        _check(lib().rck_event_elapsed_ms(self.ptr, end.ptr, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if _lib is not None:
+                _lib.rck_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def profile_enable(on=True):
+    _check(lib().rck_profile_enable(1 if on else 0))
+
+
+def profile_report():
+    """{kernel: (launches, total_ms)} since the previous report (synchronizes the device)."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(lib().rck_profile_report(buf, len(buf)))
+    return {k: (int(v[0]), float(v[1])) for k, v in json.loads(buf.value.decode()).items()}
+
+
+def galois_element_for_rotation(poly_degree, steps):
+    return int(lib().rck_galois_element_for_rotation(poly_degree, steps))
+
+
+# ------------------------------------------------------------------ context
+def _p(x):
+    """device pointer of a DeviceBuffer / int / None"""
+    if x is None:
+        return None
+    return x.ptr if isinstance(x, DeviceBuffer) else int(x)
+
+
+class Ksk:
+    def __init__(self, ptr, digits, level, aux, galois):
+        self.ptr, self.digits, self.level, self.aux, self.galois = ptr, digits, level, aux, galois
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.rck_ksk_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Context:
+    """rck_context: RnsCkksContext's hot-path entry points on device buffers."""
+
+    def __init__(self, params=None, *, poly_degree=None, primes=None, roots=None, aux_primes=(), aux_roots=(),
+                 scale_bits=40):
+        if params is not None:
+            poly_degree, primes, roots = params.poly_degree, params.primes, params.roots
+            aux_primes, aux_roots, scale_bits = params.aux_primes, params.aux_roots, params.scale_bits
+        self.n = int(poly_degree)
+        self.primes = [int(x) for x in primes]
+        self.aux_primes = [int(x) for x in aux_primes]
+        qa, qp = _u64arr(self.primes)
+        ga, gp = _u64arr(roots)
+        pa, pp = _u64arr(self.aux_primes if self.aux_primes else [0])
+        ha, hp = _u64arr(list(aux_roots) if aux_primes else [0])
+        out = vp()
+        _check(lib().rck_context_create(self.n, qp, gp, len(self.primes), pp, hp, len(self.aux_primes), scale_bits,
+                                        ctypes.byref(out)))
+        self.ptr = out.value
+        self.scale_bits = scale_bits
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and _lib is not None:
+                _lib.rck_context_destroy(self.ptr)
+        except Exception:
+            pass
+
+    @property
+    def L(self):
+        return len(self.primes)
+
+    @property
+    def K(self):
+        return len(self.aux_primes)
+
+    def num_decomp_digits(self, limbs):
+        d = ctypes.c_uint32()
+        _check(lib().rck_num_decomp_digits(self.ptr, limbs, ctypes.byref(d)))
+        return d.value
+
+    # L0 / L1 / L2
+    def ntt_forward(self, data, nq, np_=0, stream=None):
+        _check(lib().rck_ntt_forward(self.ptr, _p(data), nq, np_, stream))
+
+    def ntt_inverse(self, data, nq, np_=0, stream=None):
+        _check(lib().rck_ntt_inverse(self.ptr, _p(data), nq, np_, stream))
+
+    def poly_add(self, a, b, out, nq, np_=0, stream=None):
+        _check(lib().rck_poly_add(self.ptr, _p(a), _p(b), _p(out), nq, np_, stream))
+
+    def poly_sub(self, a, b, out, nq, np_=0, stream=None):
+        _check(lib().rck_poly_sub(self.ptr, _p(a), _p(b), _p(out), nq, np_, stream))
+
+    def poly_multiply_ntt(self, a, b, out, nq, np_=0, stream=None):
+        _check(lib().rck_poly_multiply_ntt(self.ptr, _p(a), _p(b), _p(out), nq, np_, stream))
+
+    def poly_galois(self, a, out, nq, galois, np_=0, stream=None):
+        _check(lib().rck_poly_galois(self.ptr, _p(a), _p(out), nq, np_, galois, stream))
+
+    def poly_mul_add(self, a, b, c, out, nq, np_=0, stream=None):
+        _check(lib().rck_poly_mul_add(self.ptr, _p(a), _p(b), _p(c), _p(out), nq, np_, stream))
+
+    # L3
+    def _scaled(self, fn, *args):
+        s = ctypes.c_uint32()
+        _check(fn(*args, ctypes.byref(s), None))
+        return s.value
+
+    def ct_add(self, lhs, rhs, limbs, ls, rs, out):
+        return self._scaled(lib().rck_ct_add, self.ptr, _p(lhs), _p(rhs), limbs, ls, rs, _p(out))
+
+    def mult(self, lhs, rhs, limbs, ls, rs, out):
+        return self._scaled(lib().rck_mult, self.ptr, _p(lhs), _p(rhs), limbs, ls, rs, _p(out))
+
+    def mult_plain(self, ct, pt, limbs, scale, out):
+        return self._scaled(lib().rck_mult_plain, self.ptr, _p(ct), _p(pt), limbs, scale, _p(out))
+
+    def rescale(self, ct, limbs, scale, out):
+        return self._scaled(lib().rck_rescale, self.ptr, _p(ct), limbs, scale, _p(out))
+
+    def rescale_quadratic(self, quad, limbs, scale, out):
+        return self._scaled(lib().rck_rescale_quadratic, self.ptr, _p(quad), limbs, scale, _p(out))
+
+    def ksk_create(self, key, level, aux, galois=0):
+        """key: numpy uint64 [digits][2][level+1+aux][N] (host) or a DeviceBuffer of that shape."""
+        out = vp()
+        if isinstance(key, DeviceBuffer):
+            digits = key.shape[0]
+            _check(lib().rck_ksk_create_device(self.ptr, key.ptr, digits, level, aux, galois, ctypes.byref(out)))
+        else:
+            key = np.ascontiguousarray(key, dtype=np.uint64)
+            digits = key.shape[0]
+            _check(lib().rck_ksk_create(self.ptr, key.ctypes.data, digits, level, aux, galois, ctypes.byref(out)))
+        return Ksk(out.value, digits, level, aux, galois)
+
+    def keyswitch(self, poly, limbs, ksk, out):
+        return self._scaled(lib().rck_keyswitch, self.ptr, _p(poly), limbs, ksk.ptr, _p(out))
+
+    def relinearize(self, quad, limbs, scale, ksk, out):
+        return self._scaled(lib().rck_relinearize, self.ptr, _p(quad), limbs, scale, ksk.ptr, _p(out))
+
+    def mult_relinearized(self, lhs, rhs, limbs, ls, rs, ksk, out, stream=None):
+        s = ctypes.c_uint32()
+        _check(lib().rck_mult_relinearized(self.ptr, _p(lhs), _p(rhs), limbs, ls, rs, ksk.ptr, _p(out),
+                                           ctypes.byref(s), stream))
+        return s.value
+
+    def ct_galois(self, ct, limbs, galois, out):
+        _check(lib().rck_ct_galois(self.ptr, _p(ct), limbs, galois, _p(out), None))
+
+    def rotate(self, ct, limbs, scale, ksk, out):
+        return self._scaled(lib().rck_rotate, self.ptr, _p(ct), limbs, scale, ksk.ptr, _p(out))

@@ -1,0 +1,58 @@
+"""CPU-side checks of the C ABI library: it loads and exports every symbol include/*.h declares
+(no compute calls: there is no GPU here), plus host-only logic of the package."""
+import glob
+import os
+import re
+
+import pytest
+
+from rnsckks import capi
+from rnsckks import params as PR
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(rck_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("librnsckks.so not built")
+    lib = capi.lib()
+    names = declared_symbols()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(capi.EXPORTS) == names
+
+
+def test_galois_element_for_rotation_host_only():
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("librnsckks.so not built")
+    # rns/tests.rs:314-321
+    assert [capi.galois_element_for_rotation(8, s) for s in (0, 1, 2, -1)] == [1, 5, 9, 13]
+    n = 1 << 15
+    for s in (1, -1, 4096, -8192, 13):
+        g = capi.galois_element_for_rotation(n, s)
+        want = pow(5, s, 2 * n) if s >= 0 else pow(pow(5, -1, 2 * n), -s, 2 * n)
+        assert g == want
+
+
+def test_prime_chains_match_golden():
+    import json
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "primes.json")))
+    for n in (1 << 12, 1 << 14, 1 << 15, 1 << 16):
+        below = PR.primes_below(50, n, 28)
+        above = PR.primes_above(50, n, 28)
+        assert below == [p for p, _ in gold["below_2^50"][str(n)]]
+        assert above == [p for p, _ in gold["above_2^50"][str(n)]]
+        assert [PR.smallest_primitive_root(q) for q in below[:6]] == [g for _, g in gold["below_2^50"][str(n)][:6]]
+    p = PR.config_per_op_gpu()
+    assert len(p.primes) == 24 and len(p.aux_primes) == 2
+    assert p.primes[:3] == [1125899903827969, 1125899902124033, 1125899887312897]
