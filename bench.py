@@ -33,12 +33,31 @@ def uniform(moduli, n, rs, polys):
     return out
 
 
+def time_it(G, fn, steps):
+    e0, e1 = G.Event(), G.Event()
+    G.synchronize()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    G.synchronize()
+    return e0.elapsed_ms(e1)
+
+
+def mac_bytes(n, D, G_, B, dsplit):
+    """Algorithmic HBM bytes of one digit-MAC launch: the column-pass output of B*D digits for its
+    G_ eval limbs, the key rows (2 per digit per limb, read once) and the partial sums written."""
+    return (B * D * G_ + 2 * D * G_ + dsplit * B * 2 * G_) * n * 8
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="per-op", choices=["per-op"])
+    ap.add_argument("--steps", type=int, default=10, help="timed batches (batched) / ops (per-op)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="batched", choices=["batched", "per-op"])
+    ap.add_argument("--batch", type=int, default=16, help="ciphertext pairs per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
@@ -52,47 +71,64 @@ def main():
 
     p = PR.config_per_op_gpu()
     n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
-    log(f"[bench] context N={n} L={L} K={K}")
+    B = args.batch if args.mode == "batched" else 1
+    log(f"[bench] context N={n} L={L} K={K} mode={args.mode} B={B}")
     ctx = G.Context(p)
     D = ctx.num_decomp_digits(L)
     rs = np.random.default_rng(0)
-    lhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2))
-    rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2))
-    log(f"[bench] generating relin key: {D} digits x 2 x {L + K} limbs")
+    P = L * n
+    lhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B))
+    rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B))
+    log(f"[bench] relin key: {D} digits x 2 x {L + K} limbs")
     key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
     ksk = ctx.ksk_create(key, L - 1, K)
     del key
     quad = G.DeviceBuffer((3, L, n))
-    out = G.DeviceBuffer((2, L, n))
+    out = G.DeviceBuffer((B, 2, L, n))
 
-    def step():
+    def per_op_step():
         ctx.mult(lhs, rhs, L, 40, 40, quad)
         ctx.relinearize(quad, L, 80, ksk, out)
 
+    # per-op path (always measured: the batched form is reported against it)
     for _ in range(args.warmup):
-        step()
-    G.synchronize()
-    log(f"[bench] warmup done; timing {args.steps} steps")
-    e0, e1 = G.Event(), G.Event()
-    G.synchronize()
-    t0 = time.perf_counter()
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    e1.synchronize()
-    G.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_ms(e1)
-    ms_per_step = ms / args.steps
-    value = 1000.0 / ms_per_step
-    log(f"[bench] {ms_per_step:.3f} ms/op ({wall / args.steps * 1e3:.3f} ms wall)")
+        per_op_step()
+    per_op_steps = args.steps if args.mode == "per-op" else max(4, args.steps // 2)
+    per_op_ms = time_it(G, per_op_step, per_op_steps) / per_op_steps
+    per_op_value = 1000.0 / per_op_ms
+    log(f"[bench] per-op: {per_op_ms:.3f} ms/op = {per_op_value:.1f} ops/s")
 
-    # per-kernel timing over a few more steps of the same workload (HIP events on the launch stream)
+    report = None
+    if args.mode == "batched":
+        bt = G.Batch(ctx)
+        for b in range(B):
+            bt.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ksk, out.offset(b * 2 * P))
+        t = time.perf_counter()
+        bt.compile(tune=True)
+        report = bt.report()
+        log(f"[bench] batch compiled+tuned in {time.perf_counter() - t:.1f} s: {report}")
+
+        def step():
+            bt.run(None)
+
+        for _ in range(args.warmup):
+            step()
+        ms = time_it(G, step, args.steps)
+        ms_per_step = ms / args.steps
+        value = B * 1000.0 / ms_per_step
+        log(f"[bench] batched: {ms_per_step:.3f} ms per batch of {B} = {value:.1f} ops/s "
+            f"({value / per_op_value:.2f}x per-op)")
+        prof_fn = bt.run_eager
+    else:
+        ms_per_step, value = per_op_ms, per_op_value
+        prof_fn = per_op_step
+
+    # per-kernel timing: the same launches run eagerly with HIP events around each launch, on the
+    # stream each kernel is launched on
     G.profile_enable(True)
-    prof_steps = 3
+    prof_steps = 2
     for _ in range(prof_steps):
-        step()
+        prof_fn()
     prof = G.profile_report()
     G.profile_enable(False)
     top = sorted(prof.items(), key=lambda kv: -kv[1][1])
@@ -100,14 +136,15 @@ def main():
         log(f"[bench]   {k:16s} launches={cnt:5d} total={tot:9.3f} ms avg={tot / cnt * 1e3:9.2f} us")
     name, (cnt, tot) = top[0]
     avg_s = tot / cnt / 1e3
-    if name == "digit_mac_rows":
-        alg = (3 * D + 2) * n * 8
-    elif name in ("ntt_fwd_cols", "ntt_fwd_rows", "ntt_inv_rows", "ntt_inv_cols"):
-        alg = None
-    else:
-        alg = None
     roofline = None
-    if alg is not None:
+    if name == "digit_mac":
+        launches = max(1, cnt // prof_steps)
+        G_ = -(-(L + K) // launches)  # eval limbs per launch
+        if report:
+            ds = report["groups"][0]["digit_splits"]
+        else:  # the library's automatic shape for one ciphertext: ~2048 MAC workgroups (16 row tiles/limb)
+            ds = max(1, min(4, -(-2048 // (16 * (L + K) * B))))
+        alg = mac_bytes(n, D, G_, B, ds)
         achieved = alg / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -122,8 +159,10 @@ def main():
         "value": round(value, 2), "unit": "ops/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64 residues (FP64-exact modular arithmetic)", "data": "synthetic (seeded uniform residues)",
-        "config": {"workload": "per-op mult+relinearize, N=2^16, L=24 primes < 2^50, K=2 aux, D=40 digits",
-                   "mode": args.mode, "poly_degree": n, "limbs": L, "aux": K, "digits": D},
+        "config": {"workload": f"{args.mode} mult+relinearize, N=2^16, L=24 primes < 2^50, K=2 aux, D={D} digits"
+                               + (f", {B} ciphertext pairs per step sharing one relin key" if B > 1 else ""),
+                   "mode": args.mode, "batch": B, "per_op_ops_s": round(per_op_value, 2),
+                   "batched_vs_per_op": round(value / per_op_value, 3)},
         "roofline": roofline, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in top},
     }

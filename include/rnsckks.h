@@ -146,6 +146,35 @@ int rck_ct_galois(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint64_t
 int rck_rotate(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale, const rck_ksk* key,
                uint64_t* out, uint32_t* out_scale, void* stream);
 
+/* ---- batched executor (no single counterpart in the crate: it runs many of the calls above) ----
+ * Independent operations at any levels, of mixed kinds, are planned into grouped launches
+ * (same kind, level and key -> one group whose kernels process all its ciphertexts together and
+ * stream the key once) and captured in one hipGraph.  Results are bit-identical to the per-op
+ * calls.  Operands stay caller-owned device buffers; ops of one batch must not alias each other's
+ * outputs.  Output scales follow the per-op rules (mult: sum of scales, relinearize/rotate: kept,
+ * mult_plain: doubled, rescale: minus floor(log2 q_last)). */
+#define RCK_OP_MULT_RELIN 1         /* out = relinearize(mult(in0, in1), key)        (context/ops.rs:63-122) */
+#define RCK_OP_ROTATE 2             /* out = rotate(in0, key) with the key's galois  (context/rotation.rs:104-127) */
+#define RCK_OP_MULT_PLAIN 3         /* out = mult_plain(in0, in1)                    (context/ops.rs:20-30) */
+#define RCK_OP_MULT_RELIN_RESCALE 4 /* out = mult_relinearized(in0, in1, key)        (context/ops.rs:124-133) */
+typedef struct rck_batch rck_batch;
+int rck_batch_create(rck_context* ctx, rck_batch** out);
+/* limbs: the operands' Q-limb count; in1 unused for RCK_OP_ROTATE; key unused for RCK_OP_MULT_PLAIN */
+int rck_batch_add(rck_batch* batch, int op, uint32_t limbs, const uint64_t* in0, const uint64_t* in1,
+                  const rck_ksk* key, uint64_t* out);
+/* Plans the groups, optionally autotunes each group's digit-MAC launch shape (tune != 0) by timing
+ * candidates on the group's operands, runs the batch once eagerly and captures it in a hipGraph. */
+int rck_batch_compile(rck_batch* batch, int tune);
+/* Launches the captured graph on `stream` (NULL: the batch's own stream). */
+int rck_batch_run(rck_batch* batch, void* stream);
+/* Runs the planned launches directly on the batch's stream, without the graph (same results;
+ * lets per-kernel timing see the individual launches). */
+int rck_batch_run_eager(rck_batch* batch);
+int rck_batch_synchronize(rck_batch* batch);
+/* JSON: groups, their tuned launch shapes and timings. */
+int rck_batch_report(rck_batch* batch, char* buf, size_t len);
+int rck_batch_destroy(rck_batch* batch);
+
 /* ---- diagnostics (no counterpart in the crate) ----
  * Per-kernel timing: while enabled, every kernel launch is bracketed by HIP events on the stream it
  * is launched on.  rck_profile_report synchronizes the device and writes a JSON object

@@ -1,0 +1,349 @@
+// Batched executor: many independent ciphertext operations (at uneven levels, with mixed
+// operation kinds) planned into grouped launches and captured in one hipGraph.
+//
+// Grouping: ops with the same (kind, limbs, key) form a group; a group runs the crate's
+// operation for all of its B ciphertexts in the same launches (mult_batch / keyswitch_batch),
+// so every kernel sees B ciphertexts at once and the key stream of a keyswitch is read from HBM
+// once per group (the digit-MAC grid places the B users of a key tile on one XCD back to back).
+// The arithmetic is exactly the per-op path's: the same kernels on the same operands.
+//
+// Tuning: the digit-MAC launch shape (eval limbs per group, digit splits) is chosen per
+// (N, limbs, digits, B) by timing the candidates on the group's own operands (autotune), and
+// cached in the context's tuning table.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <sstream>
+
+#include "../../include/rnsckks.h"
+#include "capi_types.hpp"
+
+using namespace rck;
+
+namespace {
+
+struct BatchOp {
+    int kind;
+    uint32_t limbs;
+    const uint64_t* a;
+    const uint64_t* b;
+    const rck_ksk* key;
+    uint64_t* out;
+};
+
+struct BatchGroup {
+    int kind;
+    uint32_t limbs;
+    const rck_ksk* key;
+    std::vector<int> ops;
+    KsShape shape;
+    float tuned_ms = 0;
+};
+
+enum { kSlotLhs = 10, kSlotRhs = 11, kSlotQuad = 12, kSlotTmp = 13 };
+
+// (logn, limbs, eval limbs, digits, B) -> tuned shape, shared by every batch of a process
+std::map<std::tuple<int, int, int, int, int>, std::pair<KsShape, float>> g_tuned;
+
+}  // namespace
+
+struct rck_batch {
+    rck_context* ctx = nullptr;
+    Context* sc = nullptr;  // scratch clone: own workspace, shared tables
+    hipStream_t stream = nullptr;
+    std::vector<BatchOp> ops;
+    std::vector<BatchGroup> groups;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool compiled = false;
+    double capture_ms = 0;
+};
+
+template <typename F>
+static int bguard(F&& f) {
+    try {
+        f();
+        return RCK_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return RCK_ERR_INTERNAL;
+    }
+}
+
+// Are ptr(op_0) .. ptr(op_{B-1}) equally spaced?  Returns the stride (elements) or -1.
+template <typename P>
+static long long common_stride(const std::vector<P>& ptrs) {
+    if (ptrs.size() < 2) return 0;
+    const long long st = (long long)(ptrs[1] - ptrs[0]);
+    if (st <= 0) return -1;
+    for (size_t i = 2; i < ptrs.size(); i++)
+        if ((long long)(ptrs[i] - ptrs[i - 1]) != st) return -1;
+    return st;
+}
+
+static void run_group(rck_batch* bt, const BatchGroup& g, hipStream_t s, KsShape shape) {
+    Context& c = *bt->sc;
+    const int B = (int)g.ops.size();
+    const int l = (int)g.limbs;
+    const long long n = c.n, P = (long long)l * n;
+    std::vector<const uint64_t*> as, bs;
+    std::vector<uint64_t*> outs;
+    for (int i : g.ops) {
+        as.push_back(bt->ops[i].a);
+        bs.push_back(bt->ops[i].b);
+        outs.push_back(bt->ops[i].out);
+    }
+    switch (g.kind) {
+        case RCK_OP_MULT_RELIN:
+        case RCK_OP_MULT_RELIN_RESCALE: {
+            const long long sa = common_stride(as), sb = common_stride(bs);
+            const uint64_t *lhs = as[0], *rhs = bs[0];
+            long long st = sa;
+            if (B > 1 && !(sa == sb && sa >= 2 * P)) {  // stage both operands contiguously
+                uint64_t* lb = static_cast<uint64_t*>(c.workspace(kSlotLhs, (size_t)B * 2 * P * 8));
+                uint64_t* rb = static_cast<uint64_t*>(c.workspace(kSlotRhs, (size_t)B * 2 * P * 8));
+                for (int i = 0; i < B; i++) {
+                    RCK_CHECK_HIP(hipMemcpyAsync(lb + i * 2 * P, as[i], 2 * P * 8, hipMemcpyDeviceToDevice, s));
+                    RCK_CHECK_HIP(hipMemcpyAsync(rb + i * 2 * P, bs[i], 2 * P * 8, hipMemcpyDeviceToDevice, s));
+                }
+                lhs = lb;
+                rhs = rb;
+                st = 2 * P;
+            }
+            uint64_t* quad = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * 3 * P * 8));
+            mult_batch(c, lhs, rhs, st, B, l, quad, 3 * P, s);
+            if (g.kind == RCK_OP_MULT_RELIN) {
+                keyswitch_batch(c, quad + 2 * P, 3 * P, B, l, g.key, quad, quad + P, 3 * P, outs, s, shape);
+            } else {
+                uint64_t* rel = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
+                std::vector<uint64_t*> rels;
+                for (int i = 0; i < B; i++) rels.push_back(rel + i * 2 * P);
+                keyswitch_batch(c, quad + 2 * P, 3 * P, B, l, g.key, quad, quad + P, 3 * P, rels, s, shape);
+                for (int i = 0; i < B; i++) rescale_impl(c, rels[i], l, 2, outs[i], s);
+            }
+            break;
+        }
+        case RCK_OP_ROTATE: {
+            uint64_t* rot = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
+            PrimeMap pm{l, (int)c.L, 0};
+            for (int i = 0; i < B; i++)
+                launch_galois(c, {as[i], n, P}, {rot + i * 2 * P, n, P}, l, 2, g.key->galois, pm, s);
+            keyswitch_batch(c, rot + P, 2 * P, B, l, g.key, rot, nullptr, 2 * P, outs, s, shape);
+            break;
+        }
+        case RCK_OP_MULT_PLAIN: {
+            PrimeMap pm{l, (int)c.L, 0};
+            for (int i = 0; i < B; i++) poly_mul(c, as[i], P, bs[i], 0, outs[i], P, l, 2, pm, s);
+            break;
+        }
+        default:
+            RCK_REQUIRE(false, "unknown batch op kind");
+    }
+}
+
+static void check_group(const Context& c, const BatchGroup& g) {
+    RCK_REQUIRE(g.limbs >= 1 && g.limbs <= c.L, "batch op limb count out of range");
+    if (g.kind == RCK_OP_MULT_RELIN_RESCALE) RCK_REQUIRE(g.limbs > 1, "rescale requires at least two levels");
+    if (g.kind == RCK_OP_ROTATE || g.kind == RCK_OP_MULT_RELIN || g.kind == RCK_OP_MULT_RELIN_RESCALE) {
+        RCK_REQUIRE(g.key != nullptr, "null key");
+        if (g.kind == RCK_OP_ROTATE)
+            RCK_REQUIRE((int)g.key->level == (int)g.limbs - 1, "rotation key level must match the ciphertext level");
+    }
+}
+
+static bool has_keyswitch(int kind) { return kind != RCK_OP_MULT_PLAIN; }
+
+// Candidate digit-MAC shapes for a group: eval-limb grouping x digit splits.
+static std::vector<KsShape> candidates(const Context& c, int E, int D) {
+    std::vector<KsShape> out;
+    if (c.logn <= 11) {
+        out.push_back({E, 1});
+        return out;
+    }
+    std::vector<int> gs = {E, (E + 1) / 2, (E + 3) / 4, 1};
+    std::sort(gs.begin(), gs.end());
+    gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+    for (int g : gs)
+        for (int ds : {1, 2, 4})
+            if (ds <= D) out.push_back({g, ds});
+    return out;
+}
+
+static void autotune(rck_batch* bt, BatchGroup& g) {
+    Context& c = *bt->sc;
+    const int E = (int)g.limbs + (int)g.key->aux;
+    const int D = g.key->aux ? (int)((c.qbits[g.limbs] + 29) / 30) : (int)g.key->digits;
+    const int B = (int)g.ops.size();
+    auto tk = std::make_tuple((int)c.logn, (int)g.limbs, E, D, B);
+    auto it = g_tuned.find(tk);
+    if (it != g_tuned.end()) {
+        g.shape = it->second.first;
+        g.tuned_ms = it->second.second;
+        return;
+    }
+    hipEvent_t e0, e1;
+    RCK_CHECK_HIP(hipEventCreate(&e0));
+    RCK_CHECK_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    KsShape best_shape{};
+    for (const KsShape& sh : candidates(c, E, D)) {
+        run_group(bt, g, bt->stream, sh);  // warm-up (sizes the scratch)
+        RCK_CHECK_HIP(hipEventRecord(e0, bt->stream));
+        const int reps = 2;
+        for (int r = 0; r < reps; r++) run_group(bt, g, bt->stream, sh);
+        RCK_CHECK_HIP(hipEventRecord(e1, bt->stream));
+        RCK_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RCK_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= reps;
+        if (ms < best) {
+            best = ms;
+            best_shape = sh;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g.shape = best_shape;
+    g.tuned_ms = best;
+    g_tuned[tk] = {best_shape, best};
+}
+
+extern "C" {
+
+int rck_batch_create(rck_context* ctx, rck_batch** out) {
+    return bguard([&] {
+        RCK_REQUIRE(ctx && ctx->c && out, "null argument");
+        rck_batch* b = new rck_batch();
+        b->ctx = ctx;
+        b->sc = ctx->c->scratch_clone();
+        RCK_CHECK_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        *out = b;
+    });
+}
+
+int rck_batch_add(rck_batch* bt, int op, uint32_t limbs, const uint64_t* in0, const uint64_t* in1,
+                  const rck_ksk* key, uint64_t* out) {
+    return bguard([&] {
+        RCK_REQUIRE(bt, "null batch");
+        RCK_REQUIRE(!bt->compiled, "batch already compiled");
+        RCK_REQUIRE(op >= RCK_OP_MULT_RELIN && op <= RCK_OP_MULT_RELIN_RESCALE, "unknown batch op kind");
+        RCK_REQUIRE(in0 && out, "null operand");
+        RCK_REQUIRE(op == RCK_OP_ROTATE || in1, "null second operand");
+        bt->ops.push_back(BatchOp{op, limbs, in0, in1, key, out});
+    });
+}
+
+int rck_batch_compile(rck_batch* bt, int tune) {
+    return bguard([&] {
+        RCK_REQUIRE(bt && !bt->compiled, "batch missing or already compiled");
+        Context& c = *bt->sc;
+        std::map<std::tuple<int, uint32_t, const rck_ksk*>, int> index;
+        for (size_t i = 0; i < bt->ops.size(); i++) {
+            const BatchOp& o = bt->ops[i];
+            const rck_ksk* k = o.kind == RCK_OP_MULT_PLAIN ? nullptr : o.key;
+            auto key = std::make_tuple(o.kind, o.limbs, k);
+            auto it = index.find(key);
+            if (it == index.end()) {
+                index[key] = (int)bt->groups.size();
+                BatchGroup g;
+                g.kind = o.kind;
+                g.limbs = o.limbs;
+                g.key = k;
+                g.ops.push_back((int)i);
+                bt->groups.push_back(g);
+            } else {
+                bt->groups[it->second].ops.push_back((int)i);
+            }
+        }
+        for (auto& g : bt->groups) check_group(c, g);
+        const bool prof = prof_enabled();
+        prof_set(false);
+        try {
+            for (auto& g : bt->groups)
+                if (tune && has_keyswitch(g.kind)) autotune(bt, g);
+            // eager run: sizes the scratch and leaves every output computed
+            for (auto& g : bt->groups) run_group(bt, g, bt->stream, g.shape);
+            RCK_CHECK_HIP(hipStreamSynchronize(bt->stream));
+            auto t0 = std::chrono::steady_clock::now();
+            c.frozen = true;
+            RCK_CHECK_HIP(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                for (auto& g : bt->groups) run_group(bt, g, bt->stream, g.shape);
+            } catch (...) {
+                hipGraph_t tmp;
+                (void)hipStreamEndCapture(bt->stream, &tmp);
+                if (tmp) (void)hipGraphDestroy(tmp);
+                throw;
+            }
+            RCK_CHECK_HIP(hipStreamEndCapture(bt->stream, &bt->graph));
+            RCK_CHECK_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+            c.frozen = false;
+            bt->capture_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        } catch (...) {
+            c.frozen = false;
+            prof_set(prof);
+            throw;
+        }
+        prof_set(prof);
+        bt->compiled = true;
+    });
+}
+
+int rck_batch_run(rck_batch* bt, void* stream) {
+    return bguard([&] {
+        RCK_REQUIRE(bt && bt->compiled, "batch not compiled");
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : bt->stream;
+        RCK_CHECK_HIP(hipGraphLaunch(bt->exec, s));
+    });
+}
+
+int rck_batch_run_eager(rck_batch* bt) {
+    return bguard([&] {
+        RCK_REQUIRE(bt && bt->compiled, "batch not compiled");
+        for (auto& g : bt->groups) run_group(bt, g, bt->stream, g.shape);
+    });
+}
+
+int rck_batch_synchronize(rck_batch* bt) {
+    return bguard([&] {
+        RCK_REQUIRE(bt, "null batch");
+        RCK_CHECK_HIP(hipStreamSynchronize(bt->stream));
+    });
+}
+
+int rck_batch_report(rck_batch* bt, char* buf, size_t len) {
+    return bguard([&] {
+        RCK_REQUIRE(bt, "null batch");
+        std::ostringstream o;
+        o << "{\"ops\": " << bt->ops.size() << ", \"groups\": [";
+        for (size_t i = 0; i < bt->groups.size(); i++) {
+            const auto& g = bt->groups[i];
+            o << (i ? ", " : "") << "{\"kind\": " << g.kind << ", \"limbs\": " << g.limbs
+              << ", \"ops\": " << g.ops.size() << ", \"eval_limb_group\": " << g.shape.group
+              << ", \"digit_splits\": " << g.shape.dsplit << ", \"tuned_ms\": " << g.tuned_ms << "}";
+        }
+        o << "], \"capture_ms\": " << bt->capture_ms << "}";
+        std::string r = o.str();
+        RCK_REQUIRE(buf && len > r.size(), "report buffer too small");
+        std::memcpy(buf, r.c_str(), r.size() + 1);
+    });
+}
+
+int rck_batch_destroy(rck_batch* bt) {
+    return bguard([&] {
+        if (!bt) return;
+        if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+        if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+        if (bt->graph) (void)hipGraphDestroy(bt->graph);
+        delete bt->sc;
+        if (bt->stream) (void)hipStreamDestroy(bt->stream);
+        delete bt;
+    });
+}
+
+}  // extern "C"

@@ -5,7 +5,7 @@
 #include <new>
 
 #include "../../include/rnsckks.h"
-#include "ops.hpp"
+#include "capi_types.hpp"
 
 namespace rck {
 Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32_t L, const uint64_t* p,
@@ -14,16 +14,6 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
 
 using namespace rck;
 
-struct rck_context {
-    Context* c;
-};
-
-struct rck_ksk {
-    rck_context* owner;
-    double* data;  // NTT form: [digits][2][limbs][N]
-    uint32_t digits, level, aux, limbs;
-    uint64_t galois;
-};
 
 template <typename F>
 static int guard(F&& f) {
@@ -55,7 +45,6 @@ static void check_limbs(const Context& c, uint32_t nq, uint32_t np) {
     RCK_REQUIRE(np == 0 || np == c.K, "aux limb count must be 0 or the aux basis size");
 }
 
-static void check_ptr(const void* p, const char* what) { RCK_REQUIRE(p != nullptr, std::string("null ") + what); }
 
 // floor(log2 q) of the dropped prime (ops.rs:74)
 static uint32_t floor_log2(uint64_t q) { return 63 - __builtin_clzll(q); }
@@ -221,19 +210,6 @@ int rck_poly_sub(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_
     });
 }
 
-// negacyclic product of `polys` polynomial pairs a[z] * b[z or 0] -> out[z] (NTT resident)
-static void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
-                     long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
-    const long long n = c.n, P = (long long)limbs * n;
-    const int bpolys = b_ps ? polys : 1;
-    double* t = static_cast<double*>(c.workspace(0, (size_t)(polys + bpolys) * P * 8));
-    double* tb = t + (size_t)polys * P;
-    launch_fwd(c, kInU64, a, n, a_ps, t, n, P, limbs, polys, pm, s);
-    launch_fwd(c, kInU64, b, n, b_ps, tb, n, P, limbs, bpolys, pm, s);
-    launch_mul_f64(c, {t, n, P}, {tb, n, b_ps ? P : 0}, {t, n, P}, limbs, polys, pm, s);
-    launch_inv(c, t, n, P, kOutU64, out, n, out_ps, limbs, polys, pm, s);
-}
-
 int rck_poly_multiply_ntt(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq,
                           uint32_t np, void* stream) {
     return guard([&] {
@@ -283,17 +259,6 @@ int rck_ct_add(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint3
     });
 }
 
-static void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, uint64_t* out, hipStream_t s) {
-    const long long n = c.n, P = (long long)limbs * n;
-    PrimeMap pm{limbs, (int)c.L, 0};
-    double* x = static_cast<double*>(c.workspace(0, (size_t)4 * P * 8));
-    double* d = static_cast<double*>(c.workspace(1, (size_t)3 * P * 8));
-    launch_fwd(c, kInU64, lhs, n, P, x, n, P, limbs, 2, pm, s);
-    launch_fwd(c, kInU64, rhs, n, P, x + 2 * P, n, P, limbs, 2, pm, s);
-    launch_tensor(c, x, d, limbs, pm, s);
-    launch_inv(c, d, n, P, kOutU64, out, n, P, limbs, 3, pm, s);
-}
-
 int rck_mult(rck_context* ctx, const uint64_t* lhs, const uint64_t* rhs, uint32_t limbs, uint32_t lhs_scale,
              uint32_t rhs_scale, uint64_t* out_quad, uint32_t* out_scale, void* stream) {
     return guard([&] {
@@ -317,11 +282,6 @@ int rck_mult_plain(rck_context* ctx, const uint64_t* ct, const uint64_t* plain, 
         poly_mul(c, ct, P, plain, 0, out, P, limbs, 2, PrimeMap{(int)limbs, (int)c.L, 0}, S(stream));
         if (out_scale) *out_scale = ct_scale + ct_scale;
     });
-}
-
-static void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t* out, hipStream_t s) {
-    const long long n = c.n;
-    launch_rescale(c, {in, n, (long long)limbs * n}, {out, n, (long long)(limbs - 1) * n}, limbs, polys, s);
 }
 
 int rck_rescale(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale, uint64_t* out,
@@ -366,7 +326,14 @@ static void ksk_create(rck_context* ctx, const uint64_t* key, bool on_device, ui
     RCK_REQUIRE(digits >= 1, "keyswitch key must have digits");
     const uint32_t limbs = level + 1 + aux;
     const long long n = c.n, P = (long long)limbs * n;
-    rck_ksk* k = new rck_ksk{ctx, nullptr, digits, level, aux, limbs, galois};
+    rck_ksk* k = new rck_ksk();
+    k->owner = ctx;
+    k->data = nullptr;
+    k->digits = digits;
+    k->level = level;
+    k->aux = aux;
+    k->limbs = limbs;
+    k->galois = galois;
     try {
         RCK_CHECK_HIP(hipMalloc(&k->data, (size_t)digits * 2 * P * 8));
         uint64_t* stage = nullptr;
@@ -407,53 +374,6 @@ int rck_ksk_destroy(rck_ksk* key) {
         if (key->data) RCK_CHECK_HIP(hipFree(key->data));
         delete key;
     });
-}
-
-// keyswitch of `poly` (limbs Q-limbs); adds add0 / add1 (nullable, limbs limbs each) to the two
-// outputs.  Mirrors keyswitch_rns (keyswitching.rs:130-157) and keyswitch_rns_hybrid (:159-194).
-static void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const rck_ksk* key, const uint64_t* add0,
-                           const uint64_t* add1, uint64_t* out, hipStream_t s) {
-    check_ptr(key, "key");
-    const long long n = c.n, P = (long long)limbs * n;
-    const uint32_t active = (c.qbits[limbs] + 29) / 30;
-    int levels;
-    if (key->aux == 0) {
-        RCK_REQUIRE((int)key->level == limbs - 1, "keyswitch key level must match the active modulus chain");
-        RCK_REQUIRE(key->digits >= active, "keyswitch key must cover the active modulus chain");
-        levels = key->digits;  // keyswitching.rs:138 decomposes into ksk.ksk.len() digits
-    } else {
-        RCK_REQUIRE((int)key->level == limbs - 1, "keyswitch key level must match the active Q chain");
-        RCK_REQUIRE(key->aux == c.K, "eval-key context must match the keyswitch key auxiliary basis");
-        RCK_REQUIRE(key->digits >= active, "keyswitch key must cover the active modulus chain");
-        RCK_REQUIRE(c.qbits[limbs] > 30, "hybrid keyswitching needs a ciphertext modulus above 2^30");
-        levels = active;  // keyswitching.rs:170
-    }
-    const int naux = key->aux;
-    const int elimbs = limbs + naux;
-    const long long EP = (long long)elimbs * n;
-    int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)levels * n * 4));
-    double* acc = static_cast<double*>(c.workspace(6, (size_t)2 * EP * 8));
-    launch_decompose(c, poly, limbs, levels, digits, s);
-    launch_digit_mac(c, digits, levels, key->data, limbs, naux, acc, s);
-    PrimeMap pm{limbs, (int)c.L, 0};
-    if (naux) {
-        uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)2 * EP * 8));
-        launch_inv(c, acc, n, EP, kOutU64, accu, n, EP, elimbs, 2, pm, s);
-        launch_mod_down(c, accu, limbs, add0, add1, out, P, s);
-    } else if (add0 || add1) {
-        uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)2 * P * 8));
-        launch_inv(c, acc, n, P, kOutU64, accu, n, P, limbs, 2, pm, s);
-        if (add0)
-            launch_addsub(c, {add0, n, 0}, {accu, n, 0}, {out, n, 0}, limbs, 1, 0, pm, s);
-        else
-            RCK_CHECK_HIP(hipMemcpyAsync(out, accu, P * 8, hipMemcpyDeviceToDevice, s));
-        if (add1)
-            launch_addsub(c, {add1, n, 0}, {accu + P, n, 0}, {out + P, n, 0}, limbs, 1, 0, pm, s);
-        else
-            RCK_CHECK_HIP(hipMemcpyAsync(out + P, accu + P, P * 8, hipMemcpyDeviceToDevice, s));
-    } else {
-        launch_inv(c, acc, n, P, kOutU64, out, n, P, limbs, 2, pm, s);
-    }
 }
 
 int rck_keyswitch(rck_context* ctx, const uint64_t* poly, uint32_t limbs, const rck_ksk* key, uint64_t* out,

@@ -327,6 +327,7 @@ void* Context::workspace(int slot, size_t bytes) {
         ws_size.resize(slot + 1, 0);
     }
     if (ws_size[slot] < bytes) {
+        RCK_REQUIRE(!frozen, "workspace growth during graph capture");
         if (ws_ptr[slot]) {
             RCK_CHECK_HIP(hipDeviceSynchronize());
             RCK_CHECK_HIP(hipFree(ws_ptr[slot]));
@@ -339,10 +340,35 @@ void* Context::workspace(int slot, size_t bytes) {
     return ws_ptr[slot];
 }
 
+Context* Context::scratch_clone() const {
+    Context* c = new Context();
+    c->n = n;
+    c->logn = logn;
+    c->L = L;
+    c->K = K;
+    c->scale_bits = scale_bits;
+    c->hp = hp;
+    c->d_primes = d_primes;
+    c->d_tables = d_tables;
+    c->d_invq = d_invq;
+    c->d_qwords = d_qwords;
+    c->qwords = qwords;
+    c->qbits = qbits;
+    c->d_pwords = d_pwords;
+    c->d_pmodq = d_pmodq;
+    c->d_pinvq = d_pinvq;
+    c->d_pmod_aux = d_pmod_aux;
+    c->tune_group = tune_group;
+    c->tune_dsplit = tune_dsplit;
+    c->owns_tables = false;
+    return c;
+}
+
 Context::~Context() {
     (void)hipDeviceSynchronize();
     for (void* p : ws_ptr)
         if (p) (void)hipFree(p);
+    if (!owns_tables) return;
     void* ptrs[] = {d_primes, d_tables, d_invq, d_qwords, d_pwords, d_pmodq, d_pinvq, d_pmod_aux};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -53,12 +53,18 @@ struct Context {
     double* d_pmodq = nullptr;     // [L] P mod q_i (centred)
     double* d_pinvq = nullptr;     // [L] P^{-1} mod q_i (centred)
     double* d_pmod_aux = nullptr;  // [K][L] p_k mod q_i (centred)
+    bool owns_tables = true;  // false for a scratch clone sharing another context's tables
+    bool frozen = false;      // workspace may not grow (graph capture in progress)
+    // keyswitch launch-shape tuning (0 = automatic): eval limbs per digit-MAC group, digit splits
+    int tune_group = 0, tune_dsplit = 0;
     // scratch owned by the context, grown on demand
     std::vector<void*> ws_ptr;
     std::vector<size_t> ws_size;
 
     uint32_t prime_count() const { return L + K; }
     void* workspace(int slot, size_t bytes);
+    // a context sharing this one's device tables with its own (empty) scratch
+    Context* scratch_clone() const;
     ~Context();
 };
 

@@ -23,7 +23,8 @@ namespace rck {
 constexpr int kDecBS = 128;
 
 template <int W>
-__global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict__ poly, int n, int nq, int levels,
+__global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict__ poly, long long ps, int n, int nq,
+                                                      int levels,
                                                       const DevPrime* __restrict__ primes,
                                                       const double* __restrict__ invq, int T,
                                                       const uint32_t* __restrict__ qw, const uint32_t* __restrict__ hw,
@@ -32,6 +33,8 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     const int tid = threadIdx.x;
     const long long c = (long long)blockIdx.x * kDecBS + tid;
     if (c >= n) return;
+    poly += (long long)blockIdx.y * ps;
+    digits += (long long)blockIdx.y * levels * n;
     // Garner (row order as in the reference: m_i from r_i and m_0 .. m_{i-1})
     for (int i = 0; i < nq; i++) {
         const double q = primes[i].q, qinv = primes[i].qinv;
@@ -114,59 +117,74 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
 }
 
 template <int W>
-static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s) {
+static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
+                  long long ps) {
     const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords;
-    ProfScope ps(s, "decompose");
-    k_decompose<W><<<dim3((c.n + kDecBS - 1) / kDecBS), kDecBS, (size_t)nq * kDecBS * 8, s>>>(poly, c.n, nq, levels, c.d_primes, c.d_invq,
+    ProfScope prof(s, "decompose");
+    k_decompose<W><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, (size_t)nq * kDecBS * 8, s>>>(poly, ps, c.n, nq, levels, c.d_primes, c.d_invq,
                                                                         c.L + c.K, qw, qw + kWords, digits);
     RCK_CHECK_HIP(hipGetLastError());
 }
 
-void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s) {
+void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
+                      long long ps) {
     RCK_REQUIRE(nq >= 1 && nq <= (int)c.L, "bad limb count for decomposition");
     const int need = std::max((int)(c.qbits[nq] + 32) / 32 + 1, (30 * levels + 31) / 32 + 1);
     if (need <= 8)
-        dec_t<8>(c, poly, nq, levels, digits, s);
+        dec_t<8>(c, poly, nq, levels, digits, s, B, ps);
     else if (need <= 16)
-        dec_t<16>(c, poly, nq, levels, digits, s);
+        dec_t<16>(c, poly, nq, levels, digits, s, B, ps);
     else if (need <= 24)
-        dec_t<24>(c, poly, nq, levels, digits, s);
+        dec_t<24>(c, poly, nq, levels, digits, s, B, ps);
     else if (need <= 32)
-        dec_t<32>(c, poly, nq, levels, digits, s);
+        dec_t<32>(c, poly, nq, levels, digits, s, B, ps);
     else if (need <= 40)
-        dec_t<40>(c, poly, nq, levels, digits, s);
+        dec_t<40>(c, poly, nq, levels, digits, s, B, ps);
     else if (need <= 48)
-        dec_t<48>(c, poly, nq, levels, digits, s);
+        dec_t<48>(c, poly, nq, levels, digits, s, B, ps);
     else if (need <= 64)
-        dec_t<64>(c, poly, nq, levels, digits, s);
+        dec_t<64>(c, poly, nq, levels, digits, s, B, ps);
     else
         RCK_REQUIRE(false, "decomposition needs more than 2048 bits");
 }
 
 // ---------------------------------------------------------------------------------------
-// Digit MAC: row pass of the forward NTT of every digit for one eval prime, fused with the
-// multiply-accumulate against the NTT-form key.  The column pass ran before into `cols`
-// ([levels][N] doubles for this prime).
+// Digit MAC: the row pass of the forward NTT of every digit, fused with the multiply-
+// accumulate against the NTT-form key.  The column pass ran before into
+// cols[b][i][jl][N] for the eval limbs j0 .. j0+G-1 of this group.
+//
+// Work item = (row block, eval limb, digit split, ciphertext b).  The 1-D grid is laid out
+// so that the B work items that read the same key tile get consecutive dispatch slots on
+// the same XCD (workgroup w runs on XCD w % 8): the key rows are fetched from HBM once and
+// served to the other ciphertexts of the batch from that XCD's L2.
+// Output: partial sums part[s][b][comp][j][N] per digit split s (reduced doubles).
 // ---------------------------------------------------------------------------------------
 template <int LOGR>
-__global__ __launch_bounds__(256) void k_digit_mac_rows(const double* __restrict__ cols, int levels,
-                                                        const double* __restrict__ key, long long key_comp_stride,
-                                                        long long key_digit_stride, double* __restrict__ acc0,
-                                                        double* __restrict__ acc1, int logn,
-                                                        const DevPrime* __restrict__ primes, int prime) {
+__global__ __launch_bounds__(256) void k_digit_mac(const double* __restrict__ cols, int D, int G, int B, int dsplit,
+                                                   int ntiles, const double* __restrict__ key, int E, int j0,
+                                                   double* __restrict__ part, int logn, PrimeMap pm,
+                                                   const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[S * (R + R / 16 + 1)];
-    const int n = 1 << logn, N1 = n >> LOGR;
+    const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / S;
+    const int g = blockIdx.x, xcd = g & 7, slot = g >> 3;
+    const int cb = slot % B;
+    const int tile = (slot / B) * 8 + xcd;
+    if (tile >= ntiles) return;
+    const int rb = tile % nrb, rest = tile / nrb, jl = rest % G, sp = rest / G;
+    const int j = j0 + jl;
+    const int i0 = (int)((long long)D * sp / dsplit), i1 = (int)((long long)D * (sp + 1) / dsplit);
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
-    const int b = blockIdx.x * S + row;
-    const DevPrime P = primes[prime];
+    const int b = rb * S + row;
+    const DevPrime P = primes[pm(j)];
     const double q = P.q, qinv = P.qinv;
+    const long long off = (long long)b * R;
+    const long long EN = (long long)E * n;
     double a0[16], a1[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) a0[e] = a1[e] = 0.0;
-    const long long off = (long long)b * R;
-    for (int i = 0; i < levels; i++) {
-        const double* p = cols + (long long)i * n + off;
+    for (int i = i0; i < i1; i++) {
+        const double* p = cols + (((long long)cb * D + i) * G + jl) * n + off;
         double v[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
@@ -178,88 +196,95 @@ __global__ __launch_bounds__(256) void k_digit_mac_rows(const double* __restrict
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
         fwd_phase_b<LOGR>(v, P.fw, N1 + b, tau, q, qinv);
-        const double* k0 = key + (long long)i * key_digit_stride + off + 16 * tau;
-        const double* k1 = k0 + key_comp_stride;
+        const double* k0 = key + (2LL * i) * EN + (long long)j * n + off + 16 * tau;
+        const double* k1 = k0 + EN;
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             a0[e] += mulmod(v[e], k0[e], q, qinv);
             a1[e] += mulmod(v[e], k1[e], q, qinv);
         }
-        if ((i & 3) == 3) {
+        if (((i - i0) & 3) == 3) {
             reduce_all(a0, q, qinv);
             reduce_all(a1, q, qinv);
         }
     }
     reduce_all(a0, q, qinv);
     reduce_all(a1, q, qinv);
+    double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + off + 16 * tau;
+    double* o1 = o0 + EN;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
-        acc0[off + 16 * tau + e] = a0[e];
-        acc1[off + 16 * tau + e] = a1[e];
+        o0[e] = a0[e];
+        o1[e] = a1[e];
     }
 }
 
 template <int LOGR>
-static void mac_rows_t(const Context& c, const double* cols, int levels, const double* key, long long comp_stride,
-                       long long digit_stride, double* acc0, double* acc1, int prime, hipStream_t s) {
+static void mac_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
+                  int j0, double* part, PrimeMap pm, hipStream_t s) {
     constexpr int S = 256 / ((1 << LOGR) / 16);
-    const int n1 = 1 << (c.logn - LOGR);
-    ProfScope ps(s, "digit_mac_rows");
-    k_digit_mac_rows<LOGR><<<dim3(n1 / S), 256, 0, s>>>(cols, levels, key, comp_stride, digit_stride, acc0, acc1,
-                                                         c.logn, c.d_primes, prime);
+    const int nrb = (1 << (c.logn - LOGR)) / S;
+    const int ntiles = nrb * G * dsplit;
+    const int blocks = ((ntiles + 7) / 8) * 8 * B;
+    ProfScope ps(s, "digit_mac");
+    k_digit_mac<LOGR><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, dsplit, ntiles, key, E, j0, part, c.logn, pm,
+                                                    c.d_primes);
     RCK_CHECK_HIP(hipGetLastError());
 }
 
 // Small rings: plain forward NTT of every digit for every prime, then pointwise MAC.
-__global__ void k_mac_small(const double* __restrict__ dn, int levels, const double* __restrict__ key, int limbs,
-                            int n, double* __restrict__ acc, PrimeMap pm, const DevPrime* __restrict__ primes) {
+// dn: [B][D][E][N] NTT-form digits; part: [B][2][E][N].
+__global__ void k_mac_small(const double* __restrict__ dn, int D, const double* __restrict__ key, int E, int n,
+                            double* __restrict__ part, PrimeMap pm, const DevPrime* __restrict__ primes) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int y = blockIdx.y;
+    const int y = blockIdx.y, cb = blockIdx.z;
     const DevPrime P = primes[pm(y)];
-    const long long poly = (long long)limbs * n;
+    const long long EN = (long long)E * n;
     double s0 = 0.0, s1 = 0.0;
-    for (int d = 0; d < levels; d++) {
-        const double v = dn[d * poly + (long long)y * n + i];
-        s0 += mulmod(v, key[(2LL * d) * poly + (long long)y * n + i], P.q, P.qinv);
-        s1 += mulmod(v, key[(2LL * d + 1) * poly + (long long)y * n + i], P.q, P.qinv);
+    for (int d = 0; d < D; d++) {
+        const double v = dn[((long long)cb * D + d) * EN + (long long)y * n + i];
+        s0 += mulmod(v, key[(2LL * d) * EN + (long long)y * n + i], P.q, P.qinv);
+        s1 += mulmod(v, key[(2LL * d + 1) * EN + (long long)y * n + i], P.q, P.qinv);
         if ((d & 3) == 3) {
             s0 = reduce(s0, P.q, P.qinv);
             s1 = reduce(s1, P.q, P.qinv);
         }
     }
-    acc[(long long)y * n + i] = reduce(s0, P.q, P.qinv);
-    acc[poly + (long long)y * n + i] = reduce(s1, P.q, P.qinv);
+    part[(2LL * cb) * EN + (long long)y * n + i] = reduce(s0, P.q, P.qinv);
+    part[(2LL * cb + 1) * EN + (long long)y * n + i] = reduce(s1, P.q, P.qinv);
 }
 
-void launch_digit_mac(const Context& c, const int32_t* digits, int levels, const double* key, int nq, int naux,
-                      double* acc, hipStream_t s) {
-    const int limbs = nq + naux;
-    const long long n = c.n, poly = (long long)limbs * n;
+void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
+                      double* part, int dsplit, int group, hipStream_t s) {
+    const int E = nq + naux;
+    const long long n = c.n, EN = (long long)E * n;
     PrimeMap pm{nq, (int)c.L, 0};
+    Context& cc = const_cast<Context&>(c);
     if (c.logn <= 11) {
-        double* dn = static_cast<double*>(const_cast<Context&>(c).workspace(7, (size_t)levels * poly * 8));
-        // every digit row is NTT'd for every prime: in limb stride 0, poly stride n
-        launch_fwd(c, kInI32, digits, 0, n, dn, n, poly, limbs, levels, pm, s);
+        RCK_REQUIRE(dsplit == 1, "small rings use a single digit split");
+        double* dn = static_cast<double*>(cc.workspace(7, (size_t)B * D * EN * 8));
+        // every digit row is NTT'd for every prime: limb stride 0 on the input
+        launch_fwd(c, kInI32, digits, 0, n, dn, n, EN, E, B * D, pm, s);
         ProfScope ps(s, "mac_small");
-        k_mac_small<<<dim3((c.n + 255) / 256, limbs), 256, 0, s>>>(dn, levels, key, limbs, c.n, acc, pm, c.d_primes);
+        k_mac_small<<<dim3((c.n + 255) / 256, E, B), 256, 0, s>>>(dn, D, key, E, c.n, part, pm, c.d_primes);
         RCK_CHECK_HIP(hipGetLastError());
         return;
     }
-    double* cols = static_cast<double*>(const_cast<Context&>(c).workspace(7, (size_t)levels * n * 8));
+    RCK_REQUIRE(dsplit >= 1 && dsplit <= D, "bad digit split");
+    const int G = std::max(1, std::min(group, E));
+    double* cols = static_cast<double*>(cc.workspace(7, (size_t)B * D * G * n * 8));
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
-    for (int j = 0; j < limbs; j++) {
-        PrimeMap pj{nq, (int)c.L, j};
-        launch_fwd_cols(c, kInI32, digits, 0, n, cols, 0, n, 1, levels, pj, s);
-        const int prime = j < nq ? j : (int)c.L + (j - nq);
-        const double* kj = key + (long long)j * n;
-        double* a0 = acc + (long long)j * n;
-        double* a1 = acc + poly + (long long)j * n;
+    for (int j0 = 0; j0 < E; j0 += G) {
+        const int g = std::min(G, E - j0);
+        // column pass of digit (b, i) for eval limbs j0 .. j0+g-1 -> cols[b][i][jl][N]
+        launch_fwd_cols(c, kInI32, digits, 0, n, cols, n, (long long)g * n, g, B * D, PrimeMap{nq, (int)c.L, j0},
+                        s);
         switch (n2) {
-            case 6: mac_rows_t<6>(c, cols, levels, kj, poly, 2 * poly, a0, a1, prime, s); break;
-            case 7: mac_rows_t<7>(c, cols, levels, kj, poly, 2 * poly, a0, a1, prime, s); break;
-            case 8: mac_rows_t<8>(c, cols, levels, kj, poly, 2 * poly, a0, a1, prime, s); break;
+            case 6: mac_t<6>(c, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 7: mac_t<7>(c, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 8: mac_t<8>(c, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
             default: RCK_REQUIRE(false, "unsupported ring degree");
         }
     }

@@ -77,9 +77,11 @@ __global__ __launch_bounds__(256) void k_fwd_rows(Batch<double> data, int logn, 
 }
 
 // Row pass, inverse (in place).
+// nsum > 1: the input is the sum of nsum partial operands at element stride sstride (digit-split
+// keyswitch accumulators); the result is written over the first one.
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, PrimeMap pm,
-                                                  const DevPrime* __restrict__ primes) {
+                                                  const DevPrime* __restrict__ primes, int nsum, long long sstride) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[S * (R + R / 16 + 1)];
     const int N1 = 1 << (logn - LOGR);
@@ -92,6 +94,13 @@ __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, 
     double v[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = p[16 * tau + e];
+    if (nsum > 1) {
+        for (int k = 1; k < nsum; k++) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) v[e] += p[k * sstride + 16 * tau + e];
+        }
+        reduce_all(v, q, qinv);  // |sum| <= nsum * 0.51q < 2^53 for nsum <= 8
+    }
     inv_phase_b<LOGR>(v, P.iv, N1 + b, tau, q, qinv);
 #pragma unroll
     for (int e = 0; e < 16; e++) lds[row_lds<R>(row, 16 * tau + e)] = v[e];
@@ -101,6 +110,77 @@ __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, 
     inv_phase_a(v, P.iv, N1 + b, q, qinv);
 #pragma unroll
     for (int e = 0; e < 16; e++) p[tau + TPS * e] = v[e];
+}
+
+// ---------------------------------------------------------------------------------------
+// ct x ct tensor fused between the passes (multiply_ciphertexts_rns, src/rns/ops.rs:84-100):
+// xl[z] = [a0 a1], xr[z] = [b0 b1] after the forward column pass; this kernel finishes the forward row
+// stages of all four, forms d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1 in registers, runs the
+// inverse row stages of d0..d2 and writes them to d[z] = [d0 d1 d2].
+// ---------------------------------------------------------------------------------------
+template <int LOGR>
+__device__ __forceinline__ void row_fwd(double (&v)[16], const double* p, double* lds, int row, int tau, int base,
+                                        const double* __restrict__ fw, double q, double qinv) {
+    constexpr int R = 1 << LOGR, TPS = R / 16;
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
+    fwd_phase_a(v, fw, base, q, qinv);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, tau + TPS * e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
+    fwd_phase_b<LOGR>(v, fw, base, tau, q, qinv);
+}
+
+template <int LOGR>
+__device__ __forceinline__ void row_inv_store(double (&v)[16], double* p, double* lds, int row, int tau, int base,
+                                              const double* __restrict__ iv, double q, double qinv) {
+    constexpr int R = 1 << LOGR, TPS = R / 16;
+    inv_phase_b<LOGR>(v, iv, base, tau, q, qinv);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, 16 * tau + e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, tau + TPS * e)];
+    inv_phase_a(v, iv, base, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) p[tau + TPS * e] = v[e];
+}
+
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_mult_rows(const double* __restrict__ xl, const double* __restrict__ xr,
+                                                   double* __restrict__ d, long long P, int logn, PrimeMap pm,
+                                                   const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[S * (R + R / 16 + 1)];
+    const int n = 1 << logn, N1 = n >> LOGR;
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int b = blockIdx.x * S + row;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const DevPrime Pr = primes[pm(y)];
+    const double q = Pr.q, qinv = Pr.qinv;
+    const long long off = (long long)y * n + (long long)b * R;
+    const double* lz = xl + (long long)z * 2 * P + off;
+    const double* rz = xr + (long long)z * 2 * P + off;
+    double* dz = d + (long long)z * 3 * P + off;
+    double a0[16], a1[16], b0[16], b1[16];
+    row_fwd<LOGR>(a0, lz, lds, row, tau, N1 + b, Pr.fw, q, qinv);
+    row_fwd<LOGR>(a1, lz + P, lds, row, tau, N1 + b, Pr.fw, q, qinv);
+    row_fwd<LOGR>(b0, rz, lds, row, tau, N1 + b, Pr.fw, q, qinv);
+    row_fwd<LOGR>(b1, rz + P, lds, row, tau, N1 + b, Pr.fw, q, qinv);
+    double t[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        t[e] = reduce(mulmod(a0[e], b1[e], q, qinv) + mulmod(a1[e], b0[e], q, qinv), q, qinv);
+        a0[e] = reduce(mulmod(a0[e], b0[e], q, qinv), q, qinv);
+        a1[e] = reduce(mulmod(a1[e], b1[e], q, qinv), q, qinv);
+    }
+    row_inv_store<LOGR>(a0, dz, lds, row, tau, N1 + b, Pr.iv, q, qinv);
+    row_inv_store<LOGR>(t, dz + P, lds, row, tau, N1 + b, Pr.iv, q, qinv);
+    row_inv_store<LOGR>(a1, dz + 2 * P, lds, row, tau, N1 + b, Pr.iv, q, qinv);
 }
 
 // Column pass, inverse, with the N^{-1} scaling; writes canonical u64 or reduced doubles.
@@ -236,11 +316,12 @@ static void fwd_rows_t(const Context& c, Batch<double> d, int limbs, int polys, 
 }
 
 template <int LOGR>
-static void inv_rows_t(const Context& c, Batch<double> d, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+static void inv_rows_t(const Context& c, Batch<double> d, int limbs, int polys, PrimeMap pm, hipStream_t s, int nsum,
+                       long long sstride) {
     constexpr int S = 256 / ((1 << LOGR) / 16);
     const int n1 = 1 << (c.logn - LOGR);
     ProfScope ps(s, "ntt_inv_rows");
-    k_inv_rows<LOGR><<<dim3(n1 / S, limbs, polys), 256, 0, s>>>(d, c.logn, pm, c.d_primes);
+    k_inv_rows<LOGR><<<dim3(n1 / S, limbs, polys), 256, 0, s>>>(d, c.logn, pm, c.d_primes, nsum, sstride);
     RCK_LAUNCH_CHECK();
 }
 
@@ -259,6 +340,31 @@ static void inv_cols_t(const Context& c, Batch<const double> in, int out_kind, B
 }
 
 static bool small_ring(const Context& c) { return c.logn <= 11; }
+
+template <int LOGR>
+static void mult_rows_t(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
+                        hipStream_t s) {
+    constexpr int S = 256 / ((1 << LOGR) / 16);
+    const int n1 = 1 << (c.logn - LOGR);
+    ProfScope ps(s, "mult_rows");
+    k_mult_rows<LOGR><<<dim3(n1 / S, limbs, B), 256, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm,
+                                                              c.d_primes);
+    RCK_LAUNCH_CHECK();
+}
+
+void launch_mult_rows(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
+                      hipStream_t s) {
+    check_dims(c, limbs, B);
+    RCK_REQUIRE(!small_ring(c), "fused tensor rows need N >= 2^12");
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    switch (n2) {
+        case 6: mult_rows_t<6>(c, xl, xr, d, limbs, B, pm, s); break;
+        case 7: mult_rows_t<7>(c, xl, xr, d, limbs, B, pm, s); break;
+        case 8: mult_rows_t<8>(c, xl, xr, d, limbs, B, pm, s); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+}
 
 void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
                      long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
@@ -291,15 +397,16 @@ void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps,
 }
 
 void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
-                     hipStream_t s) {
+                     hipStream_t s, int nsum, long long sstride) {
     check_dims(c, limbs, polys);
+    RCK_REQUIRE(nsum >= 1 && nsum <= 8, "at most 8 partial sums");
     Batch<double> b{data, ls, ps};
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
     switch (n2) {
-        case 6: inv_rows_t<6>(c, b, limbs, polys, pm, s); break;
-        case 7: inv_rows_t<7>(c, b, limbs, polys, pm, s); break;
-        case 8: inv_rows_t<8>(c, b, limbs, polys, pm, s); break;
+        case 6: inv_rows_t<6>(c, b, limbs, polys, pm, s, nsum, sstride); break;
+        case 7: inv_rows_t<7>(c, b, limbs, polys, pm, s, nsum, sstride); break;
+        case 8: inv_rows_t<8>(c, b, limbs, polys, pm, s, nsum, sstride); break;
         default: RCK_REQUIRE(false, "unsupported ring degree");
     }
 }
@@ -341,9 +448,11 @@ void launch_fwd(const Context& c, int in_kind, const void* in, long long in_ls, 
 }
 
 void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, int out_kind, void* out,
-                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, int nsum,
+                long long sstride) {
     check_dims(c, limbs, polys);
     if (small_ring(c)) {
+        RCK_REQUIRE(nsum == 1, "partial sums need the two-pass NTT");
         Batch<const double> bi{in, in_ls, in_ps};
         Batch<void> bo{out, out_ls, out_ps};
         dim3 grid(1, limbs, polys);
@@ -355,7 +464,7 @@ void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, 
         RCK_LAUNCH_CHECK();
         return;
     }
-    launch_inv_rows(c, in, in_ls, in_ps, limbs, polys, pm, s);
+    launch_inv_rows(c, in, in_ls, in_ps, limbs, polys, pm, s, nsum, sstride);
     launch_inv_cols(c, in, in_ls, in_ps, out_kind, out, out_ls, out_ps, limbs, polys, pm, s);
 }
 

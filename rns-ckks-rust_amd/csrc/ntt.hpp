@@ -157,8 +157,6 @@ inline void split_logn(int logn, int* n1, int* n2) {
 // ---------------------------------------------------------------------------------------
 // Host launchers (ntt.hip)
 // ---------------------------------------------------------------------------------------
-struct Ctx;  // fwd decl not used
-
 // forward transform of a batch (coefficient form -> NTT form, doubles, reduced)
 void launch_fwd(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
                 long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
@@ -168,10 +166,16 @@ void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in
 void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
                      hipStream_t s);
 // inverse transform (NTT form doubles -> coefficient form u64 canonical, or reduced doubles)
+// nsum > 1: the input of every (poly, limb) is the sum of nsum partial operands, sstride elements apart
 void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, int out_kind, void* out,
-                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+                long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, int nsum = 1,
+                long long sstride = 0);
 void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
-                     hipStream_t s);
+                     hipStream_t s, int nsum = 1, long long sstride = 0);
+// xl: B x [a0 a1], xr: B x [b0 b1] after the forward column pass (limbs limbs each) -> d: B x [d0 d1 d2]
+// after the inverse row pass (tensor fused between the row passes)
+void launch_mult_rows(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
+                      hipStream_t s);
 void launch_inv_cols(const Context& c, const double* in, long long in_ls, long long in_ps, int out_kind, void* out,
                      long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
 

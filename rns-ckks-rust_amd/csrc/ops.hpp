@@ -27,11 +27,15 @@ void launch_twist_out(const Context& c, Batch<const double> in, Batch<uint64_t> 
 // keyswitch pipeline
 // Balanced base-2^30 decomposition of a coefficient-form polynomial with nq Q-limbs into
 // `levels` signed digits per coefficient: digits[i * N + c] (keyswitching.rs:196-236, 381-467).
-void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s);
-// acc_{0,1}[j] = sum_i NTT_j(digit_i) * key[i][{0,1}][j] for every eval prime j (Q-prefix nq, then
-// naux aux primes), NTT form.  key: [levels][2][nq + naux][N] NTT-form doubles.
-void launch_digit_mac(const Context& c, const int32_t* digits, int levels, const double* key, int nq, int naux,
-                      double* acc, hipStream_t s);
+// B polynomials at poly stride `ps` elements -> digits [B][levels][N].
+void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s,
+                      int B = 1, long long ps = 0);
+// For B ciphertexts sharing one key: acc_{0,1}[j] = sum_i NTT_j(digit_i) * key[i][{0,1}][j] for every eval
+// prime j (Q-prefix nq, then naux aux primes), NTT form.  digits: [B][D][N]; key: [>=D][2][nq+naux][N]
+// NTT-form doubles; part: [dsplit][B][2][nq+naux][N] partial sums over digit ranges.  Eval limbs
+// are processed `group` at a time (column-pass scratch of B*D*group limbs).
+void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
+                      double* part, int dsplit, int group, hipStream_t s);
 // mod_down_eval_to_q (keyswitching.rs:278-322) on coefficient-form accumulators over Q-prefix
 // nq || P (2 polys, u64), optionally adding add0 / add1 (nq limbs each) to the two results.
 void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64_t* add0, const uint64_t* add1,

@@ -1,0 +1,154 @@
+// Compositions of the kernels into the crate's operations, for one ciphertext or a batch of B
+// ciphertexts that share a level (and a key): negacyclic poly product, ct x ct mult, rescale and
+// the keyswitch pipeline.  Shared by the per-op C ABI and the batched executor.
+#include "pipeline.hpp"
+
+namespace rck {
+
+// negacyclic product of `polys` polynomial pairs a[z] * b[z or 0] -> out[z] (NTT resident)
+void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
+                     long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    const long long n = c.n, P = (long long)limbs * n;
+    const int bpolys = b_ps ? polys : 1;
+    double* t = static_cast<double*>(c.workspace(0, (size_t)(polys + bpolys) * P * 8));
+    double* tb = t + (size_t)polys * P;
+    launch_fwd(c, kInU64, a, n, a_ps, t, n, P, limbs, polys, pm, s);
+    launch_fwd(c, kInU64, b, n, b_ps, tb, n, P, limbs, bpolys, pm, s);
+    launch_mul_f64(c, {t, n, P}, {tb, n, b_ps ? P : 0}, {t, n, P}, limbs, polys, pm, s);
+    launch_inv(c, t, n, P, kOutU64, out, n, out_ps, limbs, polys, pm, s);
+}
+
+// B ciphertext products lhs[b] x rhs[b] (ciphertext strides in_ps) -> quadratic out[b] (stride out_ps)
+void mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
+                       uint64_t* out, long long out_ps, hipStream_t s) {
+    const long long n = c.n, P = (long long)limbs * n;
+    PrimeMap pm{limbs, (int)c.L, 0};
+    double* x = static_cast<double*>(c.workspace(0, (size_t)B * 4 * P * 8));
+    double* d = static_cast<double*>(c.workspace(1, (size_t)B * 3 * P * 8));
+    if (c.logn <= 11) {
+        for (int b = 0; b < B; b++) {
+            launch_fwd(c, kInU64, lhs + b * in_ps, n, P, x, n, P, limbs, 2, pm, s);
+            launch_fwd(c, kInU64, rhs + b * in_ps, n, P, x + 2 * P, n, P, limbs, 2, pm, s);
+            launch_tensor(c, x, d, limbs, pm, s);
+            launch_inv(c, d, n, P, kOutU64, out + b * out_ps, n, P, limbs, 3, pm, s);
+        }
+        return;
+    }
+    // forward column pass: lhs[b] -> xl[b] = x[b][0..1], rhs[b] -> xr[b] = x[B + b][0..1]
+    double* xl = x;
+    double* xr = x + (long long)B * 2 * P;
+    if (B == 1 || in_ps == 2 * P) {
+        launch_fwd_cols(c, kInU64, lhs, n, P, xl, n, P, limbs, 2 * B, pm, s);
+        launch_fwd_cols(c, kInU64, rhs, n, P, xr, n, P, limbs, 2 * B, pm, s);
+    } else {
+        for (int b = 0; b < B; b++) {
+            launch_fwd_cols(c, kInU64, lhs + b * in_ps, n, P, xl + b * 2 * P, n, P, limbs, 2, pm, s);
+            launch_fwd_cols(c, kInU64, rhs + b * in_ps, n, P, xr + b * 2 * P, n, P, limbs, 2, pm, s);
+        }
+    }
+    launch_mult_rows(c, xl, xr, d, limbs, B, pm, s);
+    // inverse column pass of every d[b][k] -> out[b][k]
+    if (B == 1 || out_ps == 3 * P) {
+        launch_inv_cols(c, d, n, P, kOutU64, out, n, P, limbs, 3 * B, pm, s);
+    } else {
+        for (int b = 0; b < B; b++)
+            launch_inv_cols(c, d + b * 3 * P, n, P, kOutU64, out + b * out_ps, n, P, limbs, 3, pm, s);
+    }
+}
+
+void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, uint64_t* out, hipStream_t s) {
+    mult_batch(c, lhs, rhs, 0, 1, limbs, out, 0, s);
+}
+
+void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t* out, hipStream_t s) {
+    const long long n = c.n;
+    launch_rescale(c, {in, n, (long long)limbs * n}, {out, n, (long long)(limbs - 1) * n}, limbs, polys, s);
+}
+
+// Digit-MAC launch shape for B ciphertexts at `limbs` limbs: explicit tuning values when set,
+// otherwise a heuristic that keeps >= ~2048 MAC workgroups in flight.
+KsShape ks_auto_shape(const Context& c, int E, int D, int B) {
+    KsShape r;
+    r.group = c.tune_group > 0 ? c.tune_group : E;
+    if (c.logn <= 11) {
+        r.dsplit = 1;
+        return r;
+    }
+    if (c.tune_dsplit > 0) {
+        r.dsplit = std::min(c.tune_dsplit, std::min(8, D));
+        return r;
+    }
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    const int nrb = (1 << n1) / (256 / ((1 << n2) / 16));
+    const long long tiles = (long long)nrb * E * B;
+    const int ds = (int)((2048 + tiles - 1) / tiles);
+    r.dsplit = std::max(1, std::min(ds, std::min(4, D)));
+    return r;
+}
+
+// Keyswitch of B polynomials `poly` (limbs Q-limbs each, poly stride in_ps) with one key; adds
+// add0 / add1 (nullable; element offsets relative to each ciphertext, stride add_ps) and writes
+// out[b] = [c0][c1] at stride out_ps.  Mirrors keyswitch_rns (keyswitching.rs:130-157) and
+// keyswitch_rns_hybrid (:159-194).
+void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
+                     const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
+                     hipStream_t s, KsShape shape) {
+    RCK_REQUIRE((int)outs.size() == B, "one output per ciphertext");
+    check_ptr(key, "key");
+    const long long n = c.n, P = (long long)limbs * n;
+    const uint32_t active = (c.qbits[limbs] + 29) / 30;
+    int levels;
+    if (key->aux == 0) {
+        RCK_REQUIRE((int)key->level == limbs - 1, "keyswitch key level must match the active modulus chain");
+        RCK_REQUIRE(key->digits >= active, "keyswitch key must cover the active modulus chain");
+        levels = key->digits;  // keyswitching.rs:138 decomposes into ksk.ksk.len() digits
+    } else {
+        RCK_REQUIRE((int)key->level == limbs - 1, "keyswitch key level must match the active Q chain");
+        RCK_REQUIRE(key->aux == c.K, "eval-key context must match the keyswitch key auxiliary basis");
+        RCK_REQUIRE(key->digits >= active, "keyswitch key must cover the active modulus chain");
+        RCK_REQUIRE(c.qbits[limbs] > 30, "hybrid keyswitching needs a ciphertext modulus above 2^30");
+        levels = active;  // keyswitching.rs:170
+    }
+    const int naux = key->aux;
+    const int E = limbs + naux;
+    const long long EN = (long long)E * n;
+    const KsShape aut = ks_auto_shape(c, E, levels, B);
+    const int group = shape.group > 0 ? std::min(shape.group, E) : aut.group;
+    const int dsplit = c.logn <= 11 ? 1 : (shape.dsplit > 0 ? std::min(shape.dsplit, std::min(8, levels)) : aut.dsplit);
+    int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
+    double* part = static_cast<double*>(c.workspace(6, (size_t)dsplit * B * 2 * EN * 8));
+    launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps);
+    launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, s);
+    PrimeMap pm{limbs, (int)c.L, 0};
+    const long long sstride = (long long)B * 2 * EN;
+    if (naux) {
+        uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * EN * 8));
+        launch_inv(c, part, n, EN, kOutU64, accu, n, EN, E, 2 * B, pm, s, dsplit, sstride);
+        for (int b = 0; b < B; b++)
+            launch_mod_down(c, accu + b * 2 * EN, limbs, add0 ? add0 + b * add_ps : nullptr,
+                            add1 ? add1 + b * add_ps : nullptr, outs[b], P, s);
+    } else {
+        uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * P * 8));
+        launch_inv(c, part, n, P, kOutU64, accu, n, P, limbs, 2 * B, pm, s, dsplit, sstride);
+        for (int b = 0; b < B; b++) {
+            uint64_t* ab = accu + b * 2 * P;
+            uint64_t* ob = outs[b];
+            if (add0)
+                launch_addsub(c, {add0 + b * add_ps, n, 0}, {ab, n, 0}, {ob, n, 0}, limbs, 1, 0, pm, s);
+            else
+                RCK_CHECK_HIP(hipMemcpyAsync(ob, ab, P * 8, hipMemcpyDeviceToDevice, s));
+            if (add1)
+                launch_addsub(c, {add1 + b * add_ps, n, 0}, {ab + P, n, 0}, {ob + P, n, 0}, limbs, 1, 0, pm, s);
+            else
+                RCK_CHECK_HIP(hipMemcpyAsync(ob + P, ab + P, P * 8, hipMemcpyDeviceToDevice, s));
+        }
+    }
+}
+
+void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const KeyDev* key, const uint64_t* add0,
+                           const uint64_t* add1, uint64_t* out, hipStream_t s) {
+    keyswitch_batch(c, poly, 0, 1, limbs, key, add0, add1, 0, {out}, s);
+}
+
+}  // namespace rck

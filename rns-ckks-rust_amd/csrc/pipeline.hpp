@@ -1,0 +1,43 @@
+// Kernel compositions for the crate's operations (pipeline.hip).
+#pragma once
+#include "ops.hpp"
+
+namespace rck {
+
+// A keyswitching key resident in HBM in NTT form: data[digit][comp][limb][N] (reduced doubles)
+// over level+1 Q-limbs followed by aux limbs (hybrid) or none (Q-only).
+struct KeyDev {
+    double* data = nullptr;
+    uint32_t digits = 0, level = 0, aux = 0, limbs = 0;
+    uint64_t galois = 0;
+};
+
+// Launch shape of the digit MAC: eval limbs per group and digit splits (0 = automatic).
+struct KsShape {
+    int group = 0;
+    int dsplit = 0;
+};
+
+inline void check_ptr(const void* p, const char* what) {
+    RCK_REQUIRE(p != nullptr, std::string("null ") + what);
+}
+
+// negacyclic products a[z] * b[z] (b_ps == 0: the same b for every z) of `polys` polynomials
+void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
+              long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+// B products lhs[b] x rhs[b] (ciphertext stride in_ps) -> quadratic out[b] (stride out_ps)
+void mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
+                uint64_t* out, long long out_ps, hipStream_t s);
+void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, uint64_t* out, hipStream_t s);
+void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t* out, hipStream_t s);
+// keyswitch of B polynomials (stride in_ps) with one key, adding add0/add1 (stride add_ps, nullable)
+// into outs[b] = [c0][c1]
+void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
+                     const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
+                     hipStream_t s, KsShape shape = {});
+void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const KeyDev* key, const uint64_t* add0,
+                    const uint64_t* add1, uint64_t* out, hipStream_t s);
+// the automatic digit-MAC shape for B ciphertexts with E eval limbs and D digits
+KsShape ks_auto_shape(const Context& c, int E, int D, int B);
+
+}  // namespace rck

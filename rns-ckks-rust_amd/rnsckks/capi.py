@@ -58,6 +58,14 @@ EXPORTS = {
     "rck_mult_relinearized": (ctypes.c_int, [vp, vp, vp, u32, u32, u32, vp, vp, u32p, vp]),
     "rck_ct_galois": (ctypes.c_int, [vp, vp, u32, u64, vp, vp]),
     "rck_rotate": (ctypes.c_int, [vp, vp, u32, u32, vp, vp, u32p, vp]),
+    "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+    "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
+    "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
+    "rck_batch_run": (ctypes.c_int, [vp, vp]),
+    "rck_batch_run_eager": (ctypes.c_int, [vp]),
+    "rck_batch_synchronize": (ctypes.c_int, [vp]),
+    "rck_batch_report": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+    "rck_batch_destroy": (ctypes.c_int, [vp]),
     "rck_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "rck_profile_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
 }
@@ -319,3 +327,47 @@ class Context:
 
     def rotate(self, ct, limbs, scale, ksk, out):
         return self._scaled(lib().rck_rotate, self.ptr, _p(ct), limbs, scale, ksk.ptr, _p(out))
+
+
+OP_MULT_RELIN, OP_ROTATE, OP_MULT_PLAIN, OP_MULT_RELIN_RESCALE = 1, 2, 3, 4
+
+
+class Batch:
+    """rck_batch: independent ops planned into grouped launches and captured in one hipGraph."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        p = vp()
+        _check(lib().rck_batch_create(ctx.ptr, ctypes.byref(p)))
+        self.ptr = p.value
+        self._keep = []
+
+    def add(self, op, limbs, in0, in1, key, out):
+        self._keep.append((in0, in1, key, out))
+        _check(lib().rck_batch_add(self.ptr, op, limbs, _p(in0), _p(in1), key.ptr if key is not None else None,
+                                   _p(out)))
+
+    def compile(self, tune=True):
+        _check(lib().rck_batch_compile(self.ptr, 1 if tune else 0))
+
+    def run(self, stream=None):
+        _check(lib().rck_batch_run(self.ptr, stream))
+
+    def run_eager(self):
+        _check(lib().rck_batch_run_eager(self.ptr))
+
+    def synchronize(self):
+        _check(lib().rck_batch_synchronize(self.ptr))
+
+    def report(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 20)
+        _check(lib().rck_batch_report(self.ptr, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and _lib is not None:
+                _lib.rck_batch_destroy(self.ptr)
+        except Exception:
+            pass

@@ -1,0 +1,141 @@
+"""Batched executor: a captured hipGraph over many ciphertexts at uneven levels with mixed
+operations gives exactly the per-op results, on every replay (with eager work in between)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rnsckks import capi as G
+from rnsckks import params as PR
+
+pytestmark = pytest.mark.gpu
+
+
+def uniform(moduli, n, rs, polys):
+    out = np.empty((polys, len(moduli), n), dtype=np.uint64)
+    for l, q in enumerate(moduli):
+        out[:, l] = rs.integers(0, q, size=(polys, n), dtype=np.uint64)
+    return out
+
+
+def dev(a):
+    return G.DeviceBuffer.from_numpy(np.ascontiguousarray(a, dtype=np.uint64))
+
+
+def test_batched_mult_relin_matches_per_op_and_oracle():
+    """B=4 mult+relin at one level sharing one hybrid key, contiguous (strided) operands."""
+    n, L, B = 1 << 13, 8, 4
+    p = PR.realistic_8_level()
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots,
+                    scale_bits=45)
+    qb = O.Basis(n, p.primes, p.roots)
+    eb = O.Basis(n, p.primes + p.aux_primes, p.roots + p.aux_roots)
+    rs = np.random.default_rng(5)
+    D = ctx.num_decomp_digits(L)
+    key = np.stack([uniform(eb.moduli, n, rs, 2) for _ in range(D)])
+    k = ctx.ksk_create(key, L - 1, 2)
+    lhs = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
+    rhs = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
+    dl, dr = dev(lhs), dev(rhs)
+    out = G.DeviceBuffer((B, 2, L, n))
+    bt = G.Batch(ctx)
+    P = L * n
+    for b in range(B):
+        bt.add(G.OP_MULT_RELIN, L, dl.offset(b * 2 * P), dr.offset(b * 2 * P), k, out.offset(b * 2 * P))
+    bt.compile(tune=True)
+    got = out.numpy()
+    one = G.DeviceBuffer((2, L, n))
+    quad = G.DeviceBuffer((3, L, n))
+    for b in range(B):
+        ctx.mult(dl.offset(b * 2 * P), dr.offset(b * 2 * P), L, 45, 45, quad)
+        ctx.relinearize(quad, L, 90, k, one)
+        assert np.array_equal(one.numpy(), got[b]), f"ciphertext {b}"
+    want0 = O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs[0], rhs[0]), key)
+    assert np.array_equal(got[0], want0)
+    # replays: clear the outputs between launches with eager work, every replay must match
+    for _ in range(3):
+        out.upload(np.zeros((B, 2, L, n), dtype=np.uint64))
+        ctx.mult(dl, dr, L, 45, 45, quad)  # eager work on the context between replays
+        bt.run()
+        bt.synchronize()
+        assert np.array_equal(out.numpy(), got)
+    rep = bt.report()
+    assert rep["ops"] == B and len(rep["groups"]) == 1 and rep["groups"][0]["ops"] == B
+
+
+def test_batched_mixed_levels_and_ops_match_per_op():
+    """Config-3 shape in miniature: ciphertexts at levels 4..8 limbs (primes just above 2^50),
+    40% mult+relin, 40% rotation by +-2^k, 20% ct x pt, scattered (staged) operands."""
+    n = 1 << 12
+    p = PR.config_batched(n=1 << 15, limbs=8, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    rs = np.random.default_rng(17)
+    eprimes = p.primes + p.aux_primes
+    keys = {}
+
+    def key_for(limbs, galois):
+        kk = (limbs, galois)
+        if kk not in keys:
+            D = ctx.num_decomp_digits(limbs)
+            moduli = p.primes[:limbs] + p.aux_primes
+            kd = np.stack([uniform(moduli, n, rs, 2) for _ in range(D)])
+            keys[kk] = ctx.ksk_create(kd, limbs - 1, 2, galois)
+        return keys[kk]
+
+    bt = G.Batch(ctx)
+    ops = []
+    mix = rs.permutation(["mr"] * 8 + ["rot"] * 8 + ["pt"] * 4)
+    bufs = []
+    for t, kind in enumerate(mix):
+        limbs = int(rs.integers(4, 9))
+        a = dev(uniform(p.primes[:limbs], n, rs, 2))
+        out = G.DeviceBuffer((2, limbs, n))
+        if kind == "mr":
+            b = dev(uniform(p.primes[:limbs], n, rs, 2))
+            k = key_for(limbs, 0)
+            bt.add(G.OP_MULT_RELIN, limbs, a, b, k, out)
+        elif kind == "rot":
+            step = int(rs.choice([1, 2, 4, 8, 1024])) * int(rs.choice([-1, 1]))
+            gel = G.galois_element_for_rotation(n, step)
+            b, k = None, key_for(limbs, gel)
+            bt.add(G.OP_ROTATE, limbs, a, None, k, out)
+        else:
+            b, k = dev(uniform(p.primes[:limbs], n, rs, 1)[0]), None
+            bt.add(G.OP_MULT_PLAIN, limbs, a, b, None, out)
+        ops.append((kind, limbs, a, b, k, out))
+        bufs.append((a, b, out))
+    bt.compile(tune=True)
+    first = [o[5].numpy() for o in ops]
+    for rep in range(2):
+        for o in ops:
+            o[5].upload(np.zeros(o[5].shape, dtype=np.uint64))
+        bt.run()
+        bt.synchronize()
+        for i, o in enumerate(ops):
+            assert np.array_equal(o[5].numpy(), first[i]), f"replay {rep} op {i}"
+    for i, (kind, limbs, a, b, k, out) in enumerate(ops):
+        ref = G.DeviceBuffer((2, limbs, n))
+        if kind == "mr":
+            quad = G.DeviceBuffer((3, limbs, n))
+            ctx.mult(a, b, limbs, 40, 40, quad)
+            ctx.relinearize(quad, limbs, 80, k, ref)
+        elif kind == "rot":
+            ctx.rotate(a, limbs, 40, k, ref)
+        else:
+            ctx.mult_plain(a, b, limbs, 40, ref)
+        assert np.array_equal(ref.numpy(), first[i]), f"op {i} ({kind}, {limbs} limbs)"
+
+
+def test_batch_rejects_bad_ops():
+    n = 1 << 12
+    p = PR.config_batched(n=1 << 15, limbs=4, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    bt = G.Batch(ctx)
+    a = G.DeviceBuffer((2, 4, n))
+    with pytest.raises(G.RckError, match="unknown batch op kind"):
+        bt.add(9, 4, a, a, None, a)
+    rs = np.random.default_rng(1)
+    D = ctx.num_decomp_digits(3)
+    k = ctx.ksk_create(np.stack([uniform(p.primes[:3] + p.aux_primes, n, rs, 2) for _ in range(D)]), 2, 2, 5)
+    bt.add(G.OP_ROTATE, 4, a, None, k, G.DeviceBuffer((2, 4, n)))
+    with pytest.raises(G.RckError, match="rotation key level must match the ciphertext level"):
+        bt.compile()
