@@ -33,13 +33,14 @@ def uniform(moduli, n, rs, polys):
     return out
 
 
-def time_it(G, fn, steps):
+def time_it(G, fn, steps, stream=None):
+    """HIP events on the stream the work is launched on, bracketing exactly `steps` steps."""
     e0, e1 = G.Event(), G.Event()
     G.synchronize()
-    e0.record()
+    e0.record(stream.ptr if stream is not None else None)
     for _ in range(steps):
         fn()
-    e1.record()
+    e1.record(stream.ptr if stream is not None else None)
     e1.synchronize()
     G.synchronize()
     return e0.elapsed_ms(e1)
@@ -90,6 +91,10 @@ def main():
         ctx.mult(lhs, rhs, L, 40, 40, quad)
         ctx.relinearize(quad, L, 80, ksk, out)
 
+    # autotune the per-op keyswitch launch shape (results are identical for every shape)
+    tune1 = ctx.tune_keyswitch(L, ksk, 1)
+    log(f"[bench] per-op keyswitch tuned: {tune1}")
+
     # per-op path (always measured: the batched form is reported against it)
     for _ in range(args.warmup):
         per_op_step()
@@ -108,12 +113,14 @@ def main():
         report = bt.report()
         log(f"[bench] batch compiled+tuned in {time.perf_counter() - t:.1f} s: {report}")
 
+        stream = G.Stream()
+
         def step():
-            bt.run(None)
+            bt.run(stream)
 
         for _ in range(args.warmup):
             step()
-        ms = time_it(G, step, args.steps)
+        ms = time_it(G, step, args.steps, stream)
         ms_per_step = ms / args.steps
         value = B * 1000.0 / ms_per_step
         log(f"[bench] batched: {ms_per_step:.3f} ms per batch of {B} = {value:.1f} ops/s "
@@ -140,10 +147,7 @@ def main():
     if name == "digit_mac":
         launches = max(1, cnt // prof_steps)
         G_ = -(-(L + K) // launches)  # eval limbs per launch
-        if report:
-            ds = report["groups"][0]["digit_splits"]
-        else:  # the library's automatic shape for one ciphertext: ~2048 MAC workgroups (16 row tiles/limb)
-            ds = max(1, min(4, -(-2048 // (16 * (L + K) * B))))
+        ds = report["groups"][0]["digit_splits"] if report else tune1["digit_splits"]
         alg = mac_bytes(n, D, G_, B, ds)
         achieved = alg / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -163,7 +167,7 @@ def main():
                                + (f", {B} ciphertext pairs per step sharing one relin key" if B > 1 else ""),
                    "mode": args.mode, "batch": B, "per_op_ops_s": round(per_op_value, 2),
                    "batched_vs_per_op": round(value / per_op_value, 3)},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "tuning": {"per_op": tune1, "batched": report},
         "kernels_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in top},
     }
     print(json.dumps(line), flush=True)

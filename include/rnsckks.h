@@ -175,6 +175,13 @@ int rck_batch_synchronize(rck_batch* batch);
 int rck_batch_report(rck_batch* batch, char* buf, size_t len);
 int rck_batch_destroy(rck_batch* batch);
 
+/* Autotuner for the per-op path: times the candidate digit-MAC launch shapes (eval-limb grouping,
+ * digit splits, rows per workgroup, key prefetch) of a keyswitch of `batch` ciphertexts at `limbs`
+ * limbs with `key`, and records the fastest in the process-wide tuning table that rck_keyswitch,
+ * rck_relinearize, rck_rotate and batches consult.  buf (optional) receives the choice as JSON.
+ * Results are identical for every shape; only the launch geometry changes. */
+int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uint32_t batch, char* buf, size_t len);
+
 /* ---- diagnostics (no counterpart in the crate) ----
  * Per-kernel timing: while enabled, every kernel launch is bracketed by HIP events on the stream it
  * is launched on.  rck_profile_report synchronizes the device and writes a JSON object

@@ -43,9 +43,6 @@ struct BatchGroup {
 
 enum { kSlotLhs = 10, kSlotRhs = 11, kSlotQuad = 12, kSlotTmp = 13 };
 
-// (logn, limbs, eval limbs, digits, B) -> tuned shape, shared by every batch of a process
-std::map<std::tuple<int, int, int, int, int>, std::pair<KsShape, float>> g_tuned;
-
 }  // namespace
 
 struct rck_batch {
@@ -157,40 +154,18 @@ static void check_group(const Context& c, const BatchGroup& g) {
 
 static bool has_keyswitch(int kind) { return kind != RCK_OP_MULT_PLAIN; }
 
-// Candidate digit-MAC shapes for a group: eval-limb grouping x digit splits.
-static std::vector<KsShape> candidates(const Context& c, int E, int D) {
-    std::vector<KsShape> out;
-    if (c.logn <= 11) {
-        out.push_back({E, 1});
-        return out;
-    }
-    std::vector<int> gs = {E, (E + 1) / 2, (E + 3) / 4, 1};
-    std::sort(gs.begin(), gs.end());
-    gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
-    for (int g : gs)
-        for (int ds : {1, 2, 4})
-            if (ds <= D) out.push_back({g, ds});
-    return out;
-}
-
 static void autotune(rck_batch* bt, BatchGroup& g) {
     Context& c = *bt->sc;
     const int E = (int)g.limbs + (int)g.key->aux;
     const int D = g.key->aux ? (int)((c.qbits[g.limbs] + 29) / 30) : (int)g.key->digits;
     const int B = (int)g.ops.size();
-    auto tk = std::make_tuple((int)c.logn, (int)g.limbs, E, D, B);
-    auto it = g_tuned.find(tk);
-    if (it != g_tuned.end()) {
-        g.shape = it->second.first;
-        g.tuned_ms = it->second.second;
-        return;
-    }
+    if (ks_tuned_find(c, (int)g.limbs, E, D, B, &g.shape, &g.tuned_ms)) return;
     hipEvent_t e0, e1;
     RCK_CHECK_HIP(hipEventCreate(&e0));
     RCK_CHECK_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     KsShape best_shape{};
-    for (const KsShape& sh : candidates(c, E, D)) {
+    for (const KsShape& sh : ks_candidates(c, E, D, B)) {
         run_group(bt, g, bt->stream, sh);  // warm-up (sizes the scratch)
         RCK_CHECK_HIP(hipEventRecord(e0, bt->stream));
         const int reps = 2;
@@ -209,10 +184,92 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
     (void)hipEventDestroy(e1);
     g.shape = best_shape;
     g.tuned_ms = best;
-    g_tuned[tk] = {best_shape, best};
+    ks_tuned_store(c, (int)g.limbs, E, D, B, best_shape, best);
+}
+
+// Per-op tuning: time every candidate shape of keyswitch_batch for B ciphertexts at `limbs` limbs
+// with `key` (on zero operands: the kernels' timing does not depend on the residues) and store the
+// fastest in the process-wide table that per-op calls consult.
+static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsShape* best_out, float* best_ms) {
+    const long long n = c.n, P = (long long)limbs * n;
+    const int E = limbs + (int)key->aux;
+    const int D = key->aux ? (int)((c.qbits[limbs] + 29) / 30) : (int)key->digits;
+    uint64_t *in = nullptr, *out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        RCK_CHECK_HIP(hipMalloc(&in, (size_t)B * P * 8));
+        RCK_CHECK_HIP(hipMalloc(&out, (size_t)B * 2 * P * 8));
+        RCK_CHECK_HIP(hipMemset(in, 0, (size_t)B * P * 8));
+        RCK_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        RCK_CHECK_HIP(hipEventCreate(&e0));
+        RCK_CHECK_HIP(hipEventCreate(&e1));
+        std::vector<uint64_t*> outs;
+        for (int b = 0; b < B; b++) outs.push_back(out + b * 2 * P);
+        float best = 1e30f;
+        KsShape bs{};
+        for (const KsShape& sh : ks_candidates(c, E, D, B)) {
+            keyswitch_batch(c, in, P, B, limbs, key, nullptr, nullptr, 0, outs, s, sh);
+            RCK_CHECK_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < 2; r++) keyswitch_batch(c, in, P, B, limbs, key, nullptr, nullptr, 0, outs, s, sh);
+            RCK_CHECK_HIP(hipEventRecord(e1, s));
+            RCK_CHECK_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            RCK_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / 2 < best) {
+                best = ms / 2;
+                bs = sh;
+            }
+        }
+        ks_tuned_store(c, limbs, E, D, B, bs, best);
+        *best_out = bs;
+        *best_ms = best;
+    } catch (...) {
+        if (in) (void)hipFree(in);
+        if (out) (void)hipFree(out);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+        throw;
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(in);
+    (void)hipFree(out);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
 }
 
 extern "C" {
+
+int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uint32_t batch, char* buf, size_t len) {
+    return bguard([&] {
+        RCK_REQUIRE(ctx && ctx->c && key, "null argument");
+        RCK_REQUIRE(limbs >= 1 && limbs <= ctx->c->L && (int)key->level == (int)limbs - 1,
+                    "keyswitch key level must match the limb count");
+        RCK_REQUIRE(batch >= 1, "batch must be positive");
+        const bool prof = prof_enabled();
+        prof_set(false);
+        KsShape sh;
+        float ms = 0;
+        try {
+            tune_keyswitch(*ctx->c, (int)limbs, key, (int)batch, &sh, &ms);
+        } catch (...) {
+            prof_set(prof);
+            throw;
+        }
+        prof_set(prof);
+        if (buf) {
+            std::ostringstream o;
+            o << "{\"limbs\": " << limbs << ", \"batch\": " << batch << ", \"eval_limb_group\": " << sh.group
+              << ", \"digit_splits\": " << sh.dsplit << ", \"mac_rows\": " << sh.rows << ", \"key_prefetch\": "
+              << sh.prefetch << ", \"keyswitch_ms\": " << ms << "}";
+            std::string r = o.str();
+            RCK_REQUIRE(len > r.size(), "report buffer too small");
+            std::memcpy(buf, r.c_str(), r.size() + 1);
+        }
+    });
+}
 
 int rck_batch_create(rck_context* ctx, rck_batch** out) {
     return bguard([&] {
@@ -325,7 +382,8 @@ int rck_batch_report(rck_batch* bt, char* buf, size_t len) {
             const auto& g = bt->groups[i];
             o << (i ? ", " : "") << "{\"kind\": " << g.kind << ", \"limbs\": " << g.limbs
               << ", \"ops\": " << g.ops.size() << ", \"eval_limb_group\": " << g.shape.group
-              << ", \"digit_splits\": " << g.shape.dsplit << ", \"tuned_ms\": " << g.tuned_ms << "}";
+              << ", \"digit_splits\": " << g.shape.dsplit << ", \"mac_rows\": " << g.shape.rows
+              << ", \"key_prefetch\": " << g.shape.prefetch << ", \"tuned_ms\": " << g.tuned_ms << "}";
         }
         o << "], \"capture_ms\": " << bt->capture_ms << "}";
         std::string r = o.str();

@@ -57,22 +57,29 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     for (int i = nq - 2; i >= 0; i--) {
         const uint64_t q = (uint64_t)primes[i].q;
         const uint32_t qlo = (uint32_t)q, qhi = (uint32_t)(q >> 32);
+        // X * q_i + m_i < q_i ... q_{nq-1} < 2^{51 (nq - i)}: words beyond `len` stay zero
+        // (len is wave-uniform, so the skipped words cost no issue slots)
+        const int len = min(W, (51 * (nq - i) + 31) / 32 + 1);
         uint64_t carry = 0;
         uint32_t y[W];
 #pragma unroll
         for (int k = 0; k < W; k++) {  // Y = X * qlo  (t < 2^64 - 2^32)
-            uint64_t t = (uint64_t)x[k] * qlo + carry;
-            y[k] = (uint32_t)t;
-            carry = t >> 32;
+            if (k < len) {
+                uint64_t t = (uint64_t)x[k] * qlo + carry;
+                y[k] = (uint32_t)t;
+                carry = t >> 32;
+            }
         }
         carry = (uint64_t)m[i * kDecBS + tid];  // + m_i (t < 2^53 below)
         uint32_t prev = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {  // X = Y + (X * qhi) << 32 + m_i
-            uint64_t t = (uint64_t)y[k] + (uint64_t)prev * qhi + carry;
-            prev = x[k];
-            x[k] = (uint32_t)t;
-            carry = t >> 32;
+            if (k < len) {
+                uint64_t t = (uint64_t)y[k] + (uint64_t)prev * qhi + carry;
+                prev = x[k];
+                x[k] = (uint32_t)t;
+                carry = t >> 32;
+            }
         }
     }
     // centring: X > floor(Q/2)  ->  X - Q
@@ -159,14 +166,15 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 // served to the other ciphertexts of the batch from that XCD's L2.
 // Output: partial sums part[s][b][comp][j][N] per digit split s (reduced doubles).
 // ---------------------------------------------------------------------------------------
-template <int LOGR>
-__global__ __launch_bounds__(256) void k_digit_mac(const double* __restrict__ cols, int D, int G, int B, int dsplit,
-                                                   int ntiles, const double* __restrict__ key, int E, int j0,
-                                                   double* __restrict__ part, int logn, PrimeMap pm,
-                                                   const DevPrime* __restrict__ primes) {
-    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
-    __shared__ double lds[S * (R + R / 16 + 1)];
-    const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / S;
+template <int LOGR, int ROWS, bool PREFETCH>
+__global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const double* __restrict__ cols, int D, int G,
+                                                                    int B, int dsplit, int ntiles,
+                                                                    const double* __restrict__ key, int E, int j0,
+                                                                    double* __restrict__ part, int logn, PrimeMap pm,
+                                                                    const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16;
+    __shared__ double lds[ROWS * (R + R / 16 + 1)];
+    const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / ROWS;
     const int g = blockIdx.x, xcd = g & 7, slot = g >> 3;
     const int cb = slot % B;
     const int tile = (slot / B) * 8 + xcd;
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(256) void k_digit_mac(const double* __restrict__ co
     const int j = j0 + jl;
     const int i0 = (int)((long long)D * sp / dsplit), i1 = (int)((long long)D * (sp + 1) / dsplit);
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
-    const int b = rb * S + row;
+    const int b = rb * ROWS + row;
     const DevPrime P = primes[pm(j)];
     const double q = P.q, qinv = P.qinv;
     const long long off = (long long)b * R;
@@ -185,9 +193,18 @@ __global__ __launch_bounds__(256) void k_digit_mac(const double* __restrict__ co
     for (int e = 0; e < 16; e++) a0[e] = a1[e] = 0.0;
     for (int i = i0; i < i1; i++) {
         const double* p = cols + (((long long)cb * D + i) * G + jl) * n + off;
-        double v[16];
+        const double* k0 = key + (2LL * i) * EN + (long long)j * n + off + 16 * tau;
+        const double* k1 = k0 + EN;
+        double v[16], w0[PREFETCH ? 16 : 1], w1[PREFETCH ? 16 : 1];
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                w0[e] = k0[e];
+                w1[e] = k1[e];
+            }
+        }
         fwd_phase_a(v, P.fw, N1 + b, q, qinv);
         __syncthreads();
 #pragma unroll
@@ -196,12 +213,15 @@ __global__ __launch_bounds__(256) void k_digit_mac(const double* __restrict__ co
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
         fwd_phase_b<LOGR>(v, P.fw, N1 + b, tau, q, qinv);
-        const double* k0 = key + (2LL * i) * EN + (long long)j * n + off + 16 * tau;
-        const double* k1 = k0 + EN;
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-            a0[e] += mulmod(v[e], k0[e], q, qinv);
-            a1[e] += mulmod(v[e], k1[e], q, qinv);
+            if constexpr (PREFETCH) {
+                a0[e] += mulmod(v[e], w0[e], q, qinv);
+                a1[e] += mulmod(v[e], w1[e], q, qinv);
+            } else {
+                a0[e] += mulmod(v[e], k0[e], q, qinv);
+                a1[e] += mulmod(v[e], k1[e], q, qinv);
+            }
         }
         if (((i - i0) & 3) == 3) {
             reduce_all(a0, q, qinv);
@@ -219,17 +239,33 @@ __global__ __launch_bounds__(256) void k_digit_mac(const double* __restrict__ co
     }
 }
 
-template <int LOGR>
+template <int LOGR, int ROWS, bool PF>
 static void mac_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
                   int j0, double* part, PrimeMap pm, hipStream_t s) {
-    constexpr int S = 256 / ((1 << LOGR) / 16);
-    const int nrb = (1 << (c.logn - LOGR)) / S;
+    constexpr int TPS = (1 << LOGR) / 16;
+    const int nrb = (1 << (c.logn - LOGR)) / ROWS;
     const int ntiles = nrb * G * dsplit;
     const int blocks = ((ntiles + 7) / 8) * 8 * B;
     ProfScope ps(s, "digit_mac");
-    k_digit_mac<LOGR><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, dsplit, ntiles, key, E, j0, part, c.logn, pm,
-                                                    c.d_primes);
+    k_digit_mac<LOGR, ROWS, PF><<<dim3(blocks), ROWS * TPS, 0, s>>>(cols, D, G, B, dsplit, ntiles, key, E, j0,
+                                                                   part, c.logn, pm, c.d_primes);
     RCK_CHECK_HIP(hipGetLastError());
+}
+
+template <int LOGR>
+static void mac_v(const Context& c, int rows, bool pf, const double* cols, int D, int G, int B, int dsplit,
+                  const double* key, int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
+    if (rows <= 4) {
+        if (pf)
+            mac_t<LOGR, 4, true>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+        else
+            mac_t<LOGR, 4, false>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    } else {
+        if (pf)
+            mac_t<LOGR, 16, true>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+        else
+            mac_t<LOGR, 16, false>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    }
 }
 
 // Small rings: plain forward NTT of every digit for every prime, then pointwise MAC.
@@ -256,7 +292,7 @@ __global__ void k_mac_small(const double* __restrict__ dn, int D, const double* 
 }
 
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
-                      double* part, int dsplit, int group, hipStream_t s) {
+                      double* part, int dsplit, int group, int rows, bool prefetch, hipStream_t s) {
     const int E = nq + naux;
     const long long n = c.n, EN = (long long)E * n;
     PrimeMap pm{nq, (int)c.L, 0};
@@ -282,9 +318,9 @@ void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, con
         launch_fwd_cols(c, kInI32, digits, 0, n, cols, n, (long long)g * n, g, B * D, PrimeMap{nq, (int)c.L, j0},
                         s);
         switch (n2) {
-            case 6: mac_t<6>(c, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
-            case 7: mac_t<7>(c, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
-            case 8: mac_t<8>(c, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 6: mac_v<6>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 7: mac_v<7>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 8: mac_v<8>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
             default: RCK_REQUIRE(false, "unsupported ring degree");
         }
     }

@@ -1,6 +1,10 @@
 // Compositions of the kernels into the crate's operations, for one ciphertext or a batch of B
 // ciphertexts that share a level (and a key): negacyclic poly product, ct x ct mult, rescale and
 // the keyswitch pipeline.  Shared by the per-op C ABI and the batched executor.
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "pipeline.hpp"
 
 namespace rck {
@@ -67,9 +71,33 @@ void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t
 
 // Digit-MAC launch shape for B ciphertexts at `limbs` limbs: explicit tuning values when set,
 // otherwise a heuristic that keeps >= ~2048 MAC workgroups in flight.
-KsShape ks_auto_shape(const Context& c, int E, int D, int B) {
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, std::pair<KsShape, float>> g_tuned;
+}  // namespace
+
+void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape sh, float ms) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple((int)c.logn, limbs, E, D, B)] = {sh, ms};
+}
+
+bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* sh, float* ms) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple((int)c.logn, limbs, E, D, B));
+    if (it == g_tuned.end()) return false;
+    *sh = it->second.first;
+    if (ms) *ms = it->second.second;
+    return true;
+}
+
+KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B) {
     KsShape r;
+    if (ks_tuned_find(c, limbs, E, D, B, &r, nullptr)) return r;
     r.group = c.tune_group > 0 ? c.tune_group : E;
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    r.rows = (1 << n2) / 16 >= 16 ? 4 : 16;  // at least one full wave per MAC workgroup
+    r.prefetch = 0;
     if (c.logn <= 11) {
         r.dsplit = 1;
         return r;
@@ -78,13 +106,29 @@ KsShape ks_auto_shape(const Context& c, int E, int D, int B) {
         r.dsplit = std::min(c.tune_dsplit, std::min(8, D));
         return r;
     }
-    int n1, n2;
-    split_logn(c.logn, &n1, &n2);
-    const int nrb = (1 << n1) / (256 / ((1 << n2) / 16));
+    const int nrb = (1 << n1) / r.rows;
     const long long tiles = (long long)nrb * E * B;
     const int ds = (int)((2048 + tiles - 1) / tiles);
     r.dsplit = std::max(1, std::min(ds, std::min(4, D)));
     return r;
+}
+
+std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
+    std::vector<KsShape> out;
+    if (c.logn <= 11) {
+        out.push_back({E, 1, 4, 0});
+        return out;
+    }
+    std::vector<int> gs = {E, (E + 1) / 2, (E + 3) / 4, 4, 1};
+    std::sort(gs.begin(), gs.end());
+    gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+    for (int g : gs)
+        for (int ds : {1, 2, 4})
+            for (int rows : {4, 16})
+                for (int pf : {0, 1})
+                    if (ds <= D && g >= 1 && g <= E) out.push_back({g, ds, rows, pf});
+    (void)B;
+    return out;
 }
 
 // Keyswitch of B polynomials `poly` (limbs Q-limbs each, poly stride in_ps) with one key; adds
@@ -113,13 +157,15 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     const int naux = key->aux;
     const int E = limbs + naux;
     const long long EN = (long long)E * n;
-    const KsShape aut = ks_auto_shape(c, E, levels, B);
+    const KsShape aut = ks_auto_shape(c, limbs, E, levels, B);
     const int group = shape.group > 0 ? std::min(shape.group, E) : aut.group;
     const int dsplit = c.logn <= 11 ? 1 : (shape.dsplit > 0 ? std::min(shape.dsplit, std::min(8, levels)) : aut.dsplit);
+    const int rows = shape.rows > 0 ? shape.rows : aut.rows;
+    const bool prefetch = (shape.prefetch >= 0 ? shape.prefetch : aut.prefetch) != 0;
     int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
     double* part = static_cast<double*>(c.workspace(6, (size_t)dsplit * B * 2 * EN * 8));
     launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps);
-    launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, s);
+    launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, rows, prefetch, s);
     PrimeMap pm{limbs, (int)c.L, 0};
     const long long sstride = (long long)B * 2 * EN;
     if (naux) {

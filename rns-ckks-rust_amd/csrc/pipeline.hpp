@@ -14,8 +14,10 @@ struct KeyDev {
 
 // Launch shape of the digit MAC: eval limbs per group and digit splits (0 = automatic).
 struct KsShape {
-    int group = 0;
-    int dsplit = 0;
+    int group = 0;     // eval limbs per column-pass/MAC launch
+    int dsplit = 0;    // digit ranges accumulated by separate workgroups
+    int rows = 0;      // ring rows per MAC workgroup (4 or 16)
+    int prefetch = -1; // 1: key rows loaded before the NTT stages of each digit
 };
 
 inline void check_ptr(const void* p, const char* what) {
@@ -37,7 +39,13 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
                      hipStream_t s, KsShape shape = {});
 void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const KeyDev* key, const uint64_t* add0,
                     const uint64_t* add1, uint64_t* out, hipStream_t s);
-// the automatic digit-MAC shape for B ciphertexts with E eval limbs and D digits
-KsShape ks_auto_shape(const Context& c, int E, int D, int B);
+// the automatic digit-MAC shape for B ciphertexts with E eval limbs and D digits: a tuned entry
+// when one exists for (N, limbs, E, D, B), otherwise a heuristic
+KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B);
+// tuned-shape table shared by the per-op calls and every batch of the process
+void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape s, float ms);
+bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* s, float* ms);
+// candidates the autotuner times for a shape
+std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B);
 
 }  // namespace rck

@@ -58,6 +58,7 @@ EXPORTS = {
     "rck_mult_relinearized": (ctypes.c_int, [vp, vp, vp, u32, u32, u32, vp, vp, u32p, vp]),
     "rck_ct_galois": (ctypes.c_int, [vp, vp, u32, u64, vp, vp]),
     "rck_rotate": (ctypes.c_int, [vp, vp, u32, u32, vp, vp, u32p, vp]),
+    "rck_tune_keyswitch": (ctypes.c_int, [vp, u32, vp, u32, ctypes.c_char_p, ctypes.c_size_t]),
     "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
     "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -151,6 +152,25 @@ class DeviceBuffer:
 
 def synchronize():
     _check(lib().rck_device_synchronize())
+
+
+class Stream:
+    """A non-blocking HIP stream (rck_stream_create)."""
+
+    def __init__(self):
+        p = vp()
+        _check(lib().rck_stream_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def synchronize(self):
+        _check(lib().rck_stream_synchronize(self.ptr))
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.rck_stream_destroy(self.ptr)
+        except Exception:
+            pass
 
 
 class Event:
@@ -310,6 +330,12 @@ class Context:
             _check(lib().rck_ksk_create(self.ptr, key.ctypes.data, digits, level, aux, galois, ctypes.byref(out)))
         return Ksk(out.value, digits, level, aux, galois)
 
+    def tune_keyswitch(self, limbs, ksk, batch=1):
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().rck_tune_keyswitch(self.ptr, limbs, ksk.ptr, batch, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def keyswitch(self, poly, limbs, ksk, out):
         return self._scaled(lib().rck_keyswitch, self.ptr, _p(poly), limbs, ksk.ptr, _p(out))
 
@@ -351,7 +377,8 @@ class Batch:
         _check(lib().rck_batch_compile(self.ptr, 1 if tune else 0))
 
     def run(self, stream=None):
-        _check(lib().rck_batch_run(self.ptr, stream))
+        """Launch the captured graph on `stream` (a Stream; None = the batch's own stream)."""
+        _check(lib().rck_batch_run(self.ptr, stream.ptr if isinstance(stream, Stream) else stream))
 
     def run_eager(self):
         _check(lib().rck_batch_run_eager(self.ptr))
