@@ -93,7 +93,9 @@ def main():
 
     # autotune the per-op keyswitch launch shape (results are identical for every shape)
     tune1 = ctx.tune_keyswitch(L, ksk, 1)
-    log(f"[bench] per-op keyswitch tuned: {tune1}")
+    log(f"[bench] per-op keyswitch tuned: { {k: v for k, v in tune1.items() if k != 'candidates'} }")
+    for cand in sorted(tune1["candidates"], key=lambda c: c["ms"])[:8]:
+        log(f"[bench]     candidate {cand}")
 
     # per-op path (always measured: the batched form is reported against it)
     for _ in range(args.warmup):
@@ -111,7 +113,11 @@ def main():
         t = time.perf_counter()
         bt.compile(tune=True)
         report = bt.report()
-        log(f"[bench] batch compiled+tuned in {time.perf_counter() - t:.1f} s: {report}")
+        log(f"[bench] batch compiled+tuned in {time.perf_counter() - t:.1f} s")
+        for grp in report["groups"]:
+            log(f"[bench]   group { {k: v for k, v in grp.items() if k != 'candidates'} }")
+            for cand in sorted(grp["candidates"], key=lambda c: c["ms"])[:8]:
+                log(f"[bench]     candidate {cand}")
 
         stream = G.Stream()
 

@@ -39,6 +39,7 @@ struct BatchGroup {
     std::vector<int> ops;
     KsShape shape;
     float tuned_ms = 0;
+    std::string tune_log;
 };
 
 enum { kSlotLhs = 10, kSlotRhs = 11, kSlotQuad = 12, kSlotTmp = 13 };
@@ -152,6 +153,13 @@ static void check_group(const Context& c, const BatchGroup& g) {
     }
 }
 
+static std::string shape_json(const KsShape& sh, float ms) {
+    std::ostringstream o;
+    o << "{\"eval_limb_group\": " << sh.group << ", \"digit_splits\": " << sh.dsplit << ", \"mac_rows\": "
+      << sh.rows << ", \"load_schedule\": " << sh.prefetch << ", \"ms\": " << ms << "}";
+    return o.str();
+}
+
 static bool has_keyswitch(int kind) { return kind != RCK_OP_MULT_PLAIN; }
 
 static void autotune(rck_batch* bt, BatchGroup& g) {
@@ -175,6 +183,7 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
         float ms = 0;
         RCK_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
         ms /= reps;
+        g.tune_log += (g.tune_log.empty() ? "" : ", ") + shape_json(sh, ms);
         if (ms < best) {
             best = ms;
             best_shape = sh;
@@ -190,7 +199,8 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
 // Per-op tuning: time every candidate shape of keyswitch_batch for B ciphertexts at `limbs` limbs
 // with `key` (on zero operands: the kernels' timing does not depend on the residues) and store the
 // fastest in the process-wide table that per-op calls consult.
-static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsShape* best_out, float* best_ms) {
+static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsShape* best_out, float* best_ms,
+                           std::string* log) {
     const long long n = c.n, P = (long long)limbs * n;
     const int E = limbs + (int)key->aux;
     const int D = key->aux ? (int)((c.qbits[limbs] + 29) / 30) : (int)key->digits;
@@ -216,6 +226,7 @@ static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsSh
             RCK_CHECK_HIP(hipEventSynchronize(e1));
             float ms = 0;
             RCK_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (log) *log += (log->empty() ? "" : ", ") + shape_json(sh, ms / 2);
             if (ms / 2 < best) {
                 best = ms / 2;
                 bs = sh;
@@ -252,8 +263,9 @@ int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uin
         prof_set(false);
         KsShape sh;
         float ms = 0;
+        std::string log;
         try {
-            tune_keyswitch(*ctx->c, (int)limbs, key, (int)batch, &sh, &ms);
+            tune_keyswitch(*ctx->c, (int)limbs, key, (int)batch, &sh, &ms, &log);
         } catch (...) {
             prof_set(prof);
             throw;
@@ -263,7 +275,7 @@ int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uin
             std::ostringstream o;
             o << "{\"limbs\": " << limbs << ", \"batch\": " << batch << ", \"eval_limb_group\": " << sh.group
               << ", \"digit_splits\": " << sh.dsplit << ", \"mac_rows\": " << sh.rows << ", \"key_prefetch\": "
-              << sh.prefetch << ", \"keyswitch_ms\": " << ms << "}";
+              << sh.prefetch << ", \"keyswitch_ms\": " << ms << ", \"candidates\": [" << log << "]}";
             std::string r = o.str();
             RCK_REQUIRE(len > r.size(), "report buffer too small");
             std::memcpy(buf, r.c_str(), r.size() + 1);
@@ -383,7 +395,8 @@ int rck_batch_report(rck_batch* bt, char* buf, size_t len) {
             o << (i ? ", " : "") << "{\"kind\": " << g.kind << ", \"limbs\": " << g.limbs
               << ", \"ops\": " << g.ops.size() << ", \"eval_limb_group\": " << g.shape.group
               << ", \"digit_splits\": " << g.shape.dsplit << ", \"mac_rows\": " << g.shape.rows
-              << ", \"key_prefetch\": " << g.shape.prefetch << ", \"tuned_ms\": " << g.tuned_ms << "}";
+              << ", \"load_schedule\": " << g.shape.prefetch << ", \"tuned_ms\": " << g.tuned_ms
+              << ", \"candidates\": [" << g.tune_log << "]}";
         }
         o << "], \"capture_ms\": " << bt->capture_ms << "}";
         std::string r = o.str();

@@ -20,7 +20,7 @@ namespace rck {
 // The balanced digits of a value mod 2^{30 levels} are unique, so this equals the reference's
 // base-2^30 digit arithmetic bit for bit.
 // ---------------------------------------------------------------------------------------
-constexpr int kDecBS = 128;
+constexpr int kDecBS = 64;
 
 template <int W>
 __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict__ poly, long long ps, int n, int nq,
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
                                                       const double* __restrict__ invq, int T,
                                                       const uint32_t* __restrict__ qw, const uint32_t* __restrict__ hw,
                                                       int32_t* __restrict__ digits) {
-    extern __shared__ double m[];  // [nq][kDecBS]
+    extern __shared__ double m[];  // [nq][kDecBS] doubles, then [W + 1][kDecBS] words
     const int tid = threadIdx.x;
     const long long c = (long long)blockIdx.x * kDecBS + tid;
     if (c >= n) return;
@@ -58,27 +58,21 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
         const uint64_t q = (uint64_t)primes[i].q;
         const uint32_t qlo = (uint32_t)q, qhi = (uint32_t)(q >> 32);
         // X * q_i + m_i < q_i ... q_{nq-1} < 2^{51 (nq - i)}: words beyond `len` stay zero
-        // (len is wave-uniform, so the skipped words cost no issue slots)
+        // (len is wave-uniform, so the skipped words cost no issue slots).
+        // One pass, two carry chains: t = x_k qlo + c0 (< 2^64 - 2^32), u = lo32(t) + x_{k-1} qhi + c1
+        // (< 2^53); the new word is lo32(u).
         const int len = min(W, (51 * (nq - i) + 31) / 32 + 1);
-        uint64_t carry = 0;
-        uint32_t y[W];
-#pragma unroll
-        for (int k = 0; k < W; k++) {  // Y = X * qlo  (t < 2^64 - 2^32)
-            if (k < len) {
-                uint64_t t = (uint64_t)x[k] * qlo + carry;
-                y[k] = (uint32_t)t;
-                carry = t >> 32;
-            }
-        }
-        carry = (uint64_t)m[i * kDecBS + tid];  // + m_i (t < 2^53 below)
+        uint64_t c0 = 0, c1 = (uint64_t)m[i * kDecBS + tid];
         uint32_t prev = 0;
 #pragma unroll
-        for (int k = 0; k < W; k++) {  // X = Y + (X * qhi) << 32 + m_i
+        for (int k = 0; k < W; k++) {
             if (k < len) {
-                uint64_t t = (uint64_t)y[k] + (uint64_t)prev * qhi + carry;
+                const uint64_t t = (uint64_t)x[k] * qlo + c0;
+                const uint64_t u = (t & 0xffffffffull) + (uint64_t)prev * qhi + c1;
                 prev = x[k];
-                x[k] = (uint32_t)t;
-                carry = t >> 32;
+                x[k] = (uint32_t)u;
+                c0 = t >> 32;
+                c1 = u >> 32;
             }
         }
     }
@@ -101,25 +95,26 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
             borrow = (t >> 32) & 1;
         }
     }
-    // balanced digits of X (two's complement, sign-extended over W words)
-    int carry = 0;
-    constexpr int DMAX = (32 * W) / 30;
+    // balanced digits of X (two's complement, sign-extended over W words).  X is parked in this
+    // thread's LDS column so the digit loop can index its words without a private stack array.
+    uint32_t* xs = reinterpret_cast<uint32_t*>(m + nq * kDecBS);  // [W + 1][kDecBS]
 #pragma unroll
-    for (int i = 0; i < DMAX; i++) {
-        if (i < levels) {
-            const int bit = 30 * i, w = bit >> 5, sh = bit & 31;
-            uint64_t lo = x[w];
-            uint64_t hi = (w + 1 < W) ? x[w + 1] : (x[W - 1] >> 31 ? 0xffffffffu : 0u);
-            int raw = (int)(((lo | (hi << 32)) >> sh) & 0x3fffffffu);
-            int d = raw + carry;
-            if (d >= (1 << 29)) {
-                d -= (1 << 30);
-                carry = 1;
-            } else {
-                carry = 0;
-            }
-            digits[(long long)i * n + c] = d;
+    for (int k = 0; k < W; k++) xs[k * kDecBS + tid] = x[k];
+    xs[W * kDecBS + tid] = (x[W - 1] >> 31) ? 0xffffffffu : 0u;
+    int carry = 0;
+    for (int i = 0; i < levels; i++) {
+        const int bit = 30 * i, w = bit >> 5, sh = bit & 31;
+        const uint64_t lo = xs[w * kDecBS + tid];
+        const uint64_t hi = xs[min(w + 1, W) * kDecBS + tid];
+        const int raw = (int)(((lo | (hi << 32)) >> sh) & 0x3fffffffu);
+        int d = raw + carry;
+        if (d >= (1 << 29)) {
+            d -= (1 << 30);
+            carry = 1;
+        } else {
+            carry = 0;
         }
+        digits[(long long)i * n + c] = d;
     }
 }
 
@@ -128,7 +123,7 @@ static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, in
                   long long ps) {
     const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords;
     ProfScope prof(s, "decompose");
-    k_decompose<W><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, (size_t)nq * kDecBS * 8, s>>>(poly, ps, c.n, nq, levels, c.d_primes, c.d_invq,
+    k_decompose<W><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, (size_t)nq * kDecBS * 8 + (W + 1) * kDecBS * 4, s>>>(poly, ps, c.n, nq, levels, c.d_primes, c.d_invq,
                                                                         c.L + c.K, qw, qw + kWords, digits);
     RCK_CHECK_HIP(hipGetLastError());
 }
@@ -166,7 +161,10 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 // served to the other ciphertexts of the batch from that XCD's L2.
 // Output: partial sums part[s][b][comp][j][N] per digit split s (reduced doubles).
 // ---------------------------------------------------------------------------------------
-template <int LOGR, int ROWS, bool PREFETCH>
+// PF (load schedule): 0 = key rows loaded after the NTT stages; 1 = key rows of digit i issued
+// before its NTT stages; 2 = the column-pass rows of digit i+1 issued before digit i's stages
+// (software pipelining), key rows late.
+template <int LOGR, int ROWS, int PF>
 __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const double* __restrict__ cols, int D, int G,
                                                                     int B, int dsplit, int ntiles,
                                                                     const double* __restrict__ key, int E, int j0,
@@ -188,21 +186,36 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
     const double q = P.q, qinv = P.qinv;
     const long long off = (long long)b * R;
     const long long EN = (long long)E * n;
+    const long long col_stride = (long long)G * n;  // digit i -> i + 1 in cols[b][i][jl][N]
+    const double* pc = cols + (((long long)cb * D + i0) * G + jl) * n + off + tau;
     double a0[16], a1[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) a0[e] = a1[e] = 0.0;
-    for (int i = i0; i < i1; i++) {
-        const double* p = cols + (((long long)cb * D + i) * G + jl) * n + off;
-        const double* k0 = key + (2LL * i) * EN + (long long)j * n + off + 16 * tau;
-        const double* k1 = k0 + EN;
-        double v[16], w0[PREFETCH ? 16 : 1], w1[PREFETCH ? 16 : 1];
+    double nx[PF == 2 ? 16 : 1];
+    if constexpr (PF == 2) {
 #pragma unroll
-        for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
-        if constexpr (PREFETCH) {
+        for (int e = 0; e < 16; e++) nx[e] = pc[TPS * e];
+    }
+    for (int i = i0; i < i1; i++, pc += col_stride) {
+        const double* k0 = key + (2LL * i) * EN + (long long)j * n + off + tau;
+        const double* k1 = k0 + EN;
+        double v[16], w0[PF == 1 ? 16 : 1], w1[PF == 1 ? 16 : 1];
+        if constexpr (PF == 2) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) v[e] = nx[e];
+            if (i + 1 < i1) {
+#pragma unroll
+                for (int e = 0; e < 16; e++) nx[e] = pc[col_stride + TPS * e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; e++) v[e] = pc[TPS * e];
+        }
+        if constexpr (PF == 1) {
 #pragma unroll
             for (int e = 0; e < 16; e++) {
-                w0[e] = k0[e];
-                w1[e] = k1[e];
+                w0[e] = k0[e * TPS];
+                w1[e] = k1[e * TPS];
             }
         }
         fwd_phase_a(v, P.fw, N1 + b, q, qinv);
@@ -215,12 +228,12 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
         fwd_phase_b<LOGR>(v, P.fw, N1 + b, tau, q, qinv);
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-            if constexpr (PREFETCH) {
+            if constexpr (PF == 1) {
                 a0[e] += mulmod(v[e], w0[e], q, qinv);
                 a1[e] += mulmod(v[e], w1[e], q, qinv);
             } else {
-                a0[e] += mulmod(v[e], k0[e], q, qinv);
-                a1[e] += mulmod(v[e], k1[e], q, qinv);
+                a0[e] += mulmod(v[e], k0[e * TPS], q, qinv);
+                a1[e] += mulmod(v[e], k1[e * TPS], q, qinv);
             }
         }
         if (((i - i0) & 3) == 3) {
@@ -230,16 +243,16 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
     }
     reduce_all(a0, q, qinv);
     reduce_all(a1, q, qinv);
-    double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + off + 16 * tau;
+    double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + off + tau;
     double* o1 = o0 + EN;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
-        o0[e] = a0[e];
-        o1[e] = a1[e];
+        o0[e * TPS] = a0[e];
+        o1[e * TPS] = a1[e];
     }
 }
 
-template <int LOGR, int ROWS, bool PF>
+template <int LOGR, int ROWS, int PF>
 static void mac_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
                   int j0, double* part, PrimeMap pm, hipStream_t s) {
     constexpr int TPS = (1 << LOGR) / 16;
@@ -252,20 +265,24 @@ static void mac_t(const Context& c, const double* cols, int D, int G, int B, int
     RCK_CHECK_HIP(hipGetLastError());
 }
 
+template <int LOGR, int ROWS>
+static void mac_r(const Context& c, int pf, const double* cols, int D, int G, int B, int dsplit, const double* key,
+                  int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
+    if (pf == 1)
+        mac_t<LOGR, ROWS, 1>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    else if (pf == 2)
+        mac_t<LOGR, ROWS, 2>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    else
+        mac_t<LOGR, ROWS, 0>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+}
+
 template <int LOGR>
-static void mac_v(const Context& c, int rows, bool pf, const double* cols, int D, int G, int B, int dsplit,
+static void mac_v(const Context& c, int rows, int pf, const double* cols, int D, int G, int B, int dsplit,
                   const double* key, int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
-    if (rows <= 4) {
-        if (pf)
-            mac_t<LOGR, 4, true>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
-        else
-            mac_t<LOGR, 4, false>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
-    } else {
-        if (pf)
-            mac_t<LOGR, 16, true>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
-        else
-            mac_t<LOGR, 16, false>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
-    }
+    if (rows <= 4)
+        mac_r<LOGR, 4>(c, pf, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    else
+        mac_r<LOGR, 16>(c, pf, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
 }
 
 // Small rings: plain forward NTT of every digit for every prime, then pointwise MAC.
@@ -292,7 +309,7 @@ __global__ void k_mac_small(const double* __restrict__ dn, int D, const double* 
 }
 
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
-                      double* part, int dsplit, int group, int rows, bool prefetch, hipStream_t s) {
+                      double* part, int dsplit, int group, int rows, int prefetch, hipStream_t s) {
     const int E = nq + naux;
     const long long n = c.n, EN = (long long)E * n;
     PrimeMap pm{nq, (int)c.L, 0};

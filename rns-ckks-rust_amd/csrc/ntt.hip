@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_fwd_rows(Batch<double> data, int logn, 
     for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
     fwd_phase_b<LOGR>(v, P.fw, N1 + b, tau, q, qinv);
 #pragma unroll
-    for (int e = 0; e < 16; e++) p[16 * tau + e] = v[e];
+    for (int e = 0; e < 16; e++) p[e * TPS + tau] = v[e];  // NTT-domain row order (ntt.hpp)
 }
 
 // Row pass, inverse (in place).
@@ -93,11 +93,11 @@ __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, 
     double* p = data.at(z, y) + (long long)b * R;
     double v[16];
 #pragma unroll
-    for (int e = 0; e < 16; e++) v[e] = p[16 * tau + e];
+    for (int e = 0; e < 16; e++) v[e] = p[e * TPS + tau];  // NTT-domain row order (ntt.hpp)
     if (nsum > 1) {
         for (int k = 1; k < nsum; k++) {
 #pragma unroll
-            for (int e = 0; e < 16; e++) v[e] += p[k * sstride + 16 * tau + e];
+            for (int e = 0; e < 16; e++) v[e] += p[k * sstride + e * TPS + tau];
         }
         reduce_all(v, q, qinv);  // |sum| <= nsum * 0.51q < 2^53 for nsum <= 8
     }

@@ -16,6 +16,13 @@
 // Each pass keeps 16 elements per thread in registers (radix-16: 4 stages per register phase)
 // and exchanges once through LDS between its two phases.  256 threads per workgroup; a
 // workgroup owns 4096 elements = 32 KiB of LDS.
+//
+// NTT-domain storage order (two-pass rings): after the row pass thread tau holds the row
+// positions x = 16 tau + e; they are stored at (x % 16) * (N2 / 16) + x / 16 within the row, so
+// that for every e the lanes of a wave write (and the key multiply-accumulate later reads)
+// consecutive addresses.  Every NTT-domain operand (keys, accumulators, pointwise products)
+// uses this order; pointwise operations do not care, and the cyclic-mode permutation kernels
+// map through ntt_phys().  Rings N <= 2^11 keep the natural merged-NTT order.
 #pragma once
 #include "context.hpp"
 #include "modarith.hpp"
@@ -149,9 +156,18 @@ enum InKind { kInU64 = 0, kInI32 = 1, kInF64 = 2 };
 enum OutKind { kOutU64 = 0, kOutF64 = 1 };
 
 // Sizes of the two passes for a ring degree 2^logn (2^12 .. 2^16).
-inline void split_logn(int logn, int* n1, int* n2) {
+__host__ __device__ inline void split_logn(int logn, int* n1, int* n2) {
     *n1 = logn / 2;
     *n2 = logn - *n1;
+}
+
+// Physical position of NTT-output slot i (logical merged-NTT order) in an NTT-domain limb.
+__host__ __device__ inline unsigned ntt_phys(unsigned i, int logn) {
+    if (logn <= 11) return i;
+    int n1, n2;
+    split_logn(logn, &n1, &n2);
+    const unsigned x = i & ((1u << n2) - 1);
+    return (i & ~((1u << n2) - 1)) | ((x & 15u) << (n2 - 4)) | (x >> 4);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -115,7 +115,7 @@ __global__ void k_brv_out(Batch<const double> in, Batch<uint64_t> out, int n, in
     const int y = blockIdx.y, z = blockIdx.z;
     const DevPrime P = primes[pm(y)];
     const unsigned r = __brev((unsigned)k) >> (32 - logn);
-    out.at(z, y)[k] = f64_to_u64(canon(in.at(z, y)[r], P.q, P.qinv));
+    out.at(z, y)[k] = f64_to_u64(canon(in.at(z, y)[ntt_phys(r, logn)], P.q, P.qinv));
 }
 
 __global__ void k_brv_in(Batch<const uint64_t> in, Batch<double> out, int n, int logn, PrimeMap pm,
@@ -125,7 +125,7 @@ __global__ void k_brv_in(Batch<const uint64_t> in, Batch<double> out, int n, int
     const int y = blockIdx.y, z = blockIdx.z;
     const DevPrime P = primes[pm(y)];
     const unsigned r = __brev((unsigned)k) >> (32 - logn);
-    out.at(z, y)[k] = load_centred(in.at(z, y)[r], P.q);
+    out.at(z, y)[ntt_phys(r, logn)] = load_centred(in.at(z, y)[k], P.q);
 }
 
 __global__ void k_twist_out(Batch<const double> in, Batch<uint64_t> out, int n, PrimeMap pm,

@@ -125,7 +125,7 @@ std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
     for (int g : gs)
         for (int ds : {1, 2, 4})
             for (int rows : {4, 16})
-                for (int pf : {0, 1})
+                for (int pf : {0, 1, 2})
                     if (ds <= D && g >= 1 && g <= E) out.push_back({g, ds, rows, pf});
     (void)B;
     return out;
@@ -161,7 +161,7 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     const int group = shape.group > 0 ? std::min(shape.group, E) : aut.group;
     const int dsplit = c.logn <= 11 ? 1 : (shape.dsplit > 0 ? std::min(shape.dsplit, std::min(8, levels)) : aut.dsplit);
     const int rows = shape.rows > 0 ? shape.rows : aut.rows;
-    const bool prefetch = (shape.prefetch >= 0 ? shape.prefetch : aut.prefetch) != 0;
+    const int prefetch = shape.prefetch >= 0 ? shape.prefetch : aut.prefetch;
     int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
     double* part = static_cast<double*>(c.workspace(6, (size_t)dsplit * B * 2 * EN * 8));
     launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps);

@@ -17,7 +17,7 @@ struct KsShape {
     int group = 0;     // eval limbs per column-pass/MAC launch
     int dsplit = 0;    // digit ranges accumulated by separate workgroups
     int rows = 0;      // ring rows per MAC workgroup (4 or 16)
-    int prefetch = -1; // 1: key rows loaded before the NTT stages of each digit
+    int prefetch = -1; // load schedule: 0 late keys, 1 early keys, 2 next digit's rows prefetched
 };
 
 inline void check_ptr(const void* p, const char* what) {
