@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="ciphertext pairs per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuners (automatic launch shapes)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (5 times): the known-volume reference "
+                         "for rocprofv3 FETCH_SIZE/WRITE_SIZE calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path has no multi-GPU form here); "
@@ -69,6 +73,15 @@ def main():
 
     from rnsckks import capi as G
     from rnsckks import params as PR
+
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src, dst = G.DeviceBuffer(nbytes // 8), G.DeviceBuffer(nbytes // 8)
+        for _ in range(5):
+            G.diag_copy(dst, src, nbytes)
+        G.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 5}), flush=True)
+        return
 
     p = PR.config_per_op_gpu()
     n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
@@ -92,10 +105,12 @@ def main():
         ctx.relinearize(quad, L, 80, ksk, out)
 
     # autotune the per-op keyswitch launch shape (results are identical for every shape)
-    tune1 = ctx.tune_keyswitch(L, ksk, 1)
-    log(f"[bench] per-op keyswitch tuned: { {k: v for k, v in tune1.items() if k != 'candidates'} }")
-    for cand in sorted(tune1["candidates"], key=lambda c: c["ms"])[:8]:
-        log(f"[bench]     candidate {cand}")
+    tune1 = {"digit_splits": None, "candidates": []}
+    if not args.no_tune:
+        tune1 = ctx.tune_keyswitch(L, ksk, 1)
+        log(f"[bench] per-op keyswitch tuned: { {k: v for k, v in tune1.items() if k != 'candidates'} }")
+        for cand in sorted(tune1["candidates"], key=lambda c: c["ms"])[:8]:
+            log(f"[bench]     candidate {cand}")
 
     # per-op path (always measured: the batched form is reported against it)
     for _ in range(args.warmup):
@@ -111,7 +126,7 @@ def main():
         for b in range(B):
             bt.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ksk, out.offset(b * 2 * P))
         t = time.perf_counter()
-        bt.compile(tune=True)
+        bt.compile(tune=not args.no_tune)
         report = bt.report()
         log(f"[bench] batch compiled+tuned in {time.perf_counter() - t:.1f} s")
         for grp in report["groups"]:
@@ -154,6 +169,8 @@ def main():
         launches = max(1, cnt // prof_steps)
         G_ = -(-(L + K) // launches)  # eval limbs per launch
         ds = report["groups"][0]["digit_splits"] if report else tune1["digit_splits"]
+        if ds is None:  # automatic shape without tuning: the library's heuristic for one ciphertext
+            ds = max(1, min(4, -(-2048 // (64 * (L + K) * B))))
         alg = mac_bytes(n, D, G_, B, ds)
         achieved = alg / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -187,6 +187,9 @@ int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uin
  * is launched on.  rck_profile_report synchronizes the device and writes a JSON object
  * {"kernel": [launches, total_ms], ...} for the launches since the previous report. */
 int rck_profile_enable(int on);
+/* A plain streaming copy kernel of `bytes` (multiple of 32): the known-volume reference for
+ * calibrating hardware byte counters (FETCH_SIZE / WRITE_SIZE). */
+int rck_diag_copy(void* dst, const void* src, size_t bytes, void* stream);
 int rck_profile_report(char* buf, size_t len);
 
 #ifdef __cplusplus

@@ -451,6 +451,10 @@ int rck_rotate(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t sc
     });
 }
 
+int rck_diag_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    return guard([&] { launch_copy(src, dst, bytes, S(stream)); });
+}
+
 int rck_profile_enable(int on) {
     return guard([&] { prof_set(on != 0); });
 }

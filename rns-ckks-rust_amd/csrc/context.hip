@@ -340,6 +340,24 @@ void* Context::workspace(int slot, size_t bytes) {
     return ws_ptr[slot];
 }
 
+hipStream_t Context::aux_stream() {
+    if (!aux) {
+        RCK_REQUIRE(!frozen, "stream creation during graph capture");
+        RCK_CHECK_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    }
+    return aux;
+}
+
+hipEvent_t Context::event(int i) {
+    while ((int)events.size() <= i) {
+        RCK_REQUIRE(!frozen, "event creation during graph capture");
+        hipEvent_t e;
+        RCK_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        events.push_back(e);
+    }
+    return events[i];
+}
+
 Context* Context::scratch_clone() const {
     Context* c = new Context();
     c->n = n;
@@ -368,6 +386,8 @@ Context::~Context() {
     (void)hipDeviceSynchronize();
     for (void* p : ws_ptr)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (aux) (void)hipStreamDestroy(aux);
     if (!owns_tables) return;
     void* ptrs[] = {d_primes, d_tables, d_invq, d_qwords, d_pwords, d_pmodq, d_pinvq, d_pmod_aux};
     for (void* p : ptrs)

@@ -63,6 +63,12 @@ struct Context {
 
     uint32_t prime_count() const { return L + K; }
     void* workspace(int slot, size_t bytes);
+    // a second (default-priority, non-blocking) stream for overlapping independent launches, and a
+    // small pool of timing-free events for ordering work between the two
+    hipStream_t aux_stream();
+    hipEvent_t event(int i);
+    hipStream_t aux = nullptr;
+    std::vector<hipEvent_t> events;
     // a context sharing this one's device tables with its own (empty) scratch
     Context* scratch_clone() const;
     ~Context();

@@ -326,37 +326,55 @@ void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, con
     }
     RCK_REQUIRE(dsplit >= 1 && dsplit <= D, "bad digit split");
     const int G = std::max(1, std::min(group, E));
-    double* cols = static_cast<double*>(cc.workspace(7, (size_t)B * D * G * n * 8));
+    const int ngroups = (E + G - 1) / G;
+    const size_t slot_elems = (size_t)B * D * G * n;
+    double* cols_base = static_cast<double*>(cc.workspace(7, (ngroups > 1 ? 2 : 1) * slot_elems * 8));
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
-    for (int j0 = 0; j0 < E; j0 += G) {
+    // With several groups the column pass of group g+1 runs on the context's second stream while
+    // the MAC of group g runs on `s` (double-buffered column scratch, event-ordered both ways).
+    hipStream_t s2 = ngroups > 1 ? cc.aux_stream() : s;
+    if (ngroups > 1) {
+        RCK_CHECK_HIP(hipEventRecord(cc.event(0), s));
+        RCK_CHECK_HIP(hipStreamWaitEvent(s2, cc.event(0), 0));
+    }
+    for (int gi = 0, j0 = 0; j0 < E; gi++, j0 += G) {
         const int g = std::min(G, E - j0);
+        const int slot = gi & 1;
+        double* cols = cols_base + slot * slot_elems;
+        if (ngroups > 1 && gi >= 2) RCK_CHECK_HIP(hipStreamWaitEvent(s2, cc.event(1 + slot), 0));
         // column pass of digit (b, i) for eval limbs j0 .. j0+g-1 -> cols[b][i][jl][N]
         launch_fwd_cols(c, kInI32, digits, 0, n, cols, n, (long long)g * n, g, B * D, PrimeMap{nq, (int)c.L, j0},
-                        s);
+                        s2);
+        if (ngroups > 1) {
+            RCK_CHECK_HIP(hipEventRecord(cc.event(3 + slot), s2));
+            RCK_CHECK_HIP(hipStreamWaitEvent(s, cc.event(3 + slot), 0));
+        }
         switch (n2) {
             case 6: mac_v<6>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
             case 7: mac_v<7>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
             case 8: mac_v<8>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
             default: RCK_REQUIRE(false, "unsupported ring degree");
         }
+        if (ngroups > 1) RCK_CHECK_HIP(hipEventRecord(cc.event(1 + slot), s));
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// mod-down (keyswitching.rs:278-322) with the optional relinearization add.
+// mod-down (keyswitching.rs:278-322) with the optional relinearization / rotation add.
 //   x_P = Garner(acc_P) in [0, P);  c = centred(x_P) mod q_i;  out = (x_i - c) P^{-1} mod q_i.
-// One thread per (coefficient, Q-limb, accumulator).
+// One thread per (coefficient, accumulator): the P-part is converted once and applied to every
+// Q-limb (coalesced row reads per limb).
 // ---------------------------------------------------------------------------------------
 __global__ void k_mod_down(const uint64_t* __restrict__ acc, int n, int nq, int K, int L, int T,
                            const uint64_t* __restrict__ add0, const uint64_t* __restrict__ add1,
-                           uint64_t* __restrict__ out,
-                           long long out_ps, const DevPrime* __restrict__ primes, const double* __restrict__ invq,
-                           const uint32_t* __restrict__ pwords, const double* __restrict__ pmodq,
-                           const double* __restrict__ pinvq, const double* __restrict__ pmod_aux) {
+                           uint64_t* __restrict__ out, long long out_ps, const DevPrime* __restrict__ primes,
+                           const double* __restrict__ invq, const uint32_t* __restrict__ pwords,
+                           const double* __restrict__ pmodq, const double* __restrict__ pinvq,
+                           const double* __restrict__ pmod_aux) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const int i = blockIdx.y, z = blockIdx.z;
+    const int z = blockIdx.y;
     const long long limbs = nq + K;
     const uint64_t* a = acc + (long long)z * limbs * n;
     // Garner over the aux primes (mixed_radix_digits, keyswitching.rs:397-410)
@@ -370,35 +388,28 @@ __global__ void k_mod_down(const uint64_t* __restrict__ acc, int n, int nq, int 
         }
         m[k] = k == 0 ? v : canon(v, Pk.q, Pk.qinv);
     }
-    // X_P as words (<= 8 x 32 bits), compared with floor(P/2)
+    // X_P as 8 words (Horner, one pass with two carry chains), compared with floor(P/2)
     uint32_t xw[8];
 #pragma unroll
     for (int w = 0; w < 8; w++) xw[w] = 0;
-    for (int k = K - 1; k >= 0; k--) {
-        uint64_t pk = k == K - 1 ? 0 : (uint64_t)primes[L + k].q;
-        if (k == K - 1) {
-            uint64_t t = (uint64_t)m[k];
-            xw[0] = (uint32_t)t;
-            xw[1] = (uint32_t)(t >> 32);
-            continue;
-        }
+    {
+        const uint64_t t = (uint64_t)m[K - 1];
+        xw[0] = (uint32_t)t;
+        xw[1] = (uint32_t)(t >> 32);
+    }
+    for (int k = K - 2; k >= 0; k--) {
+        const uint64_t pk = (uint64_t)primes[L + k].q;
         const uint32_t plo = (uint32_t)pk, phi = (uint32_t)(pk >> 32);
-        uint64_t carry = 0;
-        uint32_t y[8];
-#pragma unroll
-        for (int w = 0; w < 8; w++) {
-            uint64_t t = (uint64_t)xw[w] * plo + carry;
-            y[w] = (uint32_t)t;
-            carry = t >> 32;
-        }
-        carry = (uint64_t)m[k];
+        uint64_t c0 = 0, c1 = (uint64_t)m[k];
         uint32_t prev = 0;
 #pragma unroll
         for (int w = 0; w < 8; w++) {
-            uint64_t t = (uint64_t)y[w] + (uint64_t)prev * phi + carry;
+            const uint64_t t = (uint64_t)xw[w] * plo + c0;
+            const uint64_t u = (t & 0xffffffffull) + (uint64_t)prev * phi + c1;
             prev = xw[w];
-            xw[w] = (uint32_t)t;
-            carry = t >> 32;
+            xw[w] = (uint32_t)u;
+            c0 = t >> 32;
+            c1 = u >> 32;
         }
     }
     bool gt = false, decided = false;
@@ -410,25 +421,188 @@ __global__ void k_mod_down(const uint64_t* __restrict__ acc, int n, int nq, int 
             decided = true;
         }
     }
-    // centred(x_P) mod q_i by Horner (evaluate_mixed_radix_mod, keyswitching.rs:412-423)
-    const DevPrime Pi = primes[i];
-    const double q = Pi.q, qinv = Pi.qinv;
+    const uint64_t* add = z == 0 ? add0 : add1;
+    uint64_t* o = out + (long long)z * out_ps;
+    for (int i = 0; i < nq; i++) {
+        // centred(x_P) mod q_i by Horner (evaluate_mixed_radix_mod, keyswitching.rs:412-423)
+        const DevPrime Pi = primes[i];
+        const double q = Pi.q, qinv = Pi.qinv;
+        double cv = reduce(m[K - 1], q, qinv);
+        for (int k = K - 2; k >= 0; k--)
+            cv = reduce(mulmod(cv, pmod_aux[(long long)k * L + i], q, qinv) + m[k], q, qinv);
+        if (gt) cv -= pmodq[i];
+        const double xi = (double)a[(long long)i * n + c];
+        double r = mulmod(reduce(xi - cv, q, qinv), pinvq[i], q, qinv);
+        if (add) r += (double)add[(long long)i * n + c];
+        o[(long long)i * n + c] = f64_to_u64(canon(r, q, qinv));
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Inverse column pass of the Q-limbs of the keyswitch accumulators fused with mod-down and the
+// relinearization / rotation add: the coefficient x_i leaves the inverse NTT in registers, the
+// aux residues of the same coefficient (already in coefficient form, accP) are converted to the
+// centred x_P mod q_i, and (x_i - c) * P^{-1} (+ addend) is written straight to the output.
+// Same arithmetic as k_mod_down; saves writing and re-reading the Q-part of the accumulators.
+// ---------------------------------------------------------------------------------------
+struct OutPtrs {
+    uint64_t* p[32];
+};
+
+// centred(x_P) mod q_i for the aux residues r[0..K) of one coefficient
+__device__ __forceinline__ double centred_p_mod_q(const double* r, int K, int L, int T, int i, double q, double qinv,
+                                                  const DevPrime* __restrict__ primes,
+                                                  const double* __restrict__ invq,
+                                                  const uint32_t* __restrict__ pwords,
+                                                  const double* __restrict__ pmodq,
+                                                  const double* __restrict__ pmod_aux) {
+    double m[kMaxP];
+    for (int k = 0; k < K; k++) {
+        const DevPrime Pk = primes[L + k];
+        double v = r[k];
+        for (int j = 0; j < k; j++) {
+            double d = reduce(v - m[j], Pk.q, Pk.qinv);
+            v = mulmod(d, invq[(long long)(L + k) * T + (L + j)], Pk.q, Pk.qinv);
+        }
+        m[k] = k == 0 ? v : canon(v, Pk.q, Pk.qinv);
+    }
+    bool gt;
+    if (K == 2) {  // X_P = m0 + m1 p0 < 2^102 as (hi, lo) 64-bit words
+        const uint64_t p0 = (uint64_t)primes[L].q, m1 = (uint64_t)m[1];
+        uint64_t lo = m1 * p0, hi = __umul64hi(m1, p0);
+        const uint64_t l2 = lo + (uint64_t)m[0];
+        hi += l2 < lo;
+        lo = l2;
+        const uint64_t hlo = (uint64_t)pwords[8] | ((uint64_t)pwords[9] << 32);
+        const uint64_t hhi = (uint64_t)pwords[10] | ((uint64_t)pwords[11] << 32);
+        gt = hi > hhi || (hi == hhi && lo > hlo);
+    } else {
+        uint32_t xw[8];
+#pragma unroll
+        for (int w = 0; w < 8; w++) xw[w] = 0;
+        {
+            const uint64_t t = (uint64_t)m[K - 1];
+            xw[0] = (uint32_t)t;
+            xw[1] = (uint32_t)(t >> 32);
+        }
+        for (int k = K - 2; k >= 0; k--) {
+            const uint64_t pk = (uint64_t)primes[L + k].q;
+            const uint32_t plo = (uint32_t)pk, phi = (uint32_t)(pk >> 32);
+            uint64_t c0 = 0, c1 = (uint64_t)m[k];
+            uint32_t prev = 0;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                const uint64_t t = (uint64_t)xw[w] * plo + c0;
+                const uint64_t u = (t & 0xffffffffull) + (uint64_t)prev * phi + c1;
+                prev = xw[w];
+                xw[w] = (uint32_t)u;
+                c0 = t >> 32;
+                c1 = u >> 32;
+            }
+        }
+        gt = false;
+        bool decided = false;
+#pragma unroll
+        for (int w = 7; w >= 0; w--) {
+            const uint32_t h = pwords[8 + w];
+            if (!decided && xw[w] != h) {
+                gt = xw[w] > h;
+                decided = true;
+            }
+        }
+    }
     double cv = reduce(m[K - 1], q, qinv);
     for (int k = K - 2; k >= 0; k--) cv = reduce(mulmod(cv, pmod_aux[(long long)k * L + i], q, qinv) + m[k], q, qinv);
     if (gt) cv -= pmodq[i];
-    const double xi = (double)a[(long long)i * n + c];
-    double r = mulmod(reduce(xi - cv, q, qinv), pinvq[i], q, qinv);
-    const uint64_t* add = z == 0 ? add0 : add1;
-    if (add) r += (double)add[(long long)i * n + c];
-    out[(long long)z * out_ps + (long long)i * n + c] = f64_to_u64(canon(r, q, qinv));
+    return cv;
+}
+
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ acc, long long EN,
+                                                     const uint64_t* __restrict__ accP, int K, int L, int T,
+                                                     const uint64_t* __restrict__ add0, const uint64_t* __restrict__ add1,
+                                                     long long add_ps, OutPtrs outs, long long P, int logn,
+                                                     const DevPrime* __restrict__ primes,
+                                                     const double* __restrict__ invq,
+                                                     const uint32_t* __restrict__ pwords,
+                                                     const double* __restrict__ pmodq,
+                                                     const double* __restrict__ pinvq,
+                                                     const double* __restrict__ pmod_aux) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[R * S];
+    const int n = 1 << logn, N2 = n >> LOGR;
+    const int tid = threadIdx.x, col = tid % S, tau = tid / S;
+    const int r = blockIdx.x * S + col;
+    const int i = blockIdx.y, z = blockIdx.z, cb = z >> 1, comp = z & 1;
+    const DevPrime Pi = primes[i];
+    const double q = Pi.q, qinv = Pi.qinv;
+    const double* src = acc + (long long)z * EN + (long long)i * n;
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = src[r + (long long)N2 * (16 * tau + e)];
+    inv_phase_b<LOGR>(v, Pi.iv, 1, tau, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[(16 * tau + e) * S + col] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
+    inv_phase_a(v, Pi.iv, 1, q, qinv);
+    const uint64_t* pp = accP + (long long)z * K * n;
+    const uint64_t* add = comp == 0 ? add0 : add1;
+    uint64_t* o = outs.p[cb] + (long long)comp * P + (long long)i * n;
+    const double pinv = pinvq[i];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const long long idx = r + (long long)N2 * (tau + TPS * e);
+        const double x = mulmod(v[e], Pi.ninv, q, qinv);  // x_i, |x| <= 0.94 q
+        double rp[kMaxP];
+        for (int k = 0; k < K; k++) rp[k] = (double)pp[(long long)k * n + idx];
+        const double cv = centred_p_mod_q(rp, K, L, T, i, q, qinv, primes, invq, pwords, pmodq, pmod_aux);
+        double res = mulmod(reduce(x - cv, q, qinv), pinv, q, qinv);
+        if (add) res += (double)add[(long long)cb * add_ps + (long long)i * n + idx];
+        o[idx] = f64_to_u64(canon(res, q, qinv));
+    }
+}
+
+void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B, const uint64_t* accP,
+                             const uint64_t* add0, const uint64_t* add1, long long add_ps,
+                             const std::vector<uint64_t*>& outs, hipStream_t s) {
+    RCK_REQUIRE(c.K > 0 && c.logn >= 12, "fused mod-down needs an aux basis and the two-pass NTT");
+    const long long n = c.n, EN = (long long)(nq + c.K) * n, P = (long long)nq * n;
+    int n1, n2;
+    split_logn(c.logn, &n1, &n2);
+    for (int b0 = 0; b0 < B; b0 += 32) {
+        const int bc = std::min(32, B - b0);
+        OutPtrs op{};
+        for (int b = 0; b < bc; b++) op.p[b] = outs[b0 + b];
+        const double* accb = acc + (long long)b0 * 2 * EN;
+        const uint64_t* accPb = accP + (long long)b0 * 2 * c.K * n;
+        const uint64_t* a0 = add0 ? add0 + b0 * add_ps : nullptr;
+        const uint64_t* a1 = add1 ? add1 + b0 * add_ps : nullptr;
+        ProfScope ps(s, "inv_cols_moddown");
+        const int S = 256 / ((1 << n1) / 16);
+        dim3 grid((1 << n2) / S, nq, 2 * bc);
+#define RCK_MD(LR)                                                                                                \
+    k_inv_cols_md<LR><<<grid, 256, 0, s>>>(accb, EN, accPb, c.K, c.L, c.L + c.K, a0, a1, add_ps, op, P, c.logn,   \
+                                           c.d_primes, c.d_invq, c.d_pwords, c.d_pmodq, c.d_pinvq, c.d_pmod_aux)
+        switch (n1) {
+            case 6: RCK_MD(6); break;
+            case 7: RCK_MD(7); break;
+            case 8: RCK_MD(8); break;
+            default: RCK_REQUIRE(false, "unsupported ring degree");
+        }
+#undef RCK_MD
+        RCK_CHECK_HIP(hipGetLastError());
+    }
 }
 
 void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64_t* add0, const uint64_t* add1,
                      uint64_t* out, long long out_ps, hipStream_t s) {
     RCK_REQUIRE(c.K > 0, "mod-down requires an auxiliary basis");
     ProfScope ps(s, "mod_down");
-    k_mod_down<<<dim3((c.n + 255) / 256, nq, 2), 256, 0, s>>>(acc, c.n, nq, c.K, c.L, c.L + c.K, add0, add1, out, out_ps, c.d_primes, c.d_invq, c.d_pwords,
-                                                              c.d_pmodq, c.d_pinvq, c.d_pmod_aux);
+    k_mod_down<<<dim3((c.n + 255) / 256, 2), 256, 0, s>>>(acc, c.n, nq, c.K, c.L, c.L + c.K, add0, add1, out, out_ps,
+                                                          c.d_primes, c.d_invq, c.d_pwords, c.d_pmodq, c.d_pinvq,
+                                                          c.d_pmod_aux);
     RCK_CHECK_HIP(hipGetLastError());
 }
 

@@ -137,6 +137,20 @@ __global__ void k_twist_out(Batch<const double> in, Batch<uint64_t> out, int n, 
     out.at(z, y)[j] = f64_to_u64(canon(mulmod(in.at(z, y)[j], P.twi[j], P.q, P.qinv), P.q, P.qinv));
 }
 
+// plain streaming copy (diagnostics: calibrates hardware byte counters against a known volume)
+__global__ void k_copy(const double4* __restrict__ src, double4* __restrict__ dst, long long n4) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    RCK_REQUIRE(bytes % 32 == 0, "copy size must be a multiple of 32 bytes");
+    ProfScope ps(s, "copy");
+    k_copy<<<dim3(4096), 256, 0, s>>>(static_cast<const double4*>(src), static_cast<double4*>(dst),
+                                       (long long)(bytes / 32));
+    RCK_CHECK_HIP(hipGetLastError());
+}
+
 static dim3 grid_for(int n, int limbs, int polys) { return dim3((n + 255) / 256, limbs, polys); }
 
 #define CHK() RCK_CHECK_HIP(hipGetLastError())

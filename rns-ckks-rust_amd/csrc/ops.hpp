@@ -1,5 +1,7 @@
 // Launchers of the elementwise kernels (ops.hip) and the keyswitch pipeline (keyswitch.hip).
 #pragma once
+#include <vector>
+
 #include "ntt.hpp"
 
 namespace rck {
@@ -24,6 +26,8 @@ void launch_brv_in(const Context& c, Batch<const uint64_t> in, Batch<double> out
 void launch_twist_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
                       hipStream_t s);
 
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
 // keyswitch pipeline
 // Balanced base-2^30 decomposition of a coefficient-form polynomial with nq Q-limbs into
 // `levels` signed digits per coefficient: digits[i * N + c] (keyswitching.rs:196-236, 381-467).
@@ -42,5 +46,12 @@ void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, con
 // nq || P (2 polys, u64), optionally adding add0 / add1 (nq limbs each) to the two results.
 void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64_t* add0, const uint64_t* add1,
                      uint64_t* out, long long out_ps, hipStream_t s);
+
+// Inverse column pass of the Q-limbs of B x 2 keyswitch accumulators (acc: [B][2][nq+K][N] after the
+// inverse row pass) fused with mod-down by P (aux limbs already in coefficient form in
+// accP: [B][2][K][N]) and the optional adds add0/add1 (+ b * add_ps); writes outs[b] = [c0][c1].
+void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B, const uint64_t* accP,
+                             const uint64_t* add0, const uint64_t* add1, long long add_ps,
+                             const std::vector<uint64_t*>& outs, hipStream_t s);
 
 }  // namespace rck

@@ -168,7 +168,15 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, rows, prefetch, s);
     PrimeMap pm{limbs, (int)c.L, 0};
     const long long sstride = (long long)B * 2 * EN;
-    if (naux) {
+    if (naux && c.logn >= 12) {
+        // inverse rows of every limb; inverse columns of the aux limbs to coefficient form; the Q-limb
+        // inverse columns run fused with mod-down and the adds
+        launch_inv_rows(c, part, n, EN, E, 2 * B, pm, s, dsplit, sstride);
+        uint64_t* accP = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * naux * n * 8));
+        launch_inv_cols(c, part + (long long)limbs * n, n, EN, kOutU64, accP, n, (long long)naux * n, naux, 2 * B,
+                        PrimeMap{limbs, (int)c.L, limbs}, s);
+        launch_inv_cols_moddown(c, part, limbs, B, accP, add0, add1, add_ps, outs, s);
+    } else if (naux) {
         uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * EN * 8));
         launch_inv(c, part, n, EN, kOutU64, accu, n, EN, E, 2 * B, pm, s, dsplit, sstride);
         for (int b = 0; b < B; b++)

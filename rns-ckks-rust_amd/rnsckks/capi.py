@@ -68,6 +68,7 @@ EXPORTS = {
     "rck_batch_report": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
     "rck_batch_destroy": (ctypes.c_int, [vp]),
     "rck_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "rck_diag_copy": (ctypes.c_int, [vp, vp, ctypes.c_size_t, vp]),
     "rck_profile_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 
@@ -196,6 +197,10 @@ class Event:
                 _lib.rck_event_destroy(self.ptr)
         except Exception:
             pass
+
+
+def diag_copy(dst, src, nbytes, stream=None):
+    _check(lib().rck_diag_copy(_p(dst), _p(src), nbytes, stream))
 
 
 def profile_enable(on=True):
@@ -332,7 +337,7 @@ class Context:
 
     def tune_keyswitch(self, limbs, ksk, batch=1):
         import json
-        buf = ctypes.create_string_buffer(4096)
+        buf = ctypes.create_string_buffer(1 << 20)
         _check(lib().rck_tune_keyswitch(self.ptr, limbs, ksk.ptr, batch, buf, len(buf)))
         return json.loads(buf.value.decode())
 
