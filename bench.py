@@ -52,6 +52,31 @@ def mac_bytes(n, D, G_, B, dsplit):
     return (B * D * G_ + 2 * D * G_ + dsplit * B * 2 * G_) * n * 8
 
 
+def measured_traffic(mode, n, L, K, D, B):
+    """HBM bytes per launch of the digit MAC (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    recorded in profiles/traffic.json for this exact workload; None for any other workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(mode)
+    if not rec or rec.get("workload") != f"N=2^{n.bit_length() - 1} L={L} K={K} D={D} B={B}":
+        return None
+    return rec["fetch_bytes"] + rec["write_bytes"]
+
+
+def per_op_entry(table, n, limbs, E, D):
+    """The tuning-table entry of one ciphertext at this workload, in the tuner's report form."""
+    logn = n.bit_length() - 1
+    for line in table.splitlines():
+        v = line.split()
+        if [int(x) for x in v[:5]] == [logn, limbs, E, D, 1]:
+            return {"limbs": limbs, "batch": 1, "eval_limb_group": int(v[5]), "digit_splits": int(v[6]),
+                    "mac_rows": int(v[7]), "key_prefetch": int(v[8]), "keyswitch_ms": float(v[9]),
+                    "candidates": [], "from_table": True}
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +87,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuners (automatic launch shapes)")
+    ap.add_argument("--tune-table", default=None, metavar="PATH",
+                    help="load autotuner results from PATH when it exists (tuned shapes are not re-timed) and "
+                         "save the table there afterwards")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (5 times): the known-volume reference "
                          "for rocprofv3 FETCH_SIZE/WRITE_SIZE calibration")
@@ -106,7 +134,13 @@ def main():
 
     # autotune the per-op keyswitch launch shape (results are identical for every shape)
     tune1 = {"digit_splits": None, "candidates": []}
-    if not args.no_tune:
+    loaded = 0
+    if args.tune_table and os.path.exists(args.tune_table):
+        with open(args.tune_table) as f:
+            loaded = G.tune_table_import(f.read())
+        log(f"[bench] tuning table {args.tune_table}: {loaded} entries loaded")
+        tune1 = per_op_entry(G.tune_table_export(), n, L, L + K, D) or tune1
+    if not args.no_tune and not loaded:
         tune1 = ctx.tune_keyswitch(L, ksk, 1)
         log(f"[bench] per-op keyswitch tuned: { {k: v for k, v in tune1.items() if k != 'candidates'} }")
         for cand in sorted(tune1["candidates"], key=lambda c: c["ms"])[:8]:
@@ -151,6 +185,10 @@ def main():
         ms_per_step, value = per_op_ms, per_op_value
         prof_fn = per_op_step
 
+    if args.tune_table:
+        with open(args.tune_table, "w") as f:
+            f.write(G.tune_table_export())
+
     # per-kernel timing: the same launches run eagerly with HIP events around each launch, on the
     # stream each kernel is launched on
     G.profile_enable(True)
@@ -174,7 +212,8 @@ def main():
         alg = mac_bytes(n, D, G_, B, ds)
         achieved = alg / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(args.mode, n, L, K, D, B),
                     "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg}
 
     cpu = None

@@ -283,6 +283,24 @@ int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uin
     });
 }
 
+int rck_tune_table_export(char* buf, size_t len, size_t* needed) {
+    return bguard([&] {
+        const std::string t = ks_tuned_export();
+        if (needed) *needed = t.size() + 1;
+        if (!buf) return;
+        RCK_REQUIRE(len > t.size(), "tuning table buffer too small");
+        std::memcpy(buf, t.c_str(), t.size() + 1);
+    });
+}
+
+int rck_tune_table_import(const char* text, uint32_t* entries) {
+    return bguard([&] {
+        RCK_REQUIRE(text != nullptr, "null tuning table");
+        const int k = ks_tuned_import(std::string(text));
+        if (entries) *entries = (uint32_t)k;
+    });
+}
+
 int rck_batch_create(rck_context* ctx, rck_batch** out) {
     return bguard([&] {
         RCK_REQUIRE(ctx && ctx->c && out, "null argument");

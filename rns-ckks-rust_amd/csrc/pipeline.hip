@@ -2,8 +2,10 @@
 // ciphertexts that share a level (and a key): negacyclic poly product, ct x ct mult, rescale and
 // the keyswitch pipeline.  Shared by the per-op C ABI and the batched executor.
 #include <algorithm>
+#include <cstdio>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "pipeline.hpp"
 
@@ -88,6 +90,46 @@ bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* sh
     *sh = it->second.first;
     if (ms) *ms = it->second.second;
     return true;
+}
+
+std::string ks_tuned_export() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string out;
+    char line[160];
+    for (const auto& kv : g_tuned) {
+        const KsShape& sh = kv.second.first;
+        std::snprintf(line, sizeof line, "%d %d %d %d %d %d %d %d %d %.6f\n", std::get<0>(kv.first),
+                      std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first),
+                      sh.group, sh.dsplit, sh.rows, sh.prefetch, (double)kv.second.second);
+        out += line;
+    }
+    return out;
+}
+
+int ks_tuned_import(const std::string& text) {
+    // parse everything first: a malformed table changes nothing
+    std::vector<std::pair<std::tuple<int, int, int, int, int>, std::pair<KsShape, float>>> rows;
+    size_t pos = 0;
+    while (pos < text.size()) {
+        size_t end = text.find('\n', pos);
+        if (end == std::string::npos) end = text.size();
+        const std::string line = text.substr(pos, end - pos);
+        pos = end + 1;
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        int v[9];
+        double ms = 0;
+        const int got = std::sscanf(line.c_str(), "%d %d %d %d %d %d %d %d %d %lf", &v[0], &v[1], &v[2], &v[3],
+                                    &v[4], &v[5], &v[6], &v[7], &v[8], &ms);
+        RCK_REQUIRE(got == 10, "malformed tuning table line: " + line);
+        RCK_REQUIRE(v[1] >= 1 && v[2] >= v[1] && v[3] >= 1 && v[4] >= 1, "tuning table entry out of range: " + line);
+        RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (v[7] == 4 || v[7] == 16) &&
+                        v[8] >= 0 && v[8] <= 2,
+                    "tuning table launch shape out of range: " + line);
+        rows.push_back({std::make_tuple(v[0], v[1], v[2], v[3], v[4]), {KsShape{v[5], v[6], v[7], v[8]}, (float)ms}});
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto& r : rows) g_tuned[r.first] = r.second;
+    return (int)rows.size();
 }
 
 KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B) {

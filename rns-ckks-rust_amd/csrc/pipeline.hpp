@@ -45,6 +45,10 @@ KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B);
 // tuned-shape table shared by the per-op calls and every batch of the process
 void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape s, float ms);
 bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* s, float* ms);
+// The table as text, one entry per line: "logn limbs E D B group dsplit rows prefetch ms"; import
+// validates every line before merging (returns the number of entries merged).
+std::string ks_tuned_export();
+int ks_tuned_import(const std::string& text);
 // candidates the autotuner times for a shape
 std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B);
 

@@ -59,6 +59,8 @@ EXPORTS = {
     "rck_ct_galois": (ctypes.c_int, [vp, vp, u32, u64, vp, vp]),
     "rck_rotate": (ctypes.c_int, [vp, vp, u32, u32, vp, vp, u32p, vp]),
     "rck_tune_keyswitch": (ctypes.c_int, [vp, u32, vp, u32, ctypes.c_char_p, ctypes.c_size_t]),
+    "rck_tune_table_export": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "rck_tune_table_import": (ctypes.c_int, [ctypes.c_char_p, u32p]),
     "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
     "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -201,6 +203,22 @@ class Event:
 
 def diag_copy(dst, src, nbytes, stream=None):
     _check(lib().rck_diag_copy(_p(dst), _p(src), nbytes, stream))
+
+
+def tune_table_export():
+    """The process-wide keyswitch tuning table as text (one entry per line)."""
+    need = ctypes.c_size_t(0)
+    _check(lib().rck_tune_table_export(None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    _check(lib().rck_tune_table_export(buf, len(buf), ctypes.byref(need)))
+    return buf.value.decode()
+
+
+def tune_table_import(text):
+    """Merges a table written by tune_table_export; returns the number of entries."""
+    k = ctypes.c_uint32(0)
+    _check(lib().rck_tune_table_import(text.encode(), ctypes.byref(k)))
+    return k.value
 
 
 def profile_enable(on=True):

@@ -56,3 +56,20 @@ def test_prime_chains_match_golden():
     p = PR.config_per_op_gpu()
     assert len(p.primes) == 24 and len(p.aux_primes) == 2
     assert p.primes[:3] == [1125899903827969, 1125899902124033, 1125899887312897]
+
+
+def test_tune_table_round_trip_and_validation():
+    """The keyswitch tuning table (host-side state) exports what was imported and rejects
+    malformed or out-of-range launch shapes without changing the table."""
+    G = capi
+    before = G.tune_table_export()
+    rows = "9 5 7 9 3 7 2 16 1 1.250000\n9 5 7 9 2 4 1 4 0 0.500000\n"
+    assert G.tune_table_import(rows) == 2
+    table = G.tune_table_export()
+    for line in rows.splitlines():
+        assert line in table.splitlines()
+    for bad in ("9 5 7 9 3 7 2 8 1 1.0\n", "9 5 7 9 3 8 2 16 1 1.0\n", "9 5 7 9\n"):
+        with pytest.raises(G.RckError):
+            G.tune_table_import(bad)
+        assert G.tune_table_export() == table
+    assert set(before.splitlines()) <= set(table.splitlines())
