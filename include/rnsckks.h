@@ -146,6 +146,52 @@ int rck_ct_galois(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint64_t
 int rck_rotate(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale, const rck_ksk* key,
                uint64_t* out, uint32_t* out_scale, void* stream);
 
+/* ---- seeded sampling, key generation, encryption (client side of the hot path) ----
+ * The crate draws from rand::thread_rng (unseeded).  Here one seeded stream (xoshiro256** seeded
+ * through splitmix64) is consumed in the crate's order, so a seed fixes every key and ciphertext:
+ *   uniform: for each limb, for each coefficient, next() % q          (src/rns/keys.rs:88-100)
+ *   error:   N draws of round(sigma * z), z ~ N(0,1) by Box-Muller with the pair's second normal
+ *            kept for the next draw, lifted to every limb              (src/rns/keys.rs:102-124)
+ *   ternary: N draws of next() % 3 -> 0, 1, -1                         (src/rns/keys.rs:126-151)
+ *   sparse ternary: idx = next() % N; if unused: next() & 1 ? 1 : -1   (src/rns/keys.rs:18-52)
+ * Sampling runs on the host; the arithmetic (a*s, pk*u, key plaintexts, NTT-form upload) on the
+ * device.  Polynomials are DEVICE buffers in the layout above.  These calls synchronize. */
+typedef struct rck_rng rck_rng;
+int rck_rng_create(uint64_t seed, rck_rng** out);
+int rck_rng_destroy(rck_rng* rng);
+uint64_t rck_rng_next(rck_rng* rng);
+int rck_sample_uniform(rck_context* ctx, rck_rng* rng, uint64_t* out, uint32_t nq, uint32_t np);
+int rck_sample_error(rck_context* ctx, rck_rng* rng, double sigma, uint64_t* out, uint32_t nq, uint32_t np);
+int rck_sample_ternary(rck_context* ctx, rck_rng* rng, uint64_t* out, uint32_t nq, uint32_t np);
+/* RnsSecretKey::sample_sparse_ternary (src/rns/keys.rs:18-52) */
+int rck_sample_sparse_ternary(rck_context* ctx, rck_rng* rng, uint32_t hamming_weight, uint64_t* out, uint32_t nq,
+                              uint32_t np);
+/* rns_keygen's public key (src/rns/keys.rs:153-167): pk1 = uniform, pk0 = e - pk1 * s; pk: [pk0][pk1]
+ * over `limbs` limbs. */
+int rck_keygen_public(rck_context* ctx, rck_rng* rng, double sigma, const uint64_t* secret, uint32_t limbs,
+                      uint64_t* pk);
+/* generate_relinearization_key_at_level (src/rns/context/keys.rs:46-63): source s^2 at `level`;
+ * hybrid (keyswitching.rs:81-128) when the context has aux primes, else Q-only encryption under pk
+ * (keyswitching.rs:44-79; pk has pk_limbs limbs per component).  The secret (L limbs) must be a
+ * small polynomial (|coefficients| < q_0 / 2; the crate's secrets are ternary).  The key is created
+ * NTT-resident, as by rck_ksk_create. */
+int rck_keygen_relin(rck_context* ctx, rck_rng* rng, double sigma, const uint64_t* secret, const uint64_t* pk,
+                     uint32_t pk_limbs, uint32_t level, rck_ksk** out);
+/* generate_rotation_key_at_level (src/rns/context/rotation.rs:43-60): source s(X^g),
+ * g = galois_element_for_rotation(N, steps). */
+int rck_keygen_rotation(rck_context* ctx, rck_rng* rng, double sigma, const uint64_t* secret, const uint64_t* pk,
+                        uint32_t pk_limbs, int64_t steps, uint32_t level, rck_ksk** out);
+/* The key in the crate's coefficient form, uint64_t[digits][2][level+1+aux][N] in HOST memory. */
+int rck_ksk_export(rck_context* ctx, const rck_ksk* key, uint64_t* out);
+int rck_ksk_info(const rck_ksk* key, uint32_t* digits, uint32_t* level, uint32_t* aux, uint64_t* galois);
+/* encrypt_rns (src/rns/ops.rs:8-42) at `limbs` limbs: draws u (ternary), e0, e1 (errors);
+ * out = [pk0 u + e0 + m][pk1 u + e1]; pk is truncated to `limbs` (pk_limbs limbs per component). */
+int rck_encrypt(rck_context* ctx, rck_rng* rng, double sigma, const uint64_t* plain, const uint64_t* pk,
+                uint32_t pk_limbs, uint32_t limbs, uint64_t* out);
+/* decrypt_rns (src/rns/ops.rs:44-52): out = c0 + c1 * s over `limbs` limbs */
+int rck_decrypt(rck_context* ctx, const uint64_t* ct, uint32_t limbs, const uint64_t* secret, uint64_t* out,
+                void* stream);
+
 /* ---- batched executor (no single counterpart in the crate: it runs many of the calls above) ----
  * Independent operations at any levels, of mixed kinds, are planned into grouped launches
  * (same kind, level and key -> one group whose kernels process all its ciphertexts together and

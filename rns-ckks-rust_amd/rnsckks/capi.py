@@ -61,6 +61,21 @@ EXPORTS = {
     "rck_tune_keyswitch": (ctypes.c_int, [vp, u32, vp, u32, ctypes.c_char_p, ctypes.c_size_t]),
     "rck_tune_table_export": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "rck_tune_table_import": (ctypes.c_int, [ctypes.c_char_p, u32p]),
+    "rck_rng_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(vp)]),
+    "rck_rng_destroy": (ctypes.c_int, [vp]),
+    "rck_rng_next": (ctypes.c_uint64, [vp]),
+    "rck_sample_uniform": (ctypes.c_int, [vp, vp, vp, u32, u32]),
+    "rck_sample_error": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, u32, u32]),
+    "rck_sample_ternary": (ctypes.c_int, [vp, vp, vp, u32, u32]),
+    "rck_sample_sparse_ternary": (ctypes.c_int, [vp, vp, u32, vp, u32, u32]),
+    "rck_keygen_public": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, u32, vp]),
+    "rck_keygen_relin": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, vp, u32, u32, ctypes.POINTER(vp)]),
+    "rck_keygen_rotation": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, vp, u32, ctypes.c_int64, u32,
+                                           ctypes.POINTER(vp)]),
+    "rck_ksk_export": (ctypes.c_int, [vp, vp, vp]),
+    "rck_ksk_info": (ctypes.c_int, [vp, u32p, u32p, u32p, ctypes.POINTER(ctypes.c_uint64)]),
+    "rck_encrypt": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, vp, u32, u32, vp]),
+    "rck_decrypt": (ctypes.c_int, [vp, vp, u32, vp, vp, vp]),
     "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
     "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -257,6 +272,25 @@ class Ksk:
             pass
 
 
+class Rng:
+    """rck_rng: the seeded sampling stream (xoshiro256** seeded through splitmix64)."""
+
+    def __init__(self, seed):
+        out = vp()
+        _check(lib().rck_rng_create(seed, ctypes.byref(out)))
+        self.ptr = out.value
+
+    def next(self):
+        return int(lib().rck_rng_next(self.ptr))
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.rck_rng_destroy(self.ptr)
+        except Exception:
+            pass
+
+
 class Context:
     """rck_context: RnsCkksContext's hot-path entry points on device buffers."""
 
@@ -352,6 +386,64 @@ class Context:
             digits = key.shape[0]
             _check(lib().rck_ksk_create(self.ptr, key.ctypes.data, digits, level, aux, galois, ctypes.byref(out)))
         return Ksk(out.value, digits, level, aux, galois)
+
+    # ---- seeded sampling, keys, encryption
+    def sample_uniform(self, rng, nq, np_=0):
+        out = DeviceBuffer((nq + np_, self.n))
+        _check(lib().rck_sample_uniform(self.ptr, rng.ptr, out.ptr, nq, np_))
+        return out
+
+    def sample_error(self, rng, nq, np_=0, sigma=3.19):
+        out = DeviceBuffer((nq + np_, self.n))
+        _check(lib().rck_sample_error(self.ptr, rng.ptr, sigma, out.ptr, nq, np_))
+        return out
+
+    def sample_ternary(self, rng, nq, np_=0):
+        out = DeviceBuffer((nq + np_, self.n))
+        _check(lib().rck_sample_ternary(self.ptr, rng.ptr, out.ptr, nq, np_))
+        return out
+
+    def sample_sparse_ternary(self, rng, hamming, nq, np_=0):
+        out = DeviceBuffer((nq + np_, self.n))
+        _check(lib().rck_sample_sparse_ternary(self.ptr, rng.ptr, hamming, out.ptr, nq, np_))
+        return out
+
+    def keygen_public(self, rng, secret, limbs, sigma=3.19):
+        pk = DeviceBuffer((2, limbs, self.n))
+        _check(lib().rck_keygen_public(self.ptr, rng.ptr, sigma, _p(secret), limbs, pk.ptr))
+        return pk
+
+    def _key_info(self, ptr):
+        d, l, a, g = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().rck_ksk_info(ptr, ctypes.byref(d), ctypes.byref(l), ctypes.byref(a), ctypes.byref(g)))
+        return Ksk(ptr, d.value, l.value, a.value, g.value)
+
+    def keygen_relin(self, rng, secret, level, pk=None, pk_limbs=0, sigma=3.19):
+        out = vp()
+        _check(lib().rck_keygen_relin(self.ptr, rng.ptr, sigma, _p(secret), _p(pk) if pk is not None else None,
+                                      pk_limbs, level, ctypes.byref(out)))
+        return self._key_info(out.value)
+
+    def keygen_rotation(self, rng, secret, steps, level, pk=None, pk_limbs=0, sigma=3.19):
+        out = vp()
+        _check(lib().rck_keygen_rotation(self.ptr, rng.ptr, sigma, _p(secret), _p(pk) if pk is not None else None,
+                                         pk_limbs, steps, level, ctypes.byref(out)))
+        return self._key_info(out.value)
+
+    def ksk_export(self, ksk):
+        out = np.zeros((ksk.digits, 2, ksk.level + 1 + ksk.aux, self.n), dtype=np.uint64)
+        _check(lib().rck_ksk_export(self.ptr, ksk.ptr, out.ctypes.data))
+        return out
+
+    def encrypt(self, rng, plain, pk, pk_limbs, limbs, sigma=3.19):
+        out = DeviceBuffer((2, limbs, self.n))
+        _check(lib().rck_encrypt(self.ptr, rng.ptr, sigma, _p(plain), _p(pk), pk_limbs, limbs, out.ptr))
+        return out
+
+    def decrypt(self, ct, limbs, secret, out=None, stream=None):
+        out = out if out is not None else DeviceBuffer((limbs, self.n))
+        _check(lib().rck_decrypt(self.ptr, _p(ct), limbs, _p(secret), _p(out), stream))
+        return out
 
     def tune_keyswitch(self, limbs, ksk, batch=1):
         import json
