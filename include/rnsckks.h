@@ -192,6 +192,37 @@ int rck_encrypt(rck_context* ctx, rck_rng* rng, double sigma, const uint64_t* pl
 int rck_decrypt(rck_context* ctx, const uint64_t* ct, uint32_t limbs, const uint64_t* secret, uint64_t* out,
                 void* stream);
 
+/* ---- encode / decode (host FFT; client side of the hot path) ----
+ * RnsCkksContext::encode_at_level_and_scale (src/rns/context/encoding.rs:24-35 ->
+ * encode_integer_coeffs, src/rns/encoding.rs:9-31): slots (re[i] + j im[i], i < N/2; im may be NULL)
+ * -> integer coefficients round(2^scale_bits * Re(twisted inverse embedding)) as i128 -> residues of
+ * the first `limbs` primes, written to DEVICE memory.  Bit-exact parity with the crate depends on
+ * num-complex 0.4's rounding and is not pinned; the CKKS error bounds are. */
+int rck_encode(rck_context* ctx, const double* re, const double* im, uint32_t slots, uint32_t limbs,
+               uint32_t scale_bits, uint64_t* out);
+/* RnsCkksContext::decode_at_scale (src/rns/context/encoding.rs:37-48 -> decode_integer_coeffs,
+ * src/rns/encoding.rs:33-66): CRT-reconstruct the `limbs`-limb DEVICE polynomial, centre against
+ * floor(Q/2), scale by 2^-scale_bits and evaluate at the slots; re/im (HOST, N/2 each; im may be NULL). */
+int rck_decode(rck_context* ctx, const uint64_t* poly, uint32_t limbs, uint32_t scale_bits, double* re, double* im);
+
+/* ---- Paterson-Stockmeyer polynomial evaluation (src/algorithms/paterson_stockmeyer.rs:28-350) ----
+ * coeffs[i] is the real coefficient of x^i (count >= 1).  relin_keys[level] (n_keys entries, NULL =
+ * missing) are relinearization keys at those levels, as the crate's BTreeMap<usize, key>; the keys
+ * needed are those of paterson_stockmeyer_required_levels.  The result has *out_limbs limbs and scale
+ * *out_scale (written to out: [c0][c1]).  The crate's panics (degree too large for the level budget,
+ * missing key, working scale below the input scale) return RCK_ERR_INVALID with its message.  The
+ * plaintexts the algorithm builds are constants: multiplying by / adding them is done exactly as
+ * scalar products / a coefficient-0 add, which equals the crate's encoded multiply_plain / add. */
+int rck_ps_block_size(uint32_t term_count, uint32_t* block_size);
+int rck_ps_required_levels(uint32_t ciphertext_level, uint32_t term_count, uint32_t* levels, uint32_t cap,
+                           uint32_t* count);
+/* Result metadata, ct x ct multiplications and device working-set bytes of an evaluation, without running it. */
+int rck_ps_plan(rck_context* ctx, uint32_t limbs, uint32_t scale_bits, const double* coeffs, uint32_t count,
+                uint32_t* out_limbs, uint32_t* out_scale, uint32_t* mults, uint64_t* footprint);
+int rck_ps_eval(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale_bits, const double* coeffs,
+                uint32_t count, const rck_ksk* const* relin_keys, uint32_t n_keys, uint64_t* out,
+                uint32_t* out_limbs, uint32_t* out_scale, void* stream);
+
 /* ---- batched executor (no single counterpart in the crate: it runs many of the calls above) ----
  * Independent operations at any levels, of mixed kinds, are planned into grouped launches
  * (same kind, level and key -> one group whose kernels process all its ciphertexts together and

@@ -42,8 +42,6 @@ struct BatchGroup {
     std::string tune_log;
 };
 
-enum { kSlotLhs = 10, kSlotRhs = 11, kSlotQuad = 12, kSlotTmp = 13 };
-
 }  // namespace
 
 struct rck_batch {
@@ -72,22 +70,7 @@ static int bguard(F&& f) {
     }
 }
 
-// Are ptr(op_0) .. ptr(op_{B-1}) equally spaced?  Returns the stride (elements) or -1.
-template <typename P>
-static long long common_stride(const std::vector<P>& ptrs) {
-    if (ptrs.size() < 2) return 0;
-    const long long st = (long long)(ptrs[1] - ptrs[0]);
-    if (st <= 0) return -1;
-    for (size_t i = 2; i < ptrs.size(); i++)
-        if ((long long)(ptrs[i] - ptrs[i - 1]) != st) return -1;
-    return st;
-}
-
 static void run_group(rck_batch* bt, const BatchGroup& g, hipStream_t s, KsShape shape) {
-    Context& c = *bt->sc;
-    const int B = (int)g.ops.size();
-    const int l = (int)g.limbs;
-    const long long n = c.n, P = (long long)l * n;
     std::vector<const uint64_t*> as, bs;
     std::vector<uint64_t*> outs;
     for (int i : g.ops) {
@@ -95,52 +78,7 @@ static void run_group(rck_batch* bt, const BatchGroup& g, hipStream_t s, KsShape
         bs.push_back(bt->ops[i].b);
         outs.push_back(bt->ops[i].out);
     }
-    switch (g.kind) {
-        case RCK_OP_MULT_RELIN:
-        case RCK_OP_MULT_RELIN_RESCALE: {
-            const long long sa = common_stride(as), sb = common_stride(bs);
-            const uint64_t *lhs = as[0], *rhs = bs[0];
-            long long st = sa;
-            if (B > 1 && !(sa == sb && sa >= 2 * P)) {  // stage both operands contiguously
-                uint64_t* lb = static_cast<uint64_t*>(c.workspace(kSlotLhs, (size_t)B * 2 * P * 8));
-                uint64_t* rb = static_cast<uint64_t*>(c.workspace(kSlotRhs, (size_t)B * 2 * P * 8));
-                for (int i = 0; i < B; i++) {
-                    RCK_CHECK_HIP(hipMemcpyAsync(lb + i * 2 * P, as[i], 2 * P * 8, hipMemcpyDeviceToDevice, s));
-                    RCK_CHECK_HIP(hipMemcpyAsync(rb + i * 2 * P, bs[i], 2 * P * 8, hipMemcpyDeviceToDevice, s));
-                }
-                lhs = lb;
-                rhs = rb;
-                st = 2 * P;
-            }
-            uint64_t* quad = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * 3 * P * 8));
-            mult_batch(c, lhs, rhs, st, B, l, quad, 3 * P, s);
-            if (g.kind == RCK_OP_MULT_RELIN) {
-                keyswitch_batch(c, quad + 2 * P, 3 * P, B, l, g.key, quad, quad + P, 3 * P, outs, s, shape);
-            } else {
-                uint64_t* rel = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
-                std::vector<uint64_t*> rels;
-                for (int i = 0; i < B; i++) rels.push_back(rel + i * 2 * P);
-                keyswitch_batch(c, quad + 2 * P, 3 * P, B, l, g.key, quad, quad + P, 3 * P, rels, s, shape);
-                for (int i = 0; i < B; i++) rescale_impl(c, rels[i], l, 2, outs[i], s);
-            }
-            break;
-        }
-        case RCK_OP_ROTATE: {
-            uint64_t* rot = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
-            PrimeMap pm{l, (int)c.L, 0};
-            for (int i = 0; i < B; i++)
-                launch_galois(c, {as[i], n, P}, {rot + i * 2 * P, n, P}, l, 2, g.key->galois, pm, s);
-            keyswitch_batch(c, rot + P, 2 * P, B, l, g.key, rot, nullptr, 2 * P, outs, s, shape);
-            break;
-        }
-        case RCK_OP_MULT_PLAIN: {
-            PrimeMap pm{l, (int)c.L, 0};
-            for (int i = 0; i < B; i++) poly_mul(c, as[i], P, bs[i], 0, outs[i], P, l, 2, pm, s);
-            break;
-        }
-        default:
-            RCK_REQUIRE(false, "unknown batch op kind");
-    }
+    run_op_group(*bt->sc, g.kind, (int)g.limbs, g.key, as, bs, outs, s, shape);
 }
 
 static void check_group(const Context& c, const BatchGroup& g) {

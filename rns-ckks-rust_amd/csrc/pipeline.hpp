@@ -39,6 +39,11 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
                      hipStream_t s, KsShape shape = {});
 void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const KeyDev* key, const uint64_t* add0,
                     const uint64_t* add1, uint64_t* out, hipStream_t s);
+// B same-kind operations at one level sharing a key (RCK_OP_* kinds of the C ABI): in0 as[b], in1 bs[b],
+// result outs[b].  Operands need not be contiguous (they are staged when they are not equally spaced).
+void run_op_group(Context& c, int kind, int limbs, const KeyDev* key, const std::vector<const uint64_t*>& as,
+                  const std::vector<const uint64_t*>& bs, const std::vector<uint64_t*>& outs, hipStream_t s,
+                  KsShape shape = {});
 // the automatic digit-MAC shape for B ciphertexts with E eval limbs and D digits: a tuned entry
 // when one exists for (N, limbs, E, D, B), otherwise a heuristic
 KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B);

@@ -76,6 +76,12 @@ EXPORTS = {
     "rck_ksk_info": (ctypes.c_int, [vp, u32p, u32p, u32p, ctypes.POINTER(ctypes.c_uint64)]),
     "rck_encrypt": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, vp, u32, u32, vp]),
     "rck_decrypt": (ctypes.c_int, [vp, vp, u32, vp, vp, vp]),
+    "rck_encode": (ctypes.c_int, [vp, vp, vp, u32, u32, u32, vp]),
+    "rck_decode": (ctypes.c_int, [vp, vp, u32, u32, vp, vp]),
+    "rck_ps_block_size": (ctypes.c_int, [u32, u32p]),
+    "rck_ps_required_levels": (ctypes.c_int, [u32, u32, u32p, u32, u32p]),
+    "rck_ps_plan": (ctypes.c_int, [vp, u32, u32, vp, u32, u32p, u32p, u32p, ctypes.POINTER(ctypes.c_uint64)]),
+    "rck_ps_eval": (ctypes.c_int, [vp, vp, u32, u32, vp, u32, vp, u32, vp, u32p, u32p, vp]),
     "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
     "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -234,6 +240,19 @@ def tune_table_import(text):
     k = ctypes.c_uint32(0)
     _check(lib().rck_tune_table_import(text.encode(), ctypes.byref(k)))
     return k.value
+
+
+def ps_block_size(term_count):
+    out = ctypes.c_uint32()
+    _check(lib().rck_ps_block_size(term_count, ctypes.byref(out)))
+    return out.value
+
+
+def ps_required_levels(ciphertext_level, term_count):
+    cnt = ctypes.c_uint32()
+    buf = (ctypes.c_uint32 * 128)()
+    _check(lib().rck_ps_required_levels(ciphertext_level, term_count, buf, 128, ctypes.byref(cnt)))
+    return [buf[i] for i in range(cnt.value)]
 
 
 def profile_enable(on=True):
@@ -429,6 +448,41 @@ class Context:
         _check(lib().rck_keygen_rotation(self.ptr, rng.ptr, sigma, _p(secret), _p(pk) if pk is not None else None,
                                          pk_limbs, steps, level, ctypes.byref(out)))
         return self._key_info(out.value)
+
+    def encode(self, slots, limbs, scale_bits, out=None):
+        """slots: N/2 complex (or real) values -> DeviceBuffer [limbs][N]"""
+        z = np.asarray(slots)
+        re = np.ascontiguousarray(z.real, dtype=np.float64)
+        im = np.ascontiguousarray(z.imag if np.iscomplexobj(z) else np.zeros_like(re), dtype=np.float64)
+        out = out if out is not None else DeviceBuffer((limbs, self.n))
+        _check(lib().rck_encode(self.ptr, re.ctypes.data, im.ctypes.data, re.size, limbs, scale_bits, _p(out)))
+        return out
+
+    def decode(self, poly, limbs, scale_bits):
+        """DeviceBuffer [limbs][N] -> N/2 complex slots"""
+        re = np.zeros(self.n // 2)
+        im = np.zeros(self.n // 2)
+        _check(lib().rck_decode(self.ptr, _p(poly), limbs, scale_bits, re.ctypes.data, im.ctypes.data))
+        return re + 1j * im
+
+    def ps_plan(self, limbs, scale_bits, coeffs):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        ol, os_, m, fp = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().rck_ps_plan(self.ptr, limbs, scale_bits, c.ctypes.data, c.size, ctypes.byref(ol),
+                                 ctypes.byref(os_), ctypes.byref(m), ctypes.byref(fp)))
+        return {"limbs": ol.value, "scale_bits": os_.value, "mults": m.value, "footprint": fp.value}
+
+    def ps_eval(self, ct, limbs, scale_bits, coeffs, relin_keys, out, stream=None):
+        """relin_keys: {level: Ksk}.  Returns (out_limbs, out_scale)."""
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        nk = (max(relin_keys) + 1) if relin_keys else 0
+        arr = (vp * max(nk, 1))()
+        for lv, k in relin_keys.items():
+            arr[lv] = k.ptr
+        ol, os_ = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().rck_ps_eval(self.ptr, _p(ct), limbs, scale_bits, c.ctypes.data, c.size, arr, nk, _p(out),
+                                 ctypes.byref(ol), ctypes.byref(os_), stream))
+        return ol.value, os_.value
 
     def ksk_export(self, ksk):
         out = np.zeros((ksk.digits, 2, ksk.level + 1 + ksk.aux, self.n), dtype=np.uint64)

@@ -105,3 +105,19 @@ def test_rng_stream_matches_oracle():
         s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45)
     r = capi.Rng(0)
     assert [r.next() for _ in range(5)] == want
+
+
+def test_ps_block_size_and_required_levels_kats():
+    """paterson_stockmeyer_block_size_is_ceil_sqrt / ..._required_levels_cover_all_multiplication_levels
+    (src/algorithms/paterson_stockmeyer.rs:353-366), for the library and the oracle restatement."""
+    from oracle import ps as OPS
+    for t, b in ((1, 1), (2, 2), (4, 2), (5, 3), (9, 3), (10, 4), (64, 8)):
+        assert capi.ps_block_size(t) == b == OPS.block_size(t)
+    assert capi.ps_required_levels(7, 1) == [] == OPS.required_levels(7, 1)
+    assert capi.ps_required_levels(7, 5) == [7, 6, 5] == OPS.required_levels(7, 5)
+    # degree 63: block 8, 8 blocks -> 14 multiplication levels, needs level >= 14
+    assert capi.ps_required_levels(14, 64) == list(range(14, 0, -1)) == OPS.required_levels(14, 64)
+    with pytest.raises(capi.RckError, match="polynomial degree is too large"):
+        capi.ps_required_levels(13, 64)
+    with pytest.raises(OPS.PsError, match="polynomial degree is too large"):
+        OPS.required_levels(13, 64)
