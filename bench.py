@@ -82,7 +82,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed batches (batched) / ops (per-op)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="batched", choices=["batched", "per-op"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "per-op", "serve"])
+    ap.add_argument("--requests", type=int, default=128, help="serve: requests in the Poisson stream")
+    ap.add_argument("--qps", type=float, default=250.0, help="serve: mean arrival rate (requests/s)")
+    ap.add_argument("--serve-budget-gb", type=float, default=32.0, help="serve: working-set budget (GB)")
+    ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent stream lanes")
     ap.add_argument("--batch", type=int, default=16, help="ciphertext pairs per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -109,6 +113,10 @@ def main():
             G.diag_copy(dst, src, nbytes)
         G.synchronize()
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 5}), flush=True)
+        return
+
+    if args.mode == "serve":
+        serve_main(args, G, PR)
         return
 
     p = PR.config_per_op_gpu()
@@ -233,6 +241,134 @@ def main():
         "kernels_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in top},
     }
     print(json.dumps(line), flush=True)
+
+
+def serve_main(args, G, PR):
+    """Serving loop: Poisson arrivals of Paterson-Stockmeyer requests (degree 7..63 capped by the level
+    budget, coefficients U[-1,1]) on fresh ciphertexts at N in {2^15, 2^16} with 12..24 starting
+    limbs, keys and ciphertexts resident on the device, admission by working-set bytes."""
+    rs = np.random.default_rng(0)
+    rings = {}
+    t0 = time.perf_counter()
+    plans = []
+    for _ in range(args.requests):
+        n = int(rs.choice([1 << 15, 1 << 16]))
+        limbs = int(rs.integers(12, 25))
+        deg = int(rs.integers(7, 64))
+        while True:  # cap the degree by the level budget, keeping 2 limbs for the result
+            try:     # (at 1 limb Q ~ 2^50 equals the working scale and the result cannot decode)
+                G.ps_required_levels(limbs - 2, deg + 1)
+                break
+            except G.RckError:
+                deg -= 1
+        plans.append((n, limbs, rs.uniform(-1, 1, deg + 1)))
+    need = {}
+    for n, limbs, coeffs in plans:
+        need.setdefault(n, set()).update(G.ps_required_levels(limbs - 1, len(coeffs)))
+    for n in sorted(need):
+        p = PR.config_batched(n=n, limbs=24, aux=2)
+        ctx = G.Context(p)
+        rng = G.Rng(n)
+        s = ctx.sample_sparse_ternary(rng, 64, 24)
+        pk = ctx.keygen_public(rng, s, 24, sigma=3.2)
+        keys = {lv: ctx.keygen_relin(rng, s, lv, sigma=3.2) for lv in sorted(need[n])}
+        rings[n] = (p, ctx, rng, s, pk, keys)
+        log(f"[bench] ring N={n}: {len(keys)} relinearization keys ({min(keys)}..{max(keys)}) "
+            f"generated on the device in {time.perf_counter() - t0:.1f} s")
+    # client side: encode + encrypt every request before the stream starts
+    reqs = []
+    for i, (n, limbs, coeffs) in enumerate(plans):
+        p, ctx, rng, s, pk, keys = rings[n]
+        x = rs.uniform(-1, 1, n // 2)
+        ct = ctx.encrypt(rng, ctx.encode(x, limbs, 40), pk, 24, limbs, sigma=3.2)
+        reqs.append({"n": n, "limbs": limbs, "coeffs": coeffs, "x": x, "ct": ct,
+                     "out": G.DeviceBuffer((2, limbs, n))})
+    log(f"[bench] {len(reqs)} requests encrypted ({time.perf_counter() - t0:.1f} s); "
+        f"degrees {min(len(r['coeffs']) - 1 for r in reqs)}..{max(len(r['coeffs']) - 1 for r in reqs)}")
+    budget = int(args.serve_budget_gb * 1e9)
+    srv = G.Server(budget)
+    for n, (p, ctx, rng, s, pk, keys) in rings.items():
+        srv.register_ring(ctx, keys)
+    # warm-up: one request per ring through the server (first-launch costs), not counted
+    for n, (p, ctx, rng, s, pk, keys) in rings.items():
+        r = next(q for q in reqs if q["n"] == n)
+        rid = srv.submit(ctx, r["ct"], r["limbs"], 40, r["coeffs"], r["out"])
+        srv.wait(rid, 120)
+    srv.close()
+    srv = G.Server(budget, lanes=args.lanes)
+    for n, (p, ctx, rng, s, pk, keys) in rings.items():
+        srv.register_ring(ctx, keys)
+    gaps = rs.exponential(1.0 / args.qps, len(reqs))
+    start = time.perf_counter()
+    due = start
+    ids = []
+    for r, gap in zip(reqs, gaps):
+        due += gap
+        delay = due - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(rings[r["n"]][1], r["ct"], r["limbs"], 40, r["coeffs"], r["out"]))
+    srv.drain(600)
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    log(f"[bench] served {st['completed']} requests in {wall:.2f} s: p50 {st['p50_ms']:.1f} ms, "
+        f"p99 {st['p99_ms']:.1f} ms, {st['throughput_rps']:.1f} req/s, {st['mult_launches']} batched launches "
+        f"for {st['mults']} ct x ct mults, peak working set {st['peak_bytes'] / 1e9:.1f} GB")
+    # decode error on a sample, against FP64 evaluation
+    errs = []
+    for r, rid in list(zip(reqs, ids))[:8]:
+        w = srv.wait(rid, 1.0)
+        p, ctx, rng, s, pk, keys = rings[r["n"]]
+        dec = ctx.decrypt(r["out"], w["limbs"], s)
+        got = ctx.decode(dec, w["limbs"], w["scale_bits"]).real
+        errs.append(float(np.max(np.abs(got - np.polyval(r["coeffs"][::-1], r["x"])))))
+    srv.close()
+    log(f"[bench] decode error vs FP64 over {len(errs)} requests: max {max(errs):.2e}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_ps(args.cpu_threads)
+    line = {
+        "metric": "Paterson-Stockmeyer serving: requests/s with p50/p99 latency (continuous batching)",
+        "value": round(st["throughput_rps"], 2), "unit": "requests/s", "n_gpus": 1, "steps": len(reqs), "warmup": 1,
+        "ms_per_step": round(st["span_ms"] / max(1, st["completed"]), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64 residues (FP64-exact modular arithmetic)",
+        "data": "synthetic (seeded Poisson arrivals, uniform coefficients and inputs)",
+        "config": {"workload": "PS degree 7..63 (capped by level budget), N in {2^15, 2^16}, start limbs 12..24, "
+                               "primes just above 2^50, scale 2^40",
+                   "qps": args.qps, "requests": len(reqs), "budget_bytes": budget,
+                   "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+                   "queue_p99_ms": round(st["queue_p99_ms"], 2), "mults": st["mults"],
+                   "mult_launches": st["mult_launches"], "batch_hist": st["batch_hist"],
+                   "peak_bytes": st["peak_bytes"], "decode_err_max": max(errs)},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_ps(threads):
+    """The oracle's Paterson-Stockmeyer restatement on one degree-7 request at N=2^15, 12 limbs
+    (uniform ciphertext and keys: the arithmetic is the same for any residues)."""
+    from oracle import oracle as O
+    from oracle import ps as OPS
+    from rnsckks import params as PR
+    O.set_threads(threads)
+    n, L = 1 << 15, 12
+    p = PR.config_batched(n=n, limbs=L, aux=2)
+    rs = np.random.default_rng(3)
+    coeffs = list(rs.uniform(-1, 1, 8))
+    keys = {}
+    for lv in OPS.required_levels(L - 1, len(coeffs)):
+        D = O.num_decomp_digits(O.Basis(n, p.primes[: lv + 1], p.roots[: lv + 1]).total_modulus_bits())
+        keys[lv] = np.stack([uniform(p.primes[: lv + 1] + p.aux_primes, n, rs, 2) for _ in range(D)])
+    R = OPS.Ring(n, p.primes, p.roots, p.aux_primes, p.aux_roots, keys)
+    ct = uniform(p.primes, n, rs, 2)
+    log(f"[bench] cpu baseline: 1 degree-7 PS request (N=2^15, 12 limbs) on {threads} threads")
+    t = time.perf_counter()
+    OPS.ps_eval(R, (ct, 40), coeffs)
+    dt = time.perf_counter() - t
+    log(f"[bench] cpu baseline {dt:.2f} s/request")
+    return {"value": round(1.0 / dt, 5), "unit": "requests/s", "cores": threads, "kind": "port",
+            "sample": "1 degree-7 Paterson-Stockmeyer request at N=2^15, 12 limbs (oracle, OpenMP)"}
 
 
 def cpu_baseline(p, threads):

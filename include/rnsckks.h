@@ -223,6 +223,33 @@ int rck_ps_eval(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t s
                 uint32_t count, const rck_ksk* const* relin_keys, uint32_t n_keys, uint64_t* out,
                 uint32_t* out_limbs, uint32_t* out_scale, void* stream);
 
+/* ---- request server (no counterpart in the crate: it serves many paterson_stockmeyer_eval calls) ----
+ * Requests are Paterson-Stockmeyer evaluations (same arguments and result as rck_ps_eval).  A worker
+ * thread admits them in arrival order while their device working sets (rck_ps_plan's footprint) fit
+ * hbm_budget_bytes (and at most max_active run, 0 = unlimited), advances all admitted requests
+ * together, and runs the ct x ct multiplications that share a ring, a level and a key as one
+ * batched launch.  A result is bit-identical to rck_ps_eval of the same request.  `ct` and `out`
+ * are caller-owned DEVICE buffers that must stay valid until the request completes.  Register every
+ * ring (context + relinearization keys by level) before the first submit. */
+typedef struct rck_server rck_server;
+/* lanes (1..8): streams on which independent multiplication groups of a round run concurrently,
+ * each with its own scratch workspace per ring. */
+int rck_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t lanes, rck_server** out);
+int rck_server_register_ring(rck_server* server, rck_context* ctx, const rck_ksk* const* relin_keys,
+                             uint32_t n_keys);
+int rck_server_submit(rck_server* server, rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t scale_bits,
+                      const double* coeffs, uint32_t count, uint64_t* out, uint64_t* request_id);
+/* Waits up to timeout_s for a request; *done = 1 when it completed (then out_limbs / out_scale /
+ * latency_ms = submit-to-completion time are set). */
+int rck_server_wait(rck_server* server, uint64_t request_id, double timeout_s, int* done, uint32_t* out_limbs,
+                    uint32_t* out_scale, double* latency_ms);
+/* Waits until every submitted request completed (error on time-out). */
+int rck_server_drain(rck_server* server, double timeout_s);
+/* JSON: completed, p50/p99/mean/max latency (ms), queueing, throughput, rounds, batched launches,
+ * peak working-set bytes, histogram of multiplications per launch. */
+int rck_server_stats(rck_server* server, char* buf, size_t len);
+int rck_server_destroy(rck_server* server);
+
 /* ---- batched executor (no single counterpart in the crate: it runs many of the calls above) ----
  * Independent operations at any levels, of mixed kinds, are planned into grouped launches
  * (same kind, level and key -> one group whose kernels process all its ciphertexts together and

@@ -82,6 +82,14 @@ EXPORTS = {
     "rck_ps_required_levels": (ctypes.c_int, [u32, u32, u32p, u32, u32p]),
     "rck_ps_plan": (ctypes.c_int, [vp, u32, u32, vp, u32, u32p, u32p, u32p, ctypes.POINTER(ctypes.c_uint64)]),
     "rck_ps_eval": (ctypes.c_int, [vp, vp, u32, u32, vp, u32, vp, u32, vp, u32p, u32p, vp]),
+    "rck_server_create": (ctypes.c_int, [ctypes.c_uint64, u32, u32, ctypes.POINTER(vp)]),
+    "rck_server_register_ring": (ctypes.c_int, [vp, vp, vp, u32]),
+    "rck_server_submit": (ctypes.c_int, [vp, vp, vp, u32, u32, vp, u32, vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "rck_server_wait": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int), u32p,
+                                       u32p, ctypes.POINTER(ctypes.c_double)]),
+    "rck_server_drain": (ctypes.c_int, [vp, ctypes.c_double]),
+    "rck_server_stats": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+    "rck_server_destroy": (ctypes.c_int, [vp]),
     "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
     "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -565,5 +573,58 @@ class Batch:
         try:
             if getattr(self, "ptr", None) and _lib is not None:
                 _lib.rck_batch_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Server:
+    """rck_server: continuous-batching request loop for Paterson-Stockmeyer evaluations."""
+
+    def __init__(self, budget_bytes, max_active=0, lanes=4):
+        out = vp()
+        _check(lib().rck_server_create(int(budget_bytes), max_active, lanes, ctypes.byref(out)))
+        self.ptr = out.value
+        self._keep = []
+
+    def register_ring(self, ctx, relin_keys):
+        nk = (max(relin_keys) + 1) if relin_keys else 0
+        arr = (vp * max(nk, 1))()
+        for lv, k in relin_keys.items():
+            arr[lv] = k.ptr
+        self._keep.append((ctx, dict(relin_keys)))
+        _check(lib().rck_server_register_ring(self.ptr, ctx.ptr, arr, nk))
+
+    def submit(self, ctx, ct, limbs, scale_bits, coeffs, out):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        rid = ctypes.c_uint64()
+        _check(lib().rck_server_submit(self.ptr, ctx.ptr, _p(ct), limbs, scale_bits, c.ctypes.data, c.size, _p(out),
+                                       ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout=60.0):
+        done, ol, osc, lat = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        _check(lib().rck_server_wait(self.ptr, rid, timeout, ctypes.byref(done), ctypes.byref(ol), ctypes.byref(osc),
+                                     ctypes.byref(lat)))
+        if not done.value:
+            return None
+        return {"limbs": ol.value, "scale_bits": osc.value, "latency_ms": lat.value}
+
+    def drain(self, timeout=600.0):
+        _check(lib().rck_server_drain(self.ptr, timeout))
+
+    def stats(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().rck_server_stats(self.ptr, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if self.ptr and _lib is not None:
+            _check(_lib.rck_server_destroy(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
