@@ -223,6 +223,30 @@ int rck_ps_eval(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t s
                 uint32_t count, const rck_ksk* const* relin_keys, uint32_t n_keys, uint64_t* out,
                 uint32_t* out_limbs, uint32_t* out_scale, void* stream);
 
+/* ---- baby-step giant-step diagonal transform (src/ckks/bootstrapping/linear_transform.rs:22-253,
+ *      src/algorithms/packed_diagonal.rs:29-77) ----
+ * diagonal_transform_bsgs_rotation_steps (:27-47): the rotation steps whose keys the transform uses. */
+int rck_bsgs_rotation_steps(uint32_t dim, uint32_t* steps, uint32_t cap, uint32_t* count);
+/* pack_diagonal_transform_plaintexts' giant-step diagonals (:102-116, 161-280) of a real dim x dim
+ * matrix (HOST, row-major), encoded at (level, scale_bits) with rck_encode's encoder and kept in HBM
+ * in NTT form.  dim: a power of two, at most N/2 and 4096. */
+typedef struct rck_diag rck_diag;
+int rck_diag_pack(rck_context* ctx, const double* matrix, uint32_t dim, uint32_t level, uint32_t scale_bits,
+                  rck_diag** out);
+int rck_diag_info(const rck_diag* diag, uint32_t* dim, uint32_t* baby_step, uint32_t* giant_steps, uint32_t* level,
+                  uint32_t* scale_bits);
+/* The encoded diagonals in coefficient form, uint64_t[giant_steps][baby_step][level+1][N] (HOST;
+ * diagonal steps >= dim are zero). */
+int rck_diag_export(const rck_diag* diag, uint64_t* out);
+int rck_diag_destroy(rck_diag* diag);
+/* apply_diagonal_linear_transform (:118-144) on `batch` ciphertexts cts[b] = [c0][c1] (DEVICE,
+ * contiguous, `limbs` limbs, scale scale_bits) with keys_by_step[step] (n_keys entries, NULL = missing;
+ * rotation keys at level limbs-1): out[b] (limbs-1 limbs, contiguous), *out_scale = 2 * scale_bits -
+ * floor(log2 q_{limbs-1}) (mult_plain doubles the scale, then one rescale).  Synchronizes `stream`. */
+int rck_bsgs_matvec(rck_context* ctx, const uint64_t* cts, uint32_t batch, uint32_t limbs, uint32_t scale_bits,
+                    const rck_diag* diag, const rck_ksk* const* keys_by_step, uint32_t n_keys, uint64_t* out,
+                    uint32_t* out_scale, void* stream);
+
 /* ---- request server (no counterpart in the crate: it serves many paterson_stockmeyer_eval calls) ----
  * Requests are Paterson-Stockmeyer evaluations (same arguments and result as rck_ps_eval).  A worker
  * thread admits them in arrival order while their device working sets (rck_ps_plan's footprint) fit

@@ -61,6 +61,11 @@ PsProgram ps_compile(const Context& c, int limbs, uint32_t scale_bits, const dou
 void ps_exec(Context& c, const PsProgram& p, const PsInstr& ins, const std::vector<uint64_t*>& regs,
              const std::vector<const KeyDev*>& keys, hipStream_t s);
 
+// encode_integer_coeffs + residues of the first `limbs` primes into host memory out[limbs][N]
+// (threads: split the coefficient loop over host threads)
+void encode_host(const Context& c, const double* re, const double* im, uint32_t limbs, uint32_t scale_bits,
+                 uint64_t* out, bool threads);
+
 // round(c * 2^s) mod q_i for the first `limbs` primes (the encoding of a constant slot vector)
 std::vector<uint64_t> const_residues(const Context& c, double value, uint32_t scale_bits, int limbs);
 

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/rnsckks.h"
+#include "algorithms.hpp"
 #include "capi_types.hpp"
 
 namespace rck {
@@ -188,6 +189,38 @@ double big_to_f64(const Big& x) {
 }
 
 }  // namespace
+
+void encode_host(const Context& c, const double* re, const double* im, uint32_t limbs, uint32_t scale_bits,
+                 uint64_t* out, bool threads) {
+    const size_t n = c.n;
+    std::vector<Cx> ev(n, Cx{0.0, 0.0});
+    size_t cur = 1;
+    for (size_t i = 0; i < n / 2; i++) {
+        const Cx v{re[i], im ? im[i] : 0.0};
+        ev[(cur - 1) / 2] = v;
+        ev[((2 * n - cur) % (2 * n) - 1) / 2] = Cx{v.re, -v.im};
+        cur = cur * 5 % (2 * n);
+    }
+    fft(ev, true);
+    const Cx xi = polar(1.0, M_PI / (double)n);
+    const double scale = std::ldexp(1.0, (int)scale_bits);
+    auto one = [&](size_t j) {
+        const Cx tw = cmul(ev[j], cpowi(xi, -(int32_t)j));
+        const __int128 v = round_to_i128(std::fma(tw.re, scale, 0.0));
+        for (uint32_t l = 0; l < limbs; l++) {
+            const __int128 q = (__int128)c.hp[l].q;
+            __int128 r = v % q;
+            if (r < 0) r += q;
+            out[(size_t)l * n + j] = (uint64_t)r;
+        }
+    };
+    if (threads) {
+        parallel_for(n, one);
+    } else {
+        for (size_t j = 0; j < n; j++) one(j);
+    }
+}
+
 }  // namespace rck
 
 using namespace rck;
@@ -202,29 +235,8 @@ int rck_encode(rck_context* ctx, const double* re, const double* im, uint32_t sl
         RCK_REQUIRE(re != nullptr && out != nullptr, "null argument");
         RCK_REQUIRE(slots == c.n / 2, "slot count must be N/2");
         RCK_REQUIRE(limbs >= 1 && limbs <= c.L, "requested level context must exist for encoding");
-        const size_t n = c.n;
-        std::vector<Cx> ev(n, Cx{0.0, 0.0});
-        size_t cur = 1;
-        for (size_t i = 0; i < n / 2; i++) {
-            const Cx v{re[i], im ? im[i] : 0.0};
-            ev[(cur - 1) / 2] = v;
-            ev[((2 * n - cur) % (2 * n) - 1) / 2] = Cx{v.re, -v.im};
-            cur = cur * 5 % (2 * n);
-        }
-        fft(ev, true);
-        const Cx xi = polar(1.0, M_PI / (double)n);
-        const double scale = std::ldexp(1.0, (int)scale_bits);
-        std::vector<uint64_t> host((size_t)limbs * n);
-        parallel_for(n, [&](size_t j) {
-            const Cx tw = cmul(ev[j], cpowi(xi, -(int32_t)j));
-            const __int128 v = round_to_i128(std::fma(tw.re, scale, 0.0));
-            for (uint32_t l = 0; l < limbs; l++) {
-                const __int128 q = (__int128)c.hp[l].q;
-                __int128 r = v % q;
-                if (r < 0) r += q;
-                host[(size_t)l * n + j] = (uint64_t)r;
-            }
-        });
+        std::vector<uint64_t> host((size_t)limbs * c.n);
+        encode_host(c, re, im, limbs, scale_bits, host.data(), true);
         RCK_CHECK_HIP(hipMemcpy(out, host.data(), host.size() * 8, hipMemcpyHostToDevice));
         return RCK_OK;
     } catch (const Error& e) {

@@ -90,6 +90,12 @@ EXPORTS = {
     "rck_server_drain": (ctypes.c_int, [vp, ctypes.c_double]),
     "rck_server_stats": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
     "rck_server_destroy": (ctypes.c_int, [vp]),
+    "rck_bsgs_rotation_steps": (ctypes.c_int, [u32, u32p, u32, u32p]),
+    "rck_diag_pack": (ctypes.c_int, [vp, vp, u32, u32, u32, ctypes.POINTER(vp)]),
+    "rck_diag_info": (ctypes.c_int, [vp, u32p, u32p, u32p, u32p, u32p]),
+    "rck_diag_export": (ctypes.c_int, [vp, vp]),
+    "rck_diag_destroy": (ctypes.c_int, [vp]),
+    "rck_bsgs_matvec": (ctypes.c_int, [vp, vp, u32, u32, u32, vp, vp, u32, vp, u32p, vp]),
     "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
     "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -261,6 +267,39 @@ def ps_required_levels(ciphertext_level, term_count):
     buf = (ctypes.c_uint32 * 128)()
     _check(lib().rck_ps_required_levels(ciphertext_level, term_count, buf, 128, ctypes.byref(cnt)))
     return [buf[i] for i in range(cnt.value)]
+
+
+def bsgs_rotation_steps(dim):
+    cnt = ctypes.c_uint32()
+    buf = (ctypes.c_uint32 * 4096)()
+    _check(lib().rck_bsgs_rotation_steps(dim, buf, 4096, ctypes.byref(cnt)))
+    return [buf[i] for i in range(cnt.value)]
+
+
+class Diag:
+    """rck_diag: packed BSGS diagonals of a matrix, NTT-resident."""
+
+    def __init__(self, ctx, matrix, level, scale_bits):
+        m = np.ascontiguousarray(matrix, dtype=np.float64)
+        assert m.ndim == 2 and m.shape[0] == m.shape[1]
+        out = vp()
+        _check(lib().rck_diag_pack(ctx.ptr, m.ctypes.data, m.shape[0], level, scale_bits, ctypes.byref(out)))
+        self.ptr, self.ctx = out.value, ctx
+        v = [ctypes.c_uint32() for _ in range(5)]
+        _check(lib().rck_diag_info(self.ptr, *[ctypes.byref(x) for x in v]))
+        self.dim, self.baby_step, self.giant_steps, self.level, self.scale_bits = [x.value for x in v]
+
+    def export(self):
+        out = np.zeros((self.giant_steps, self.baby_step, self.level + 1, self.ctx.n), dtype=np.uint64)
+        _check(lib().rck_diag_export(self.ptr, out.ctypes.data))
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.rck_diag_destroy(self.ptr)
+        except Exception:
+            pass
 
 
 def profile_enable(on=True):
@@ -491,6 +530,17 @@ class Context:
         _check(lib().rck_ps_eval(self.ptr, _p(ct), limbs, scale_bits, c.ctypes.data, c.size, arr, nk, _p(out),
                                  ctypes.byref(ol), ctypes.byref(os_), stream))
         return ol.value, os_.value
+
+    def bsgs_matvec(self, cts, batch, limbs, scale_bits, diag, keys_by_step, out, stream=None):
+        """keys_by_step: {step: Ksk}.  Returns the output scale."""
+        nk = (max(keys_by_step) + 1) if keys_by_step else 0
+        arr = (vp * max(nk, 1))()
+        for st, k in keys_by_step.items():
+            arr[st] = k.ptr
+        sc = ctypes.c_uint32()
+        _check(lib().rck_bsgs_matvec(self.ptr, _p(cts), batch, limbs, scale_bits, diag.ptr, arr, nk, _p(out),
+                                     ctypes.byref(sc), stream))
+        return sc.value
 
     def ksk_export(self, ksk):
         out = np.zeros((ksk.digits, 2, ksk.level + 1 + ksk.aux, self.n), dtype=np.uint64)

@@ -121,3 +121,13 @@ def test_ps_block_size_and_required_levels_kats():
         capi.ps_required_levels(13, 64)
     with pytest.raises(OPS.PsError, match="polynomial degree is too large"):
         OPS.required_levels(13, 64)
+
+
+def test_bsgs_rotation_steps_kats():
+    """diagonal_transform_bsgs_steps_cover_baby_and_giant_rotations (linear_transform.rs:315-320)."""
+    from oracle import bsgs as OB
+    for dim, want in ((1, []), (4, [1, 2]), (8, [1, 2, 3, 6])):
+        assert capi.bsgs_rotation_steps(dim) == want == OB.rotation_steps(dim)
+    steps = capi.bsgs_rotation_steps(4096)
+    assert len(steps) == 126 and steps == OB.rotation_steps(4096)
+    assert steps[:63] == list(range(1, 64)) and steps[63:] == [64 * g for g in range(1, 64)]
