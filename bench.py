@@ -82,7 +82,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed batches (batched) / ops (per-op)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="batched", choices=["batched", "per-op", "serve"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "per-op", "serve", "mixed", "bsgs"])
+    ap.add_argument("--cts", type=int, default=256, help="mixed: ciphertexts in the batch")
+    ap.add_argument("--dim", type=int, default=4096, help="bsgs: matrix dimension")
+    ap.add_argument("--vectors", type=int, default=32, help="bsgs: encrypted vectors per batch")
     ap.add_argument("--requests", type=int, default=128, help="serve: requests in the Poisson stream")
     ap.add_argument("--qps", type=float, default=250.0, help="serve: mean arrival rate (requests/s)")
     ap.add_argument("--serve-budget-gb", type=float, default=32.0, help="serve: working-set budget (GB)")
@@ -117,6 +120,12 @@ def main():
 
     if args.mode == "serve":
         serve_main(args, G, PR)
+        return
+    if args.mode == "mixed":
+        mixed_main(args, G, PR)
+        return
+    if args.mode == "bsgs":
+        bsgs_main(args, G, PR)
         return
 
     p = PR.config_per_op_gpu()
@@ -343,6 +352,196 @@ def serve_main(args, G, PR):
         "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def mixed_main(args, G, PR):
+    """Batched form of a mixed workload: ciphertexts at N=2^15 at independent levels of 4..20 limbs,
+    40% ct x ct mult+relin, 40% rotation by +-2^k (k < 14) with its Galois key, 20% ct x pt; one
+    captured hipGraph (autotuned launch shapes) against the same ops run one by one."""
+    n = 1 << 15
+    p = PR.config_batched(n=n, limbs=20, aux=2)
+    ctx = G.Context(p)
+    rs = np.random.default_rng(0)
+    rng = G.Rng(3)
+    t0 = time.perf_counter()
+    s = ctx.sample_ternary(rng, 20)
+    kinds = rs.permutation(["mr"] * int(0.4 * args.cts) + ["rot"] * int(0.4 * args.cts) +
+                           ["pt"] * (args.cts - 2 * int(0.4 * args.cts)))
+    ops = []
+    for kind in kinds:
+        limbs = int(rs.integers(4, 21))
+        step = int(rs.choice([1 << k for k in range(14)])) * int(rs.choice([-1, 1])) if kind == "rot" else 0
+        ops.append((kind, limbs, step))
+    keys = {}
+    for kind, limbs, step in ops:
+        if kind == "pt":
+            continue
+        kk = (limbs, step)
+        if kk not in keys:
+            keys[kk] = (ctx.keygen_rotation(rng, s, step, limbs - 1) if step else ctx.keygen_relin(rng, s, limbs - 1))
+    log(f"[bench] mixed: {len(ops)} ops, {len(keys)} keys generated on the device in {time.perf_counter() - t0:.1f} s")
+    bufs = []
+    bt = G.Batch(ctx)
+    for kind, limbs, step in ops:
+        a = G.DeviceBuffer.from_numpy(uniform(p.primes[:limbs], n, rs, 2))
+        out = G.DeviceBuffer((2, limbs, n))
+        if kind == "mr":
+            b = G.DeviceBuffer.from_numpy(uniform(p.primes[:limbs], n, rs, 2))
+            bt.add(G.OP_MULT_RELIN, limbs, a, b, keys[(limbs, 0)], out)
+        elif kind == "rot":
+            b = None
+            bt.add(G.OP_ROTATE, limbs, a, None, keys[(limbs, step)], out)
+        else:
+            b = G.DeviceBuffer.from_numpy(uniform(p.primes[:limbs], n, rs, 1)[0])
+            bt.add(G.OP_MULT_PLAIN, limbs, a, b, None, out)
+        bufs.append((kind, limbs, step, a, b, out))
+    t = time.perf_counter()
+    bt.compile(tune=not args.no_tune)
+    rep = bt.report()
+    log(f"[bench] compiled {len(rep['groups'])} groups in {time.perf_counter() - t:.1f} s")
+    stream = G.Stream()
+    for _ in range(args.warmup):
+        bt.run(stream)
+    ms = time_it(G, lambda: bt.run(stream), args.steps, stream) / args.steps
+    batched = [o[5].numpy() for o in bufs]
+    quad = {}
+    refs = []
+
+    def per_op():
+        for kind, limbs, step, a, b, out in bufs:
+            if kind == "mr":
+                if limbs not in quad:
+                    quad[limbs] = G.DeviceBuffer((3, limbs, n))
+                ctx.mult(a, b, limbs, 40, 40, quad[limbs])
+                ctx.relinearize(quad[limbs], limbs, 80, keys[(limbs, 0)], out)
+            elif kind == "rot":
+                ctx.rotate(a, limbs, 40, keys[(limbs, step)], out)
+            else:
+                ctx.mult_plain(a, b, limbs, 40, out)
+
+    for _ in range(args.warmup):
+        per_op()
+    ms_op = time_it(G, per_op, max(1, args.steps // 2)) / max(1, args.steps // 2)
+    same = all(np.array_equal(o[5].numpy(), batched[i]) for i, o in enumerate(bufs))
+    value = len(ops) * 1000.0 / ms
+    log(f"[bench] mixed batch: {ms:.3f} ms per {len(ops)} ops = {value:.1f} ops/s; per-op {ms_op:.3f} ms "
+        f"({ms_op / ms:.2f}x); batched == per-op: {same}")
+    G.profile_enable(True)
+    bt.run_eager()
+    prof = G.profile_report()
+    G.profile_enable(False)
+    line = {
+        "metric": "mixed batch ops/s (mult+relin / rotate / ct x pt at uneven levels), batched vs per-op; bit-exact",
+        "value": round(value, 2), "unit": "ops/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 residues (FP64-exact modular arithmetic)", "data": "synthetic (seeded uniform residues)",
+        "config": {"workload": f"{len(ops)} ciphertexts at N=2^15, 4..20 limbs (primes > 2^50, K=2), 40% mult+relin, "
+                               "40% rotate +-2^k, 20% ct x pt, one hipGraph",
+                   "groups": len(rep["groups"]), "keys": len(keys), "per_op_ms": round(ms_op, 4),
+                   "batched_vs_per_op": round(ms_op / ms, 3), "bit_exact_vs_per_op": bool(same)},
+        "roofline": None, "cpu_baseline": None,
+        "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bsgs_main(args, G, PR):
+    """Batched encrypted BSGS diagonal matvec: N=2^14 (8192 slots), a dense dim x dim matrix U[-1,1]
+    (baby step = giant step = sqrt(dim)), 12 limbs, a batch of encrypted vectors U[-1,1] repeated to
+    fill the slots; decode error against FP64."""
+    n, L, dim, B = 1 << 14, 12, args.dim, args.vectors
+    p = PR.config_batched(n=n, limbs=L, aux=2)
+    ctx = G.Context(p)
+    rs = np.random.default_rng(0)
+    rng = G.Rng(4)
+    t0 = time.perf_counter()
+    s = ctx.sample_sparse_ternary(rng, 64, L)
+    pk = ctx.keygen_public(rng, s, L, sigma=3.2)
+    steps = G.bsgs_rotation_steps(dim)
+    keys = {st: ctx.keygen_rotation(rng, s, st, L - 1, sigma=3.2) for st in steps}
+    log(f"[bench] bsgs: {len(keys)} rotation keys generated on the device in {time.perf_counter() - t0:.1f} s")
+    matrix = rs.uniform(-1, 1, (dim, dim))
+    t = time.perf_counter()
+    diag = G.Diag(ctx, matrix, L - 1, 40)
+    log(f"[bench] {dim} diagonals encoded (baby step {diag.baby_step}, {diag.giant_steps} giant steps) in "
+        f"{time.perf_counter() - t:.1f} s")
+    vecs = rs.uniform(-1, 1, (B, dim))
+    cts = G.DeviceBuffer((B, 2, L, n))
+    CT = 2 * L * n * 8
+    for v in range(B):
+        ct = ctx.encrypt(rng, ctx.encode(np.tile(vecs[v], n // 2 // dim), L, 40), pk, L, L, sigma=3.2)
+        G.lib().rck_memcpy_dtod(cts.ptr + v * CT, ct.ptr, CT, None)
+    out = G.DeviceBuffer((B, 2, L - 1, n))
+    stream = G.Stream()
+    for _ in range(args.warmup):
+        sc = ctx.bsgs_matvec(cts, B, L, 40, diag, keys, out, stream.ptr)
+    ms = time_it(G, lambda: ctx.bsgs_matvec(cts, B, L, 40, diag, keys, out, stream.ptr), args.steps, stream)
+    ms /= args.steps
+    one = G.DeviceBuffer((1, 2, L - 1, n))
+    ctx.bsgs_matvec(cts, 1, L, 40, diag, keys, one, stream.ptr)
+    ms1 = time_it(G, lambda: ctx.bsgs_matvec(cts, 1, L, 40, diag, keys, one, stream.ptr), 2, stream) / 2
+    P = 2 * (L - 1) * n * 8
+    errs = []
+    for v in range(min(B, 4)):
+        dec = ctx.decrypt(out.ptr + v * P, L - 1, s)
+        got = ctx.decode(dec, L - 1, sc).real
+        errs.append(float(np.max(np.abs(got[:dim] - matrix @ vecs[v]))))
+    G.profile_enable(True)
+    ctx.bsgs_matvec(cts, B, L, 40, diag, keys, out, stream.ptr)
+    prof = G.profile_report()
+    G.profile_enable(False)
+    value = B * 1000.0 / ms
+    log(f"[bench] bsgs: {ms:.2f} ms per batch of {B} = {value:.1f} matvecs/s; one vector alone {ms1:.2f} ms "
+        f"({ms1 * B / ms:.2f}x); decode error max {max(errs):.2e}")
+    cpu = None if args.no_cpu_baseline else cpu_baseline_bsgs(args.cpu_threads, dim)
+    line = {
+        "metric": "encrypted BSGS diagonal matvec throughput (matvecs/s), batched",
+        "value": round(value, 3), "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 residues (FP64-exact modular arithmetic)", "data": "synthetic (seeded U[-1,1] matrix and vectors)",
+        "config": {"workload": f"N=2^14, {dim}x{dim} matrix, baby/giant step {diag.baby_step}, 12 limbs (primes > 2^50,"
+                               f" K=2), {B} vectors per batch, {len(keys)} rotation keys",
+                   "one_vector_ms": round(ms1, 3), "batched_vs_one_by_one": round(ms1 * B / ms, 3),
+                   "decode_err_max": max(errs)},
+        "roofline": None, "cpu_baseline": cpu,
+        "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_bsgs(threads, dim):
+    """The oracle on a bounded sample of one matvec at N=2^14, 12 limbs: 2 rotations and bs plaintext
+    products (one giant step), extrapolated to the (bs - 1) + (gs - 1) rotations and dim products."""
+    from oracle import oracle as O
+    from rnsckks import params as PR
+    O.set_threads(threads)
+    n, L = 1 << 14, 12
+    p = PR.config_batched(n=n, limbs=L, aux=2)
+    rs = np.random.default_rng(5)
+    q = O.Basis(n, p.primes, p.roots)
+    e = O.Basis(n, p.primes + p.aux_primes, p.roots + p.aux_roots)
+    D = O.num_decomp_digits(q.total_modulus_bits())
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    ct = uniform(p.primes, n, rs, 2)
+    pt = uniform(p.primes, n, rs, 1)[0]
+    bs = 1
+    while bs * bs < dim:
+        bs += 1
+    gs = -(-dim // bs)
+    log(f"[bench] cpu baseline: 2 rotations + {bs} ct x pt products (N=2^14, 12 limbs) on {threads} threads")
+    t = time.perf_counter()
+    for _ in range(2):
+        O.rotate(q, e, ct, O.galois_element_for_rotation(n, 1), key)
+    t_rot = (time.perf_counter() - t) / 2
+    t = time.perf_counter()
+    for _ in range(bs):
+        O.multiply_plain(q, ct, pt)
+    t_pt = (time.perf_counter() - t) / bs
+    per = (bs - 1 + gs - 1) * t_rot + dim * t_pt
+    log(f"[bench] cpu baseline: {t_rot:.2f} s/rotation, {t_pt * 1e3:.0f} ms/product -> {per:.0f} s per matvec")
+    return {"value": round(1.0 / per, 6), "unit": "matvecs/s", "cores": threads, "kind": "port",
+            "sample": f"2 rotations + {bs} ct x pt products at N=2^14, 12 limbs (oracle, OpenMP), extrapolated to "
+                      f"{bs - 1 + gs - 1} rotations + {dim} products per matvec"}
 
 
 def cpu_baseline_ps(threads):

@@ -44,36 +44,55 @@ uint32_t bsgs_baby_step(uint32_t dim) {
     return b;
 }
 
-// inner[g][vz][l][x] = sum_b baby[b][vz][l][x] * diag[g][b][l][x]  (NTT domain, reduced doubles)
-// One thread per (x, vz) of one limb: its bs (<= BSMAX) baby values stay in registers while it walks
-// the giant steps; the diagonal values of a (g, b, l, x) are read by the vz threads of that x (the
-// same workgroup column) and hit in cache.  Each product is reduced (|.| <= 0.94q) and the running
-// sum every second term (|sum| <= 2.4q < 2^53 for q < 2^51: every addition stays exact).
-template <int BSMAX>
+// inner[g][vz][l][x] = sum_b diag[g][b][l][x] * baby[b][vz][l][x]  (NTT domain, reduced doubles):
+// per coefficient (l, x) a (giants x bs) by (bs x vz) product mod q_l.  Lanes run over x (coalesced
+// loads and stores); each thread owns a TG x TV register tile of (giant, vz) outputs, so every loaded
+// diagonal value feeds TV products and every baby value TG products.  Absent diagonals are stored as
+// zeros and contribute exact zeros.  Each product is reduced (|.| <= 0.94q) and the running sums
+// every second term (|sum| <= 2.4q < 2^53 for q < 2^51: every addition stays exact).
+template <int TG, int TV>
 __global__ __launch_bounds__(256) void k_diag_mac(const double* __restrict__ baby, const double* __restrict__ diag,
-                                                  const uint8_t* __restrict__ present, double* __restrict__ inner,
-                                                  int n, int limbs, int vz_count, int giants, int bs,
-                                                  const DevPrime* __restrict__ primes) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int vz = blockIdx.z * 4 + (threadIdx.x >> 6);
+                                                  double* __restrict__ inner, int n, int limbs, int vz_count,
+                                                  int giants, int bs, const DevPrime* __restrict__ primes) {
+    // tiles fastest: the workgroups sharing a coefficient block run back to back (cache reuse)
+    const int x = blockIdx.z * 256 + threadIdx.x;
     const int l = blockIdx.y;
-    if (x >= n || vz >= vz_count) return;
+    const int ngb = (giants + TG - 1) / TG;
+    const int g0 = (blockIdx.x % ngb) * TG, v0 = (blockIdx.x / ngb) * TV;
+    if (x >= n) return;
     const DevPrime P = primes[l];
     const long long lim = (long long)limbs * n;  // stride of one polynomial
-    double bb[BSMAX];
+    const long long off = (long long)l * n + x;
+    double acc[TG][TV];
 #pragma unroll
-    for (int b = 0; b < BSMAX; b++)
-        bb[b] = b < bs ? baby[((long long)b * vz_count + vz) * lim + (long long)l * n + x] : 0.0;
-    for (int g = 0; g < giants; g++) {
-        double acc = 0.0;
-        const double* dg = diag + (long long)g * bs * lim + (long long)l * n + x;
+    for (int t = 0; t < TG; t++)
 #pragma unroll
-        for (int b = 0; b < BSMAX; b++) {
-            if (b < bs && present[g * bs + b]) acc += mulmod(bb[b], dg[(long long)b * lim], P.q, P.qinv);
-            if (b & 1) acc = reduce(acc, P.q, P.qinv);
+        for (int u = 0; u < TV; u++) acc[t][u] = 0.0;
+    for (int b = 0; b < bs; b++) {
+        double d[TG], w[TV];
+#pragma unroll
+        for (int t = 0; t < TG; t++)
+            d[t] = g0 + t < giants ? diag[((long long)(g0 + t) * bs + b) * lim + off] : 0.0;
+#pragma unroll
+        for (int u = 0; u < TV; u++)
+            w[u] = v0 + u < vz_count ? baby[((long long)b * vz_count + v0 + u) * lim + off] : 0.0;
+#pragma unroll
+        for (int t = 0; t < TG; t++)
+#pragma unroll
+            for (int u = 0; u < TV; u++) acc[t][u] += mulmod(d[t], w[u], P.q, P.qinv);
+        if (b & 1) {
+#pragma unroll
+            for (int t = 0; t < TG; t++)
+#pragma unroll
+                for (int u = 0; u < TV; u++) acc[t][u] = reduce(acc[t][u], P.q, P.qinv);
         }
-        inner[((long long)g * vz_count + vz) * lim + (long long)l * n + x] = reduce(acc, P.q, P.qinv);
     }
+#pragma unroll
+    for (int t = 0; t < TG; t++)
+#pragma unroll
+        for (int u = 0; u < TV; u++)
+            if (g0 + t < giants && v0 + u < vz_count)
+                inner[((long long)(g0 + t) * vz_count + v0 + u) * lim + off] = reduce(acc[t][u], P.q, P.qinv);
 }
 
 template <typename F>
@@ -93,23 +112,19 @@ int bguard2(F&& f) {
     }
 }
 
-void launch_diag_mac(const Context& c, int bs, const double* baby, const double* diag, const uint8_t* present,
-                     double* inner, int limbs, int vz, int giants, hipStream_t s) {
-    const dim3 grid((c.n + 63) / 64, limbs, (vz + 3) / 4);
+void launch_diag_mac(const Context& c, int bs, const double* baby, const double* diag, double* inner, int limbs,
+                     int vz, int giants, hipStream_t s) {
     ProfScope prof(s, "diag_mac");
     RCK_REQUIRE(bs >= 1 && bs <= 64, "baby step must be at most 64 (matrix dimension at most 4096)");
-    if (bs <= 8)
-        hipLaunchKernelGGL(k_diag_mac<8>, grid, dim3(256), 0, s, baby, diag, present, inner, (int)c.n, limbs, vz, giants,
+    if (giants >= 8 && vz >= 8) {
+        const dim3 grid(((giants + 7) / 8) * ((vz + 7) / 8), limbs, (c.n + 255) / 256);
+        hipLaunchKernelGGL((k_diag_mac<8, 8>), grid, dim3(256), 0, s, baby, diag, inner, (int)c.n, limbs, vz, giants,
                            bs, c.d_primes);
-    else if (bs <= 16)
-        hipLaunchKernelGGL(k_diag_mac<16>, grid, dim3(256), 0, s, baby, diag, present, inner, (int)c.n, limbs, vz,
-                           giants, bs, c.d_primes);
-    else if (bs <= 32)
-        hipLaunchKernelGGL(k_diag_mac<32>, grid, dim3(256), 0, s, baby, diag, present, inner, (int)c.n, limbs, vz,
-                           giants, bs, c.d_primes);
-    else
-        hipLaunchKernelGGL(k_diag_mac<64>, grid, dim3(256), 0, s, baby, diag, present, inner, (int)c.n, limbs, vz,
-                           giants, bs, c.d_primes);
+    } else {
+        const dim3 grid(((giants + 3) / 4) * ((vz + 3) / 4), limbs, (c.n + 255) / 256);
+        hipLaunchKernelGGL((k_diag_mac<4, 4>), grid, dim3(256), 0, s, baby, diag, inner, (int)c.n, limbs, vz, giants,
+                           bs, c.d_primes);
+    }
     RCK_CHECK_HIP(hipGetLastError());
 }
 
@@ -292,7 +307,6 @@ int rck_bsgs_matvec(rck_context* ctx, const uint64_t* cts, uint32_t batch, uint3
         uint64_t* coef_p = static_cast<uint64_t*>(c.workspace(23, (size_t)gc * batch * CT * 8));
         uint64_t* acc_p = static_cast<uint64_t*>(c.workspace(24, (size_t)batch * CT * 8));
         uint64_t* rot_p = static_cast<uint64_t*>(c.workspace(25, (size_t)batch * CT * 8));
-        uint8_t* present_p = static_cast<uint8_t*>(c.workspace(26, d->present.size()));
         RCK_CHECK_HIP(hipMemcpyAsync(baby_p, cts, (size_t)batch * CT * 8, hipMemcpyDeviceToDevice, s));
         std::vector<const uint64_t*> ins;
         for (uint32_t v = 0; v < batch; v++) ins.push_back(cts + v * CT);
@@ -305,9 +319,8 @@ int rck_bsgs_matvec(rck_context* ctx, const uint64_t* cts, uint32_t batch, uint3
         launch_fwd(c, kInU64, baby_p, n, P, bntt_p, n, P, (int)limbs, (int)(nb * batch * 2), pm, s);
         // every inner sum of every giant step, then back to coefficient form: [g][v][z][l][N]
         const int vz = (int)batch * 2;
-        RCK_CHECK_HIP(hipMemcpyAsync(present_p, d->present.data(), d->present.size(), hipMemcpyHostToDevice, s));
         RCK_REQUIRE(nb == bs, "matrix dimension below the baby step is not supported");
-        launch_diag_mac(c, (int)bs, bntt_p, d->ntt, present_p, inner_p, (int)limbs, vz, (int)gc, s);
+        launch_diag_mac(c, (int)bs, bntt_p, d->ntt, inner_p, (int)limbs, vz, (int)gc, s);
         launch_inv(c, inner_p, n, P, kOutU64, coef_p, n, P, (int)limbs, (int)(gc * batch * 2), pm, s);
         // giant rotations and the sum (a giant step whose diagonals are all absent contributes nothing)
         bool have = false;

@@ -123,8 +123,12 @@ void run_op_group(Context& c, int kind, int l, const KeyDev* key, const std::vec
         case RCK_OP_ROTATE: {
             uint64_t* rot = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
             PrimeMap pm{l, (int)c.L, 0};
-            for (int i = 0; i < B; i++)
-                launch_galois(c, {as[i], n, P}, {rot + i * 2 * P, n, P}, l, 2, key->galois, pm, s);
+            if (B > 1 && common_stride(as) == 2 * P) {  // contiguous inputs: one launch for all B
+                launch_galois(c, {as[0], n, P}, {rot, n, P}, l, 2 * B, key->galois, pm, s);
+            } else {
+                for (int i = 0; i < B; i++)
+                    launch_galois(c, {as[i], n, P}, {rot + i * 2 * P, n, P}, l, 2, key->galois, pm, s);
+            }
             keyswitch_batch(c, rot + P, 2 * P, B, l, key, rot, nullptr, 2 * P, outs, s, shape);
             break;
         }
