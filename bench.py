@@ -72,8 +72,8 @@ def per_op_entry(table, n, limbs, E, D):
         v = line.split()
         if [int(x) for x in v[:5]] == [logn, limbs, E, D, 1]:
             return {"limbs": limbs, "batch": 1, "eval_limb_group": int(v[5]), "digit_splits": int(v[6]),
-                    "mac_rows": int(v[7]), "key_prefetch": int(v[8]), "keyswitch_ms": float(v[9]),
-                    "candidates": [], "from_table": True}
+                    "mac_rows": int(v[7]), "key_prefetch": int(v[8]), "mac_width": int(v[9]),
+                    "keyswitch_ms": float(v[10]), "candidates": [], "from_table": True}
     return None
 
 

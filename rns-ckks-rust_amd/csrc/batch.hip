@@ -94,7 +94,8 @@ static void check_group(const Context& c, const BatchGroup& g) {
 static std::string shape_json(const KsShape& sh, float ms) {
     std::ostringstream o;
     o << "{\"eval_limb_group\": " << sh.group << ", \"digit_splits\": " << sh.dsplit << ", \"mac_rows\": "
-      << sh.rows << ", \"load_schedule\": " << sh.prefetch << ", \"ms\": " << ms << "}";
+      << sh.rows << ", \"load_schedule\": " << sh.prefetch << ", \"mac_width\": " << sh.width << ", \"ms\": " << ms
+      << "}";
     return o.str();
 }
 
@@ -212,7 +213,8 @@ int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uin
         if (buf) {
             std::ostringstream o;
             o << "{\"limbs\": " << limbs << ", \"batch\": " << batch << ", \"eval_limb_group\": " << sh.group
-              << ", \"digit_splits\": " << sh.dsplit << ", \"mac_rows\": " << sh.rows << ", \"key_prefetch\": "
+              << ", \"digit_splits\": " << sh.dsplit << ", \"mac_rows\": " << sh.rows << ", \"mac_width\": "
+              << sh.width << ", \"key_prefetch\": "
               << sh.prefetch << ", \"keyswitch_ms\": " << ms << ", \"candidates\": [" << log << "]}";
             std::string r = o.str();
             RCK_REQUIRE(len > r.size(), "report buffer too small");

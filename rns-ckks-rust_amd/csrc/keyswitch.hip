@@ -226,14 +226,34 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
         fwd_phase_b<LOGR>(v, P.fw, N1 + b, tau, q, qinv);
+        if constexpr (PF == 3) {
+            // key rows in four groups of four positions: at most 8 key values in flight per thread
+            // (fewer live registers, more resident waves)
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            if constexpr (PF == 1) {
-                a0[e] += mulmod(v[e], w0[e], q, qinv);
-                a1[e] += mulmod(v[e], w1[e], q, qinv);
-            } else {
-                a0[e] += mulmod(v[e], k0[e * TPS], q, qinv);
-                a1[e] += mulmod(v[e], k1[e * TPS], q, qinv);
+            for (int g4 = 0; g4 < 4; g4++) {
+                double x0[4], x1[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    x0[e] = k0[(4 * g4 + e) * TPS];
+                    x1[e] = k1[(4 * g4 + e) * TPS];
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    a0[4 * g4 + e] += mulmod(v[4 * g4 + e], x0[e], q, qinv);
+                    a1[4 * g4 + e] += mulmod(v[4 * g4 + e], x1[e], q, qinv);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                if constexpr (PF == 1) {
+                    a0[e] += mulmod(v[e], w0[e], q, qinv);
+                    a1[e] += mulmod(v[e], w1[e], q, qinv);
+                } else {
+                    a0[e] += mulmod(v[e], k0[e * TPS], q, qinv);
+                    a1[e] += mulmod(v[e], k1[e * TPS], q, qinv);
+                }
             }
         }
         if (((i - i0) & 3) == 3) {
@@ -250,6 +270,136 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
         o0[e * TPS] = a0[e];
         o1[e * TPS] = a1[e];
     }
+}
+
+// Narrow variant: 8 elements per thread in the row pass (R/8 threads per ring row), so the
+// accumulators, the butterfly operands and the key values take half the registers of the 16-wide
+// kernel and more waves stay resident.  Stages of a 2^LOGR-point row transform:
+//   phase A: x = tau + (R/8) e            stages 0..2   (row bits LOGR-1 .. LOGR-3)
+//   phase B: x = hi(tau) + (e << m) + lo  stages 3..5   (bits LOGR-4 .. LOGR-6), m = LOGR - 6
+//   phase C: x = 8 tau + e                stages 6..    (bits m-1 .. 0)
+// with an LDS exchange between phases.  Twiddle of stage s for the pair starting at x:
+// tbl[(base << s) + (x >> (LOGR - s))], as in the 16-wide phases; outputs land in the same
+// NTT-domain positions (ntt_phys), so keys and accumulators are shared with the other kernels.
+template <int LOGR, int ROWS>
+__global__ __launch_bounds__(ROWS*(1 << LOGR) / 8) void k_digit_mac8(const double* __restrict__ cols, int D, int G,
+                                                                     int B, int dsplit, int ntiles,
+                                                                     const double* __restrict__ key, int E, int j0,
+                                                                     double* __restrict__ part, int logn, PrimeMap pm,
+                                                                     const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 8, T16 = R / 16, M = LOGR - 6;
+    static_assert(LOGR >= 6 && LOGR <= 8, "row length 64 .. 256");
+    __shared__ double lds[ROWS * (R + R / 16 + 1)];
+    const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / ROWS;
+    const int g = blockIdx.x, xcd = g & 7, slot = g >> 3;
+    const int cb = slot % B;
+    const int tile = (slot / B) * 8 + xcd;
+    if (tile >= ntiles) return;
+    const int rb = tile % nrb, rest = tile / nrb, jl = rest % G, sp = rest / G;
+    const int j = j0 + jl;
+    const int i0 = (int)((long long)D * sp / dsplit), i1 = (int)((long long)D * (sp + 1) / dsplit);
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int b = rb * ROWS + row;
+    const int base = N1 + b;
+    const DevPrime P = primes[pm(j)];
+    const double q = P.q, qinv = P.qinv;
+    const double* __restrict__ tbl = P.fw;
+    const long long off = (long long)b * R;
+    const long long EN = (long long)E * n;
+    const long long col_stride = (long long)G * n;
+    const double* pc = cols + (((long long)cb * D + i0) * G + jl) * n + off + tau;
+    // phase B positions: hi(tau) + (e << M) + lo(tau);  phase C / output positions: 8 tau + e
+    const int xb0 = (tau >> M) * (8 << M) + (tau & ((1 << M) - 1));
+    // output position of element e: x = 8 tau + e stored at (x % 16) * T16 + x / 16
+    const int ph0 = (8 * (tau & 1)) * T16 + (tau >> 1);
+    double a0[8], a1[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) a0[e] = a1[e] = 0.0;
+    for (int i = i0; i < i1; i++, pc += col_stride) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = pc[TPS * e];
+        // phase A: stages 0..2, x = tau + TPS e
+#pragma unroll
+        for (int st = 0; st < 3; st++) {
+            const int half = 4 >> st;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if ((e & half) == 0) ct_bfly(v[e], v[e + half], tbl[(base << st) + (e >> (3 - st))], q, qinv);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = reduce(v[e], q, qinv);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 8; e++) lds[row_lds<R>(row, tau + TPS * e)] = v[e];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = lds[row_lds<R>(row, xb0 + (e << M))];
+        // phase B: stages 3..5
+#pragma unroll
+        for (int st = 3; st < 6; st++) {
+            const int half = 4 >> (st - 3);
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if ((e & half) == 0)
+                    ct_bfly(v[e], v[e + half], tbl[(base << st) + ((xb0 + (e << M)) >> (LOGR - st))], q, qinv);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = reduce(v[e], q, qinv);
+        if constexpr (M > 0) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 8; e++) lds[row_lds<R>(row, xb0 + (e << M))] = v[e];
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = lds[row_lds<R>(row, 8 * tau + e)];
+            // phase C: stages 6 .. LOGR-1 on x = 8 tau + e
+#pragma unroll
+            for (int st = 6; st < LOGR; st++) {
+                const int half = (1 << LOGR) >> (st + 1);
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    if ((e & half) == 0)
+                        ct_bfly(v[e], v[e + half], tbl[(base << st) + ((8 * tau + e) >> (LOGR - st))], q, qinv);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = reduce(v[e], q, qinv);
+        }
+        const double* k0 = key + (2LL * i) * EN + (long long)j * n + off + ph0;
+        const double* k1 = k0 + EN;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            a0[e] += mulmod(v[e], k0[e * T16], q, qinv);
+            a1[e] += mulmod(v[e], k1[e * T16], q, qinv);
+        }
+        if (((i - i0) & 3) == 3) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                a0[e] = reduce(a0[e], q, qinv);
+                a1[e] = reduce(a1[e], q, qinv);
+            }
+        }
+    }
+    double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + off + ph0;
+    double* o1 = o0 + EN;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        o0[e * T16] = reduce(a0[e], q, qinv);
+        o1[e * T16] = reduce(a1[e], q, qinv);
+    }
+}
+
+template <int LOGR, int ROWS>
+static void mac8_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
+                   int j0, double* part, PrimeMap pm, hipStream_t s) {
+    constexpr int TPS = (1 << LOGR) / 8;
+    const int nrb = (1 << (c.logn - LOGR)) / ROWS;
+    const int ntiles = nrb * G * dsplit;
+    const int blocks = ((ntiles + 7) / 8) * 8 * B;
+    ProfScope ps(s, "digit_mac");
+    k_digit_mac8<LOGR, ROWS><<<dim3(blocks), ROWS * TPS, 0, s>>>(cols, D, G, B, dsplit, ntiles, key, E, j0, part,
+                                                                 c.logn, pm, c.d_primes);
+    RCK_CHECK_HIP(hipGetLastError());
 }
 
 template <int LOGR, int ROWS, int PF>
@@ -270,6 +420,8 @@ static void mac_r(const Context& c, int pf, const double* cols, int D, int G, in
                   int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
     if (pf == 1)
         mac_t<LOGR, ROWS, 1>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    else if (pf == 3)
+        mac_t<LOGR, ROWS, 3>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
     else if (pf == 2)
         mac_t<LOGR, ROWS, 2>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
     else
@@ -277,8 +429,15 @@ static void mac_r(const Context& c, int pf, const double* cols, int D, int G, in
 }
 
 template <int LOGR>
-static void mac_v(const Context& c, int rows, int pf, const double* cols, int D, int G, int B, int dsplit,
+static void mac_v(const Context& c, int rows, int pf, int width, const double* cols, int D, int G, int B, int dsplit,
                   const double* key, int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
+    if (width == 8) {
+        if (rows <= 8)
+            mac8_t<LOGR, 8>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+        else
+            mac8_t<LOGR, 16>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+        return;
+    }
     if (rows <= 4)
         mac_r<LOGR, 4>(c, pf, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
     else
@@ -309,7 +468,7 @@ __global__ void k_mac_small(const double* __restrict__ dn, int D, const double* 
 }
 
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
-                      double* part, int dsplit, int group, int rows, int prefetch, hipStream_t s) {
+                      double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s) {
     const int E = nq + naux;
     const long long n = c.n, EN = (long long)E * n;
     PrimeMap pm{nq, (int)c.L, 0};
@@ -351,9 +510,9 @@ void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, con
             RCK_CHECK_HIP(hipStreamWaitEvent(s, cc.event(3 + slot), 0));
         }
         switch (n2) {
-            case 6: mac_v<6>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
-            case 7: mac_v<7>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
-            case 8: mac_v<8>(c, rows, prefetch, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 6: mac_v<6>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 7: mac_v<7>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 8: mac_v<8>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
             default: RCK_REQUIRE(false, "unsupported ring degree");
         }
         if (ngroups > 1) RCK_CHECK_HIP(hipEventRecord(cc.event(1 + slot), s));

@@ -38,10 +38,11 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 // prime j (Q-prefix nq, then naux aux primes), NTT form.  digits: [B][D][N]; key: [>=D][2][nq+naux][N]
 // NTT-form doubles; part: [dsplit][B][2][nq+naux][N] partial sums over digit ranges.  Eval limbs
 // are processed `group` at a time (column-pass scratch of B*D*group limbs).
-// rows: ring rows per MAC workgroup (4 or 16); prefetch: load schedule (0 late keys, 1 early keys,
-// 2 next digit's rows prefetched).
+// rows: ring rows per MAC workgroup (4 or 16; 8 or 16 with width 8); prefetch: load schedule (0 late
+// keys, 1 early keys, 2 next digit's rows prefetched, 3 keys in groups of four positions).
+// width: elements per thread in the MAC's row pass (16, or 8: half the registers, more resident waves).
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
-                      double* part, int dsplit, int group, int rows, int prefetch, hipStream_t s);
+                      double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s);
 // mod_down_eval_to_q (keyswitching.rs:278-322) on coefficient-form accumulators over Q-prefix
 // nq || P (2 polys, u64), optionally adding add0 / add1 (nq limbs each) to the two results.
 void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64_t* add0, const uint64_t* add1,

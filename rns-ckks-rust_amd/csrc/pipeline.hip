@@ -169,9 +169,9 @@ std::string ks_tuned_export() {
     char line[160];
     for (const auto& kv : g_tuned) {
         const KsShape& sh = kv.second.first;
-        std::snprintf(line, sizeof line, "%d %d %d %d %d %d %d %d %d %.6f\n", std::get<0>(kv.first),
+        std::snprintf(line, sizeof line, "%d %d %d %d %d %d %d %d %d %d %.6f\n", std::get<0>(kv.first),
                       std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first),
-                      sh.group, sh.dsplit, sh.rows, sh.prefetch, (double)kv.second.second);
+                      sh.group, sh.dsplit, sh.rows, sh.prefetch, sh.width, (double)kv.second.second);
         out += line;
     }
     return out;
@@ -187,16 +187,19 @@ int ks_tuned_import(const std::string& text) {
         const std::string line = text.substr(pos, end - pos);
         pos = end + 1;
         if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
-        int v[9];
+        int v[10];
         double ms = 0;
-        const int got = std::sscanf(line.c_str(), "%d %d %d %d %d %d %d %d %d %lf", &v[0], &v[1], &v[2], &v[3],
-                                    &v[4], &v[5], &v[6], &v[7], &v[8], &ms);
-        RCK_REQUIRE(got == 10, "malformed tuning table line: " + line);
+        char extra = 0;
+        const int got = std::sscanf(line.c_str(), "%d %d %d %d %d %d %d %d %d %d %lf %c", &v[0], &v[1], &v[2],
+                                    &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &ms, &extra);
+        RCK_REQUIRE(got == 11, "malformed tuning table line: " + line);
         RCK_REQUIRE(v[1] >= 1 && v[2] >= v[1] && v[3] >= 1 && v[4] >= 1, "tuning table entry out of range: " + line);
-        RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (v[7] == 4 || v[7] == 16) &&
-                        v[8] >= 0 && v[8] <= 2,
+        const bool w16 = v[9] == 16 && (v[7] == 4 || v[7] == 16) && v[8] >= 0 && v[8] <= 3;
+        const bool w8 = v[9] == 8 && (v[7] == 8 || v[7] == 16) && v[8] == 0;
+        RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (w16 || w8),
                     "tuning table launch shape out of range: " + line);
-        rows.push_back({std::make_tuple(v[0], v[1], v[2], v[3], v[4]), {KsShape{v[5], v[6], v[7], v[8]}, (float)ms}});
+        rows.push_back(
+            {std::make_tuple(v[0], v[1], v[2], v[3], v[4]), {KsShape{v[5], v[6], v[7], v[8], v[9]}, (float)ms}});
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const auto& r : rows) g_tuned[r.first] = r.second;
@@ -236,10 +239,13 @@ std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
     std::sort(gs.begin(), gs.end());
     gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
     for (int g : gs)
-        for (int ds : {1, 2, 4})
+        for (int ds : {1, 2, 4}) {
+            if (ds > D) continue;
             for (int rows : {4, 16})
-                for (int pf : {0, 1, 2})
-                    if (ds <= D && g >= 1 && g <= E) out.push_back({g, ds, rows, pf});
+                for (int pf : {0, 1, 2, 3}) out.push_back({g, ds, rows, pf, 16});
+            if (g == E && ds <= 2)
+                for (int rows : {8, 16}) out.push_back({g, ds, rows, 0, 8});
+        }
     (void)B;
     return out;
 }
@@ -278,7 +284,9 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
     double* part = static_cast<double*>(c.workspace(6, (size_t)dsplit * B * 2 * EN * 8));
     launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps);
-    launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, rows, prefetch, s);
+    const int width = shape.group > 0 || shape.rows > 0 ? shape.width : aut.width;
+    launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, rows, prefetch,
+                     rows == 4 ? 16 : width, s);
     PrimeMap pm{limbs, (int)c.L, 0};
     const long long sstride = (long long)B * 2 * EN;
     if (naux && c.logn >= 12) {

@@ -17,7 +17,8 @@ struct KsShape {
     int group = 0;     // eval limbs per column-pass/MAC launch
     int dsplit = 0;    // digit ranges accumulated by separate workgroups
     int rows = 0;      // ring rows per MAC workgroup (4 or 16)
-    int prefetch = -1; // load schedule: 0 late keys, 1 early keys, 2 next digit's rows prefetched
+    int prefetch = -1; // load schedule: 0 late keys, 1 early keys, 2 next digit's rows prefetched, 3 keys in groups
+    int width = 16;    // elements per thread in the row pass (16, or 8 with rows 8 / 16)
 };
 
 inline void check_ptr(const void* p, const char* what) {
@@ -50,7 +51,7 @@ KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B);
 // tuned-shape table shared by the per-op calls and every batch of the process
 void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape s, float ms);
 bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* s, float* ms);
-// The table as text, one entry per line: "logn limbs E D B group dsplit rows prefetch ms"; import
+// The table as text, one entry per line: "logn limbs E D B group dsplit rows prefetch width ms"; import
 // validates every line before merging (returns the number of entries merged).
 std::string ks_tuned_export();
 int ks_tuned_import(const std::string& text);

@@ -63,12 +63,13 @@ def test_tune_table_round_trip_and_validation():
     malformed or out-of-range launch shapes without changing the table."""
     G = capi
     before = G.tune_table_export()
-    rows = "9 5 7 9 3 7 2 16 1 1.250000\n9 5 7 9 2 4 1 4 0 0.500000\n"
-    assert G.tune_table_import(rows) == 2
+    rows = "9 5 7 9 3 7 2 16 1 16 1.250000\n9 5 7 9 2 4 1 4 0 16 0.500000\n9 5 7 9 4 7 1 8 0 8 0.750000\n"
+    assert G.tune_table_import(rows) == 3
     table = G.tune_table_export()
     for line in rows.splitlines():
         assert line in table.splitlines()
-    for bad in ("9 5 7 9 3 7 2 8 1 1.0\n", "9 5 7 9 3 8 2 16 1 1.0\n", "9 5 7 9\n"):
+    for bad in ("9 5 7 9 3 7 2 8 1 16 1.0\n", "9 5 7 9 3 8 2 16 1 16 1.0\n", "9 5 7 9\n",
+                "9 5 7 9 3 7 2 4 0 8 1.0\n", "9 5 7 9 3 7 2 16 1 1.0\n"):
         with pytest.raises(G.RckError):
             G.tune_table_import(bad)
         assert G.tune_table_export() == table

@@ -1,0 +1,46 @@
+"""Every digit-MAC launch shape the autotuner may pick gives the same keyswitch bit for bit:
+row-pass width 16 (rows 4 / 16, every load schedule) and width 8 (rows 8 / 16), eval-limb groups
+and digit splits, for every row length (ring degrees 2^12 .. 2^16).  The default shape is pinned
+to the oracle by the parity tests; here each shape is forced through the tuning table."""
+import numpy as np
+import pytest
+
+from rnsckks import capi as G
+from rnsckks import params as PR
+
+pytestmark = pytest.mark.gpu
+
+
+def uniform(moduli, n, rs, polys):
+    out = np.empty((polys, len(moduli), n), dtype=np.uint64)
+    for l, q in enumerate(moduli):
+        out[:, l] = rs.integers(0, q, size=(polys, n), dtype=np.uint64)
+    return out
+
+
+SHAPES = [  # group, dsplit, rows, prefetch, width
+    (None, 1, 16, 0, 16), (None, 2, 4, 1, 16), (None, 1, 4, 2, 16), (None, 1, 16, 3, 16), (None, 1, 8, 0, 8),
+    (None, 2, 16, 0, 8), (4, 1, 8, 0, 8), (1, 1, 16, 3, 16),
+]
+
+
+@pytest.mark.parametrize("logn,limbs", [(12, 4), (13, 6), (14, 5), (15, 4), (16, 8)])
+def test_mac_shapes_bit_identical(logn, limbs):
+    n = 1 << logn
+    p = PR.config_batched(n=1 << 16, limbs=limbs, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    rs = np.random.default_rng(logn)
+    D = ctx.num_decomp_digits(limbs)
+    E = limbs + 2
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    k = ctx.ksk_create(key, limbs - 1, 2)
+    quad = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 3))
+    out = G.DeviceBuffer((2, limbs, n))
+    ctx.relinearize(quad, limbs, 80, k, out)
+    want = out.numpy()
+    for grp, ds, rows, pf, width in SHAPES:
+        g = E if grp is None else min(grp, E)
+        G.tune_table_import(f"{logn} {limbs} {E} {D} 1 {g} {ds} {rows} {pf} {width} 1.0\n")
+        out.upload(np.zeros_like(want))
+        ctx.relinearize(quad, limbs, 80, k, out)
+        assert np.array_equal(out.numpy(), want), (grp, ds, rows, pf, width)
