@@ -22,35 +22,69 @@ namespace rck {
 // ---------------------------------------------------------------------------------------
 constexpr int kDecBS = 64;
 
-template <int W>
+// NQM (>= nq, a multiple of 8): bound of the limb count; the Garner digits m_0 .. m_{nq-1} live in
+// registers (loops fully unrolled over NQM with limb guards), so the workgroup's LDS holds only the
+// parked words of X and more waves stay resident.  NQM = 0: the digits live in LDS (any nq).
+template <int W, int NQM>
 __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict__ poly, long long ps, int n, int nq,
                                                       int levels,
                                                       const DevPrime* __restrict__ primes,
                                                       const double* __restrict__ invq, int T,
                                                       const uint32_t* __restrict__ qw, const uint32_t* __restrict__ hw,
                                                       int32_t* __restrict__ digits) {
-    extern __shared__ double m[];  // [nq][kDecBS] doubles, then [W + 1][kDecBS] words
+    // LDS: [W + 1][kDecBS] words of X, preceded (NQM == 0) by [nq][kDecBS] Garner digits
+    extern __shared__ double lds_m[];
+    uint32_t* xs = reinterpret_cast<uint32_t*>(lds_m + (NQM == 0 ? nq * kDecBS : 0));
     const int tid = threadIdx.x;
     const long long c = (long long)blockIdx.x * kDecBS + tid;
     if (c >= n) return;
     poly += (long long)blockIdx.y * ps;
     digits += (long long)blockIdx.y * levels * n;
     // Garner (row order as in the reference: m_i from r_i and m_0 .. m_{i-1})
-    for (int i = 0; i < nq; i++) {
-        const double q = primes[i].q, qinv = primes[i].qinv;
-        double v = (double)poly[(long long)i * n + c];
-        for (int j = 0; j < i; j++) {
-            double a = reduce(v - m[j * kDecBS + tid], q, qinv);
-            v = mulmod(a, invq[(long long)i * T + j], q, qinv);
+    double m[NQM > 0 ? NQM : 1];
+    if constexpr (NQM > 0) {
+#pragma unroll
+        for (int i = 0; i < NQM; i++) {
+            m[i] = 0.0;
+            if (i < nq) {
+                const double q = primes[i].q, qinv = primes[i].qinv;
+                double v = (double)poly[(long long)i * n + c];
+#pragma unroll
+                for (int j = 0; j < i; j++) {
+                    double a = reduce(v - m[j], q, qinv);
+                    v = mulmod(a, invq[(long long)i * T + j], q, qinv);
+                }
+                m[i] = i == 0 ? v : canon(v, q, qinv);
+            }
         }
-        m[i * kDecBS + tid] = i == 0 ? v : canon(v, q, qinv);
+    } else {
+        for (int i = 0; i < nq; i++) {
+            const double q = primes[i].q, qinv = primes[i].qinv;
+            double v = (double)poly[(long long)i * n + c];
+            for (int j = 0; j < i; j++) {
+                double a = reduce(v - lds_m[j * kDecBS + tid], q, qinv);
+                v = mulmod(a, invq[(long long)i * T + j], q, qinv);
+            }
+            lds_m[i * kDecBS + tid] = i == 0 ? v : canon(v, q, qinv);
+        }
     }
+    auto mdig = [&](int i) -> double {  // Garner digit i (i < nq)
+        if constexpr (NQM > 0) {
+            double r = 0.0;
+#pragma unroll
+            for (int k = 0; k < NQM; k++)
+                if (k == i) r = m[k];
+            return r;
+        } else {
+            return lds_m[i * kDecBS + tid];
+        }
+    };
     // Horner: X = m_{nq-1}; X = X * q_i + m_i
     uint32_t x[W];
 #pragma unroll
     for (int k = 0; k < W; k++) x[k] = 0;
     {
-        uint64_t top = (uint64_t)m[(nq - 1) * kDecBS + tid];
+        uint64_t top = (uint64_t)mdig(nq - 1);
         x[0] = (uint32_t)top;
         x[1] = (uint32_t)(top >> 32);
     }
@@ -62,7 +96,7 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
         // One pass, two carry chains: t = x_k qlo + c0 (< 2^64 - 2^32), u = lo32(t) + x_{k-1} qhi + c1
         // (< 2^53); the new word is lo32(u).
         const int len = min(W, (51 * (nq - i) + 31) / 32 + 1);
-        uint64_t c0 = 0, c1 = (uint64_t)m[i * kDecBS + tid];
+        uint64_t c0 = 0, c1 = (uint64_t)mdig(i);
         uint32_t prev = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {
@@ -97,7 +131,6 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     }
     // balanced digits of X (two's complement, sign-extended over W words).  X is parked in this
     // thread's LDS column so the digit loop can index its words without a private stack array.
-    uint32_t* xs = reinterpret_cast<uint32_t*>(m + nq * kDecBS);  // [W + 1][kDecBS]
 #pragma unroll
     for (int k = 0; k < W; k++) xs[k * kDecBS + tid] = x[k];
     xs[W * kDecBS + tid] = (x[W - 1] >> 31) ? 0xffffffffu : 0u;
@@ -118,14 +151,30 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     }
 }
 
+template <int W, int NQM>
+static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
+                   long long ps) {
+    const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords;
+    ProfScope prof(s, "decompose");
+    const size_t lds = (size_t)(W + 1) * kDecBS * 4 + (NQM == 0 ? (size_t)nq * kDecBS * 8 : 0);
+    k_decompose<W, NQM><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, lds, s>>>(
+        poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords, digits);
+    RCK_CHECK_HIP(hipGetLastError());
+}
+
+// Register-resident Garner digits up to 24 limbs (beyond that the unrolled digit array would spill);
+// larger limb counts keep the digits in LDS (NQM = 0).
 template <int W>
 static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
                   long long ps) {
-    const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords;
-    ProfScope prof(s, "decompose");
-    k_decompose<W><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, (size_t)nq * kDecBS * 8 + (W + 1) * kDecBS * 4, s>>>(poly, ps, c.n, nq, levels, c.d_primes, c.d_invq,
-                                                                        c.L + c.K, qw, qw + kWords, digits);
-    RCK_CHECK_HIP(hipGetLastError());
+    if (nq <= 8)
+        dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps);
+    else if (nq <= 16)
+        dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps);
+    else if (nq <= 24)
+        dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps);
+    else
+        dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps);
 }
 
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
