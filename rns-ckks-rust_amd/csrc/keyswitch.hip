@@ -657,14 +657,11 @@ struct OutPtrs {
     uint64_t* p[32];
 };
 
-// centred(x_P) mod q_i for the aux residues r[0..K) of one coefficient
-__device__ __forceinline__ double centred_p_mod_q(const double* r, int K, int L, int T, int i, double q, double qinv,
-                                                  const DevPrime* __restrict__ primes,
-                                                  const double* __restrict__ invq,
-                                                  const uint32_t* __restrict__ pwords,
-                                                  const double* __restrict__ pmodq,
-                                                  const double* __restrict__ pmod_aux) {
-    double m[kMaxP];
+// Garner digits m_0 .. m_{K-1} of x_P from its aux residues r[0..K) and the centring flag
+// x_P > floor(P/2) (mixed_radix_digits / mod_q_biguint_to_centered, keyswitching.rs:397-410, 227-235)
+__device__ __forceinline__ bool p_garner(const double* r, int K, int L, int T, double* m,
+                                         const DevPrime* __restrict__ primes, const double* __restrict__ invq,
+                                         const uint32_t* __restrict__ pwords) {
     for (int k = 0; k < K; k++) {
         const DevPrime Pk = primes[L + k];
         double v = r[k];
@@ -719,15 +716,38 @@ __device__ __forceinline__ double centred_p_mod_q(const double* r, int K, int L,
             }
         }
     }
+    return gt;
+}
+
+// centred(x_P) mod q_i from the Garner digits and the centring flag
+__device__ __forceinline__ double p_centred_mod_q(const double* m, bool gt, int K, int L, int i, double q,
+                                                  double qinv, const double* __restrict__ pmodq,
+                                                  const double* __restrict__ pmod_aux) {
     double cv = reduce(m[K - 1], q, qinv);
     for (int k = K - 2; k >= 0; k--) cv = reduce(mulmod(cv, pmod_aux[(long long)k * L + i], q, qinv) + m[k], q, qinv);
     if (gt) cv -= pmodq[i];
     return cv;
 }
 
+// Once per coefficient of the aux part: pg[z][k][c] = Garner digit m_k (k < K), pg[z][K][c] = 1 when
+// x_P > floor(P/2) else 0.  accP: [Z][K][N] canonical residues.
+__global__ void k_p_garner(const uint64_t* __restrict__ accP, double* __restrict__ pg, int n, int K, int L, int T,
+                           const DevPrime* __restrict__ primes, const double* __restrict__ invq,
+                           const uint32_t* __restrict__ pwords) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int z = blockIdx.y;
+    double r[kMaxP], m[kMaxP];
+    for (int k = 0; k < K; k++) r[k] = (double)accP[((long long)z * K + k) * n + c];
+    const bool gt = p_garner(r, K, L, T, m, primes, invq, pwords);
+    double* o = pg + (long long)z * (K + 1) * n + c;
+    for (int k = 0; k < K; k++) o[(long long)k * n] = m[k];
+    o[(long long)K * n] = gt ? 1.0 : 0.0;
+}
+
 template <int LOGR>
-__global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ acc, long long EN,
-                                                     const uint64_t* __restrict__ accP, int K, int L, int T,
+__global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ acc, long long EN, int nq,
+                                                     const double* __restrict__ pg, int K, int L, int T,
                                                      const uint64_t* __restrict__ add0, const uint64_t* __restrict__ add1,
                                                      long long add_ps, OutPtrs outs, long long P, int logn,
                                                      const DevPrime* __restrict__ primes,
@@ -738,10 +758,14 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
                                                      const double* __restrict__ pmod_aux) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[R * S];
-    const int n = 1 << logn, N2 = n >> LOGR;
+    const int n = 1 << logn, N2 = n >> LOGR, ncb = N2 / S;
+    // XCD-aware order: the nq limb workgroups of one column block run back to back on one XCD
+    // (workgroup w -> XCD w % 8), so that XCD's L2 serves the block's aux-limb values to all of them
+    const int w = blockIdx.x, k = w >> 3, i = k % nq, cbk = (k / nq) * 8 + (w & 7);
+    if (cbk >= ncb) return;
     const int tid = threadIdx.x, col = tid % S, tau = tid / S;
-    const int r = blockIdx.x * S + col;
-    const int i = blockIdx.y, z = blockIdx.z, cb = z >> 1, comp = z & 1;
+    const int r = cbk * S + col;
+    const int z = blockIdx.z, cb = z >> 1, comp = z & 1;
     const DevPrime Pi = primes[i];
     const double q = Pi.q, qinv = Pi.qinv;
     const double* src = acc + (long long)z * EN + (long long)i * n;
@@ -755,7 +779,7 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
     inv_phase_a(v, Pi.iv, 1, q, qinv);
-    const uint64_t* pp = accP + (long long)z * K * n;
+    const double* pp = pg + (long long)z * (K + 1) * n;
     const uint64_t* add = comp == 0 ? add0 : add1;
     uint64_t* o = outs.p[cb] + (long long)comp * P + (long long)i * n;
     const double pinv = pinvq[i];
@@ -763,9 +787,10 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
     for (int e = 0; e < 16; e++) {
         const long long idx = r + (long long)N2 * (tau + TPS * e);
         const double x = mulmod(v[e], Pi.ninv, q, qinv);  // x_i, |x| <= 0.94 q
-        double rp[kMaxP];
-        for (int k = 0; k < K; k++) rp[k] = (double)pp[(long long)k * n + idx];
-        const double cv = centred_p_mod_q(rp, K, L, T, i, q, qinv, primes, invq, pwords, pmodq, pmod_aux);
+        double mp[kMaxP];
+        for (int k = 0; k < K; k++) mp[k] = pp[(long long)k * n + idx];
+        const bool gt = pp[(long long)K * n + idx] != 0.0;
+        const double cv = p_centred_mod_q(mp, gt, K, L, i, q, qinv, pmodq, pmod_aux);
         double res = mulmod(reduce(x - cv, q, qinv), pinv, q, qinv);
         if (add) res += (double)add[(long long)cb * add_ps + (long long)i * n + idx];
         o[idx] = f64_to_u64(canon(res, q, qinv));
@@ -779,19 +804,29 @@ void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B,
     const long long n = c.n, EN = (long long)(nq + c.K) * n, P = (long long)nq * n;
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
+    // the aux part's Garner digits and centring flag, once per coefficient
+    Context& cc = const_cast<Context&>(c);
+    double* pg = static_cast<double*>(cc.workspace(9, (size_t)B * 2 * (c.K + 1) * n * 8));
+    {
+        ProfScope ps(s, "p_garner");
+        k_p_garner<<<dim3((c.n + 255) / 256, 2 * B), 256, 0, s>>>(accP, pg, c.n, c.K, c.L, c.L + c.K, c.d_primes,
+                                                                 c.d_invq, c.d_pwords);
+        RCK_CHECK_HIP(hipGetLastError());
+    }
     for (int b0 = 0; b0 < B; b0 += 32) {
         const int bc = std::min(32, B - b0);
         OutPtrs op{};
         for (int b = 0; b < bc; b++) op.p[b] = outs[b0 + b];
         const double* accb = acc + (long long)b0 * 2 * EN;
-        const uint64_t* accPb = accP + (long long)b0 * 2 * c.K * n;
+        const double* accPb = pg + (long long)b0 * 2 * (c.K + 1) * n;
         const uint64_t* a0 = add0 ? add0 + b0 * add_ps : nullptr;
         const uint64_t* a1 = add1 ? add1 + b0 * add_ps : nullptr;
         ProfScope ps(s, "inv_cols_moddown");
         const int S = 256 / ((1 << n1) / 16);
-        dim3 grid((1 << n2) / S, nq, 2 * bc);
+        const int ncb = (1 << n2) / S;
+        dim3 grid(((ncb + 7) / 8) * 8 * nq, 1, 2 * bc);
 #define RCK_MD(LR)                                                                                                \
-    k_inv_cols_md<LR><<<grid, 256, 0, s>>>(accb, EN, accPb, c.K, c.L, c.L + c.K, a0, a1, add_ps, op, P, c.logn,   \
+    k_inv_cols_md<LR><<<grid, 256, 0, s>>>(accb, EN, nq, accPb, c.K, c.L, c.L + c.K, a0, a1, add_ps, op, P, c.logn, \
                                            c.d_primes, c.d_invq, c.d_pwords, c.d_pmodq, c.d_pinvq, c.d_pmod_aux)
         switch (n1) {
             case 6: RCK_MD(6); break;
