@@ -274,6 +274,19 @@ int rck_server_drain(rck_server* server, double timeout_s);
 int rck_server_stats(rck_server* server, char* buf, size_t len);
 int rck_server_destroy(rck_server* server);
 
+/* ---- versioned binary format (no counterpart in the crate, which has no serialization) ----
+ * "RCK1" header (kind, ring degree, counts, limbs, scale or key level, galois element, a fingerprint
+ * of the prime chain, payload size) followed by canonical u64 residues, limb-major.  Buffers are HOST
+ * memory; polys are DEVICE memory.  Passing buf = NULL returns the needed size.  Deserialization
+ * checks the magic, version, kind, ring degree, limb counts, prime-chain fingerprint and sizes. */
+int rck_serialize_polys(rck_context* ctx, const uint64_t* polys, uint32_t count, uint32_t nq, uint32_t np,
+                        uint32_t scale_bits, uint8_t* buf, size_t len, size_t* needed);
+int rck_deserialize_polys(rck_context* ctx, const uint8_t* buf, size_t len, uint64_t* polys, size_t capacity,
+                          uint32_t* count, uint32_t* nq, uint32_t* np, uint32_t* scale_bits);
+/* Keys travel in the crate's coefficient form (rck_ksk_export / rck_ksk_create). */
+int rck_ksk_serialize(rck_context* ctx, const rck_ksk* key, uint8_t* buf, size_t len, size_t* needed);
+int rck_ksk_deserialize(rck_context* ctx, const uint8_t* buf, size_t len, rck_ksk** out);
+
 /* ---- batched executor (no single counterpart in the crate: it runs many of the calls above) ----
  * Independent operations at any levels, of mixed kinds, are planned into grouped launches
  * (same kind, level and key -> one group whose kernels process all its ciphertexts together and

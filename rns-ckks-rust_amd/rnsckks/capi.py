@@ -96,6 +96,11 @@ EXPORTS = {
     "rck_diag_export": (ctypes.c_int, [vp, vp]),
     "rck_diag_destroy": (ctypes.c_int, [vp]),
     "rck_bsgs_matvec": (ctypes.c_int, [vp, vp, u32, u32, u32, vp, vp, u32, vp, u32p, vp]),
+    "rck_serialize_polys": (ctypes.c_int, [vp, vp, u32, u32, u32, u32, vp, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_size_t)]),
+    "rck_deserialize_polys": (ctypes.c_int, [vp, vp, ctypes.c_size_t, vp, ctypes.c_size_t, u32p, u32p, u32p, u32p]),
+    "rck_ksk_serialize": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "rck_ksk_deserialize": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
     "rck_batch_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "rck_batch_add": (ctypes.c_int, [vp, ctypes.c_int, u32, vp, vp, vp, vp]),
     "rck_batch_compile": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -541,6 +546,36 @@ class Context:
         _check(lib().rck_bsgs_matvec(self.ptr, _p(cts), batch, limbs, scale_bits, diag.ptr, arr, nk, _p(out),
                                      ctypes.byref(sc), stream))
         return sc.value
+
+    def serialize_polys(self, polys, count, nq, np_=0, scale_bits=0):
+        need = ctypes.c_size_t()
+        _check(lib().rck_serialize_polys(self.ptr, _p(polys), count, nq, np_, scale_bits, None, 0, ctypes.byref(need)))
+        buf = (ctypes.c_uint8 * need.value)()
+        _check(lib().rck_serialize_polys(self.ptr, _p(polys), count, nq, np_, scale_bits, buf, need.value,
+                                         ctypes.byref(need)))
+        return bytes(buf)
+
+    def deserialize_polys(self, data):
+        raw = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        v = [ctypes.c_uint32() for _ in range(4)]
+        _check(lib().rck_deserialize_polys(self.ptr, raw, len(data), None, 0, *[ctypes.byref(x) for x in v]))
+        count, nq, np_, scale = [x.value for x in v]
+        out = DeviceBuffer((count, nq + np_, self.n))
+        _check(lib().rck_deserialize_polys(self.ptr, raw, len(data), out.ptr, out.nbytes, *[ctypes.byref(x) for x in v]))
+        return out, {"count": count, "nq": nq, "np": np_, "scale_bits": scale}
+
+    def ksk_serialize(self, ksk):
+        need = ctypes.c_size_t()
+        _check(lib().rck_ksk_serialize(self.ptr, ksk.ptr, None, 0, ctypes.byref(need)))
+        buf = (ctypes.c_uint8 * need.value)()
+        _check(lib().rck_ksk_serialize(self.ptr, ksk.ptr, buf, need.value, ctypes.byref(need)))
+        return bytes(buf)
+
+    def ksk_deserialize(self, data):
+        raw = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        out = vp()
+        _check(lib().rck_ksk_deserialize(self.ptr, raw, len(data), ctypes.byref(out)))
+        return self._key_info(out.value)
 
     def ksk_export(self, ksk):
         out = np.zeros((ksk.digits, 2, ksk.level + 1 + ksk.aux, self.n), dtype=np.uint64)
