@@ -200,6 +200,116 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 }
 
 // ---------------------------------------------------------------------------------------
+// Digit column pass: the first log2(N1) stages of the forward NTT of digit z = (b, i) under the
+// eval limbs jl = y * jper .. min(g, (y + 1) * jper) - 1 of a group (prime pm(jl)), written to
+// out[z][jl][N].  A digit is prime-independent, so a thread loads its 16 int32 values once and
+// transforms them under every prime of its range; the stores of one prime drain while the next
+// prime's stages run.  Same thread mapping and LDS exchange as k_fwd_cols.
+// ---------------------------------------------------------------------------------------
+// Phase-B twiddles of the column pass for one prime (stages 4..LOGR-1 need 16 >> (LOGR - s)
+// entries of stage s, starting at (1 << s) + (16 tau >> (LOGR - s))).  They are loaded into
+// registers before the previous prime's stores are issued, so waiting for them never waits for
+// those stores (phase A's entries are lane-uniform and come through the scalar cache).
+
+template <int LOGR>
+struct ColTw {
+    static constexpr int count() {
+        int c = 0;
+        for (int s = 4; s < LOGR; s++) c += 16 >> (LOGR - s);
+        return c;
+    }
+    double w[count() > 0 ? count() : 1];
+    __device__ __forceinline__ void load(const double* fw, int tau) {
+        int k = 0;
+#pragma unroll
+        for (int s = 4; s < LOGR; s++) {
+#pragma unroll
+            for (int i = 0; i < (16 >> (LOGR - s)); i++) w[k++] = tw_load<kTwGlobal>(fw, (1 << s) + ((16 * tau) >> (LOGR - s)) + i);
+        }
+    }
+};
+
+template <int LOGR>
+__device__ __forceinline__ void col_phase_b(double (&v)[16], const ColTw<LOGR>& t, double q, double qinv) {
+    int k = 0;
+#pragma unroll
+    for (int s = 4; s < LOGR; s++) {
+        const int half = (1 << LOGR) >> (s + 1);
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if ((e & half) == 0) ct_bfly(v[e], v[e + half], t.w[k + (e >> (LOGR - s))], q, qinv);
+        }
+        k += 16 >> (LOGR - s);
+    }
+    reduce_all(v, q, qinv);
+}
+
+// Workgroup barrier ordering LDS only.  __syncthreads() also waits for every outstanding global
+// store of the wave (a release fence), which would serialise one prime's stores with the next
+// prime's transform.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_digit_cols(const int32_t* __restrict__ digits, double* __restrict__ out,
+                                                    int g, int jper, int logn, PrimeMap pm,
+                                                    const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[R * S];
+    const int n = 1 << logn, N2 = n >> LOGR;
+    const int tid = threadIdx.x, col = tid % S, tau = tid / S;
+    const int r = blockIdx.x * S + col;
+    const long long z = blockIdx.z;
+    const int32_t* src = digits + z * n + r;
+    int32_t dv[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) dv[e] = src[(long long)N2 * (tau + TPS * e)];
+    const int jb = blockIdx.y * jper, je = min(g, jb + jper);
+    double* dst = out + (z * g + jb) * n;  // wave-uniform base
+    const int lane = r + N2 * 16 * tau;      // 32-bit lane offset of row 16 tau
+    ColTw<LOGR> tw;
+    tw.load(primes[pm(jb)].fw, tau);
+#pragma unroll 1
+    for (int jl = jb; jl < je; jl++, dst += n) {
+        const DevPrime P = primes[pm(jl)];
+        const double q = P.q, qinv = P.qinv;
+        double v[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            // keep the digits as 16 int registers across the prime loop (converted per prime)
+            int32_t t = dv[e];
+            asm volatile("" : "+v"(t));
+            v[e] = (double)t;
+        }
+        fwd_phase_a<kTwScalar>(v, P.fw, 1, q, qinv);
+        lds_barrier();  // the previous prime's LDS reads are done
+#pragma unroll
+        for (int e = 0; e < 16; e++) lds[(tau + TPS * e) * S + col] = v[e];
+        lds_barrier();
+#pragma unroll
+        for (int e = 0; e < 16; e++) v[e] = lds[(16 * tau + e) * S + col];
+        col_phase_b<LOGR>(v, tw, q, qinv);
+        if (jl + 1 < je) tw.load(primes[pm(jl + 1)].fw, tau);
+#pragma unroll
+        for (int e = 0; e < 16; e++) (dst + (long long)N2 * e)[lane] = v[e];
+    }
+}
+
+template <int LOGR>
+static void digit_cols_t(const Context& c, const int32_t* digits, double* out, int g, int polys, PrimeMap pm,
+                         hipStream_t s) {
+    constexpr int S = 256 / ((1 << LOGR) / 16);
+    const int nx = (1 << (c.logn - LOGR)) / S;
+    // primes per workgroup: as many as keeps >= 8 workgroups per CU of the 256 in flight
+    const long long base = (long long)nx * polys;
+    int jper = (int)std::max(1LL, std::min<long long>(g, base * g / 2048));
+    const int ny = (g + jper - 1) / jper;
+    jper = (g + ny - 1) / ny;
+    ProfScope ps(s, "digit_cols");
+    k_digit_cols<LOGR><<<dim3(nx, ny, polys), 256, 0, s>>>(digits, out, g, jper, c.logn, pm, c.d_primes);
+    RCK_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
 // Digit MAC: the row pass of the forward NTT of every digit, fused with the multiply-
 // accumulate against the NTT-form key.  The column pass ran before into
 // cols[b][i][jl][N] for the eval limbs j0 .. j0+G-1 of this group.
@@ -237,6 +347,17 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
     const long long EN = (long long)E * n;
     const long long col_stride = (long long)G * n;  // digit i -> i + 1 in cols[b][i][jl][N]
     const double* pc = cols + (((long long)cb * D + i0) * G + jl) * n + off + tau;
+    // Schedules that keep loads in flight across the NTT stages (1: keys, 2: next digit's rows)
+    // read twiddles as global loads and use LDS-only barriers, so neither waits for those loads;
+    // the others consume every load at once and keep the generic path (measured faster for them).
+    constexpr bool kInFlight = PF == 1 || PF == 2;
+    constexpr int TWM = kInFlight ? kTwGlobal : kTwFlat;
+    auto sync = [] {
+        if constexpr (kInFlight)
+            lds_barrier();
+        else
+            __syncthreads();
+    };
     double a0[16], a1[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) a0[e] = a1[e] = 0.0;
@@ -267,14 +388,14 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
                 w1[e] = k1[e * TPS];
             }
         }
-        fwd_phase_a(v, P.fw, N1 + b, q, qinv);
-        __syncthreads();
+        fwd_phase_a<TWM>(v, P.fw, N1 + b, q, qinv);
+        sync();
 #pragma unroll
         for (int e = 0; e < 16; e++) lds[row_lds<R>(row, tau + TPS * e)] = v[e];
-        __syncthreads();
+        sync();
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
-        fwd_phase_b<LOGR>(v, P.fw, N1 + b, tau, q, qinv);
+        fwd_phase_b<LOGR, TWM>(v, P.fw, N1 + b, tau, q, qinv);
         if constexpr (PF == 3) {
             // key rows in four groups of four positions: at most 8 key values in flight per thread
             // (fewer live registers, more resident waves)
@@ -374,7 +495,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 8) void k_digit_mac8(const doubl
             const int half = 4 >> st;
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                if ((e & half) == 0) ct_bfly(v[e], v[e + half], tbl[(base << st) + (e >> (3 - st))], q, qinv);
+                if ((e & half) == 0) ct_bfly(v[e], v[e + half], tw_load<kTwFlat>(tbl, (base << st) + (e >> (3 - st))), q, qinv);
         }
 #pragma unroll
         for (int e = 0; e < 8; e++) v[e] = reduce(v[e], q, qinv);
@@ -391,7 +512,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 8) void k_digit_mac8(const doubl
 #pragma unroll
             for (int e = 0; e < 8; e++)
                 if ((e & half) == 0)
-                    ct_bfly(v[e], v[e + half], tbl[(base << st) + ((xb0 + (e << M)) >> (LOGR - st))], q, qinv);
+                    ct_bfly(v[e], v[e + half], tw_load<kTwFlat>(tbl, (base << st) + ((xb0 + (e << M)) >> (LOGR - st))), q, qinv);
         }
 #pragma unroll
         for (int e = 0; e < 8; e++) v[e] = reduce(v[e], q, qinv);
@@ -409,7 +530,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 8) void k_digit_mac8(const doubl
 #pragma unroll
                 for (int e = 0; e < 8; e++)
                     if ((e & half) == 0)
-                        ct_bfly(v[e], v[e + half], tbl[(base << st) + ((8 * tau + e) >> (LOGR - st))], q, qinv);
+                        ct_bfly(v[e], v[e + half], tw_load<kTwFlat>(tbl, (base << st) + ((8 * tau + e) >> (LOGR - st))), q, qinv);
             }
 #pragma unroll
             for (int e = 0; e < 8; e++) v[e] = reduce(v[e], q, qinv);
@@ -552,8 +673,13 @@ void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, con
         double* cols = cols_base + slot * slot_elems;
         if (ngroups > 1 && gi >= 2) RCK_CHECK_HIP(hipStreamWaitEvent(s2, cc.event(1 + slot), 0));
         // column pass of digit (b, i) for eval limbs j0 .. j0+g-1 -> cols[b][i][jl][N]
-        launch_fwd_cols(c, kInI32, digits, 0, n, cols, n, (long long)g * n, g, B * D, PrimeMap{nq, (int)c.L, j0},
-                        s2);
+        const PrimeMap pmg{nq, (int)c.L, j0};
+        switch (n1) {
+            case 6: digit_cols_t<6>(c, digits, cols, g, B * D, pmg, s2); break;
+            case 7: digit_cols_t<7>(c, digits, cols, g, B * D, pmg, s2); break;
+            case 8: digit_cols_t<8>(c, digits, cols, g, B * D, pmg, s2); break;
+            default: RCK_REQUIRE(false, "unsupported ring degree");
+        }
         if (ngroups > 1) {
             RCK_CHECK_HIP(hipEventRecord(cc.event(3 + slot), s2));
             RCK_CHECK_HIP(hipStreamWaitEvent(s, cc.event(3 + slot), 0));
@@ -778,7 +904,7 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
-    inv_phase_a(v, Pi.iv, 1, q, qinv);
+    inv_phase_a<kTwScalar>(v, Pi.iv, 1, q, qinv);
     const double* pp = pg + (long long)z * (K + 1) * n;
     const uint64_t* add = comp == 0 ? add0 : add1;
     uint64_t* o = outs.p[cb] + (long long)comp * P + (long long)i * n;

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_fwd_cols(Batch<const void> in, int in_e
     double v[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = load_in<IN>(src, r + (long long)N2 * (tau + TPS * e), q);
-    fwd_phase_a(v, P.fw, 1, q, qinv);
+    fwd_phase_a<kTwScalar>(v, P.fw, 1, q, qinv);
 #pragma unroll
     for (int e = 0; e < 16; e++) lds[(tau + TPS * e) * S + col] = v[e];
     __syncthreads();
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_inv_cols(Batch<const double> in, Batch<
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
-    inv_phase_a(v, P.iv, 1, q, qinv);
+    inv_phase_a<kTwScalar>(v, P.iv, 1, q, qinv);
     char* dst = static_cast<char*>(out.base) + ((long long)z * out.ps + (long long)y * out.ls) * out_elem;
 #pragma unroll
     for (int e = 0; e < 16; e++) {

@@ -47,6 +47,24 @@ __device__ __forceinline__ void gs_bfly(double& a, double& b, double w, double q
     b = mulmod(u - v, w, q, qinv);
 }
 
+// Twiddle-table reads.  The tables are device (global) memory.  kTwGlobal names that address
+// space, so the compiler counts these loads precisely and can keep other loads in flight across
+// them; kTwScalar (the index is the same in every lane: column passes, base 1) reads through the
+// scalar cache; kTwFlat is a generic load, after which the compiler waits for every outstanding
+// memory operation (measured faster in the digit MAC, whose loads are all consumed at once).
+enum TwMode { kTwGlobal = 0, kTwScalar = 1, kTwFlat = 2 };
+
+template <int MODE>
+__device__ __forceinline__ double tw_load(const double* tbl, int i) {
+    if constexpr (MODE == kTwScalar) {
+        return ((const __attribute__((address_space(4))) double*)tbl)[i];
+    } else if constexpr (MODE == kTwGlobal) {
+        return ((const __attribute__((address_space(1))) double*)tbl)[i];
+    } else {
+        return tbl[i];
+    }
+}
+
 // LDS index of a row tile with one pad word per 16 (conflict-free for the x = 16 tau + e reads).
 template <int R>
 __device__ __forceinline__ int row_lds(int row, int x) {
@@ -61,7 +79,8 @@ __device__ __forceinline__ void reduce_all(double (&v)[E], double q, double qinv
 
 // Phase A of a 2^LOGR-point sub-transform: the thread holds x = tau + (R/16) e, e < 16, and
 // runs stages 0..3 (distance bits LOGR-1 .. LOGR-4).  Twiddle of stage s, group i:
-// tbl[(base << s) + i].  Inputs |v| <= 0.51q; outputs reduced.
+// tbl[(base << s) + i].  Inputs |v| <= 0.51q; outputs reduced.  MODE: how the twiddles are read.
+template <int MODE = kTwGlobal>
 __device__ __forceinline__ void fwd_phase_a(double (&v)[16], const double* __restrict__ tbl, int base, double q,
                                             double qinv) {
 #pragma unroll
@@ -70,7 +89,7 @@ __device__ __forceinline__ void fwd_phase_a(double (&v)[16], const double* __res
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) {
-                double w = tbl[(base << s) + (e >> (4 - s))];
+                double w = tw_load<MODE>(tbl, (base << s) + (e >> (4 - s)));
                 ct_bfly(v[e], v[e + half], w, q, qinv);
             }
         }
@@ -79,7 +98,7 @@ __device__ __forceinline__ void fwd_phase_a(double (&v)[16], const double* __res
 }
 
 // Phase B: the thread holds x = 16 tau + e and runs stages 4..LOGR-1.
-template <int LOGR>
+template <int LOGR, int MODE = kTwGlobal>
 __device__ __forceinline__ void fwd_phase_b(double (&v)[16], const double* __restrict__ tbl, int base, int tau,
                                             double q, double qinv) {
 #pragma unroll
@@ -88,7 +107,7 @@ __device__ __forceinline__ void fwd_phase_b(double (&v)[16], const double* __res
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) {
-                double w = tbl[(base << s) + ((16 * tau + e) >> (LOGR - s))];
+                double w = tw_load<MODE>(tbl, (base << s) + ((16 * tau + e) >> (LOGR - s)));
                 ct_bfly(v[e], v[e + half], w, q, qinv);
             }
         }
@@ -97,7 +116,7 @@ __device__ __forceinline__ void fwd_phase_b(double (&v)[16], const double* __res
 }
 
 // Inverse phase B: stages LOGR-1 .. 4 (reduce every two stages: the sums double each stage).
-template <int LOGR>
+template <int LOGR, int MODE = kTwGlobal>
 __device__ __forceinline__ void inv_phase_b(double (&v)[16], const double* __restrict__ tbl, int base, int tau,
                                             double q, double qinv) {
     int cnt = 0;
@@ -107,7 +126,7 @@ __device__ __forceinline__ void inv_phase_b(double (&v)[16], const double* __res
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) {
-                double w = tbl[(base << s) + ((16 * tau + e) >> (LOGR - s))];
+                double w = tw_load<MODE>(tbl, (base << s) + ((16 * tau + e) >> (LOGR - s)));
                 gs_bfly(v[e], v[e + half], w, q, qinv);
             }
         }
@@ -120,6 +139,7 @@ __device__ __forceinline__ void inv_phase_b(double (&v)[16], const double* __res
 }
 
 // Inverse phase A: stages 3 .. 0 on x = tau + (R/16) e.
+template <int MODE = kTwGlobal>
 __device__ __forceinline__ void inv_phase_a(double (&v)[16], const double* __restrict__ tbl, int base, double q,
                                             double qinv) {
 #pragma unroll
@@ -128,7 +148,7 @@ __device__ __forceinline__ void inv_phase_a(double (&v)[16], const double* __res
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) {
-                double w = tbl[(base << s) + (e >> (4 - s))];
+                double w = tw_load<MODE>(tbl, (base << s) + (e >> (4 - s)));
                 gs_bfly(v[e], v[e + half], w, q, qinv);
             }
         }
