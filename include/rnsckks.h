@@ -324,7 +324,7 @@ int rck_batch_destroy(rck_batch* batch);
 int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uint32_t batch, char* buf, size_t len);
 /* The process-wide tuning table that per-op calls and rck_batch_compile consult (entries keyed by
  * ring degree, limbs, eval limbs, digits and batch size), as text: one line per entry,
- * "log2N limbs eval_limbs digits batch group digit_splits mac_rows load_schedule ms".  Export writes
+ * "log2N limbs eval_limbs digits batch group digit_splits mac_rows load_schedule mac_width ms".  Export writes
  * the table into buf (needed = bytes including the terminating NUL; buf may be NULL to query);
  * import validates every line, then merges the entries (a tuned group is not re-timed).  No
  * counterpart in the crate (it has no GPU launch shapes); lets a process reuse earlier tuning. */

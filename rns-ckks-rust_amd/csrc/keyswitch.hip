@@ -322,7 +322,8 @@ static void digit_cols_t(const Context& c, const int32_t* digits, double* out, i
 // ---------------------------------------------------------------------------------------
 // PF (load schedule): 0 = key rows loaded after the NTT stages; 1 = key rows of digit i issued
 // before its NTT stages; 2 = the column-pass rows of digit i+1 issued before digit i's stages
-// (software pipelining), key rows late.
+// (software pipelining), key rows late; 3 = key rows after the stages in four groups of four
+// positions; 4 = key rows issued between the LDS exchange and the second radix-16 phase.
 template <int LOGR, int ROWS, int PF>
 __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const double* __restrict__ cols, int D, int G,
                                                                     int B, int dsplit, int ntiles,
@@ -350,7 +351,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
     // Schedules that keep loads in flight across the NTT stages (1: keys, 2: next digit's rows)
     // read twiddles as global loads and use LDS-only barriers, so neither waits for those loads;
     // the others consume every load at once and keep the generic path (measured faster for them).
-    constexpr bool kInFlight = PF == 1 || PF == 2;
+    constexpr bool kInFlight = PF == 1 || PF == 2 || PF == 4;
     constexpr int TWM = kInFlight ? kTwGlobal : kTwFlat;
     auto sync = [] {
         if constexpr (kInFlight)
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
     for (int i = i0; i < i1; i++, pc += col_stride) {
         const double* k0 = key + (2LL * i) * EN + (long long)j * n + off + tau;
         const double* k1 = k0 + EN;
-        double v[16], w0[PF == 1 ? 16 : 1], w1[PF == 1 ? 16 : 1];
+        double v[16], w0[PF == 1 || PF == 4 ? 16 : 1], w1[PF == 1 || PF == 4 ? 16 : 1];
         if constexpr (PF == 2) {
 #pragma unroll
             for (int e = 0; e < 16; e++) v[e] = nx[e];
@@ -395,6 +396,13 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
         sync();
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
+        if constexpr (PF == 4) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                w0[e] = k0[e * TPS];
+                w1[e] = k1[e * TPS];
+            }
+        }
         fwd_phase_b<LOGR, TWM>(v, P.fw, N1 + b, tau, q, qinv);
         if constexpr (PF == 3) {
             // key rows in four groups of four positions: at most 8 key values in flight per thread
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
         } else {
 #pragma unroll
             for (int e = 0; e < 16; e++) {
-                if constexpr (PF == 1) {
+                if constexpr (PF == 1 || PF == 4) {
                     a0[e] += mulmod(v[e], w0[e], q, qinv);
                     a1[e] += mulmod(v[e], w1[e], q, qinv);
                 } else {
@@ -434,6 +442,133 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
     reduce_all(a0, q, qinv);
     reduce_all(a1, q, qinv);
     double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + off + tau;
+    double* o1 = o0 + EN;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        o0[e * TPS] = a0[e];
+        o1[e * TPS] = a1[e];
+    }
+}
+
+// Streaming variant: a workgroup is four waves, one ciphertext each, on the same key tile
+// (ring rows b0 .. b0 + ROWS - 1 of one eval limb, ROWS = 64 / (R / 16) rows per wave).
+//   * the column-pass rows of digit i + 1 are copied HBM -> LDS by global_load_lds while digit i
+//     is transformed (no registers held; the copy is waited for when digit i + 1 starts);
+//   * the row-pass twiddles of the tile are staged in LDS once and shared by the four waves;
+//   * the key rows are read in groups after the second phase; the four waves share them through
+//     the CU's L1 and L2.
+// Each wave's LDS buffer holds its rows at the padded pitch of row_lds (R + R/16 + 1): the copy
+// lands a row linearly, phase A reads it linearly and writes its outputs back in place at the
+// padded positions, which phase B reads without bank conflicts.
+
+// s_waitcnt immediates (vmcnt [3:0]+[15:14], expcnt [6:4], lgkmcnt [11:8]; the others at maximum)
+constexpr int kWaitVm0 = 0x0F70;
+constexpr int kWaitLgkm0 = 0xC07F;
+
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict__ cols, int D, int G, int B,
+                                                       int nbq, int dsplit, int ntiles,
+                                                       const double* __restrict__ key, int E, int j0,
+                                                       double* __restrict__ part, int logn, PrimeMap pm,
+                                                       const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, ROWS = 64 / TPS, PITCH = R + R / 16 + 1;
+    // copy granule per lane: 16 bytes (1 KiB per instruction) when a row is at least 1 KiB, else 4
+    constexpr int SZ = R * 8 >= 1024 ? 16 : 4, PER_ROW = R * 8 / (64 * SZ);
+    __shared__ double twl[ROWS * R];          // twl[row * R + (1 << s) + t]: stage-s twiddle t of the row
+    __shared__ double bufl[4][ROWS * PITCH];  // per wave: the digit's rows
+    const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / ROWS;
+    const int g = blockIdx.x, xcd = g & 7, slot = g >> 3;
+    const int cq = slot % nbq;
+    const int tile = (slot / nbq) * 8 + xcd;
+    if (tile >= ntiles) return;
+    const int rb = tile % nrb, rest = tile / nrb, jl = rest % G, sp = rest / G;
+    const int j = j0 + jl;
+    const int i0 = (int)((long long)D * sp / dsplit), i1 = (int)((long long)D * (sp + 1) / dsplit);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tau = lane % TPS, row = lane / TPS;
+    const int b0 = rb * ROWS, b = b0 + row;
+    const DevPrime P = primes[pm(j)];
+    const double q = P.q, qinv = P.qinv;
+    const int cb = cq * 4 + wave;
+    const bool active = cb < B;
+    const long long EN = (long long)E * n;
+    const long long col_stride = (long long)G * n;
+    double* buf = bufl[wave];
+    const double* tile0 = cols + (((long long)(active ? cb : 0) * D + i0) * G + jl) * n + (long long)b0 * R;
+    auto dma = [&](const double* src) {
+#pragma unroll
+        for (int r = 0; r < ROWS; r++) {
+#pragma unroll
+            for (int h = 0; h < PER_ROW; h++) {
+                const char* gsrc = reinterpret_cast<const char*>(src + r * R) + (h * 64 + lane) * SZ;
+                char* ldst = reinterpret_cast<char*>(buf + r * PITCH) + h * 64 * SZ;
+                if constexpr (SZ == 16)
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)gsrc,
+                                                     (__attribute__((address_space(3))) void*)ldst, 16, 0, 0);
+                else
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)gsrc,
+                                                     (__attribute__((address_space(3))) void*)ldst, 4, 0, 0);
+            }
+        }
+    };
+    if (active) dma(tile0);
+    // twiddles of the tile's rows: stage s < LOGR, t < 2^s
+    for (int t = tid; t < ROWS * R; t += 256) {
+        const int r = t / R, k = t % R;
+        if (k == 0) continue;
+        const int st = 31 - __builtin_clz(k);
+        twl[t] = tw_load<kTwGlobal>(P.fw, ((N1 + b0 + r) << st) + (k - (1 << st)));
+    }
+    __syncthreads();
+    if (!active) return;
+    const double* tw = twl + row * R;
+    double a0[16], a1[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) a0[e] = a1[e] = 0.0;
+    const double* src = tile0;
+    for (int i = i0; i < i1; i++, src += col_stride) {
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this digit's rows are in LDS
+        asm volatile("" ::: "memory");
+        double v[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) v[e] = buf[row * PITCH + tau + TPS * e];
+        fwd_phase_a<kTwFlat>(v, tw, 1, q, qinv);
+#pragma unroll
+        for (int e = 0; e < 16; e++) buf[row_lds<R>(row, tau + TPS * e)] = v[e];
+#pragma unroll
+        for (int e = 0; e < 16; e++) v[e] = buf[row_lds<R>(row, 16 * tau + e)];
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // every read of the buffer has completed
+        asm volatile("" ::: "memory");
+        if (i + 1 < i1) dma(src + col_stride);
+        __builtin_amdgcn_sched_barrier(0);
+        fwd_phase_b<LOGR, kTwFlat>(v, tw, 1, tau, q, qinv);
+        // key rows in four groups of four positions (issued after the copy: by the time they are
+        // waited for, the copy has had the whole second phase to land)
+        const double* k0 = key + (2LL * i) * EN + (long long)j * n + (long long)b * R + tau;
+        const double* k1 = k0 + EN;
+#pragma unroll
+        for (int g4 = 0; g4 < 4; g4++) {
+            double x0[4], x1[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                x0[e] = k0[(4 * g4 + e) * TPS];
+                x1[e] = k1[(4 * g4 + e) * TPS];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                a0[4 * g4 + e] += mulmod(v[4 * g4 + e], x0[e], q, qinv);
+                a1[4 * g4 + e] += mulmod(v[4 * g4 + e], x1[e], q, qinv);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (((i - i0) & 3) == 3) {
+            reduce_all(a0, q, qinv);
+            reduce_all(a1, q, qinv);
+        }
+    }
+    reduce_all(a0, q, qinv);
+    reduce_all(a1, q, qinv);
+    double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + (long long)b * R + tau;
     double* o1 = o0 + EN;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
@@ -572,6 +707,20 @@ static void mac8_t(const Context& c, const double* cols, int D, int G, int B, in
     RCK_CHECK_HIP(hipGetLastError());
 }
 
+template <int LOGR>
+static void mac_dma_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
+                      int j0, double* part, PrimeMap pm, hipStream_t s) {
+    constexpr int ROWS = 64 / ((1 << LOGR) / 16);
+    const int nrb = (1 << (c.logn - LOGR)) / ROWS;
+    const int ntiles = nrb * G * dsplit;
+    const int nbq = (B + 3) / 4;
+    const int blocks = ((ntiles + 7) / 8) * 8 * nbq;
+    ProfScope ps(s, "digit_mac");
+    k_digit_mac_dma<LOGR><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E, j0, part, c.logn,
+                                                       pm, c.d_primes);
+    RCK_CHECK_HIP(hipGetLastError());
+}
+
 template <int LOGR, int ROWS, int PF>
 static void mac_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
                   int j0, double* part, PrimeMap pm, hipStream_t s) {
@@ -588,8 +737,12 @@ static void mac_t(const Context& c, const double* cols, int D, int G, int B, int
 template <int LOGR, int ROWS>
 static void mac_r(const Context& c, int pf, const double* cols, int D, int G, int B, int dsplit, const double* key,
                   int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
-    if (pf == 1)
+    if (pf == 5)
+        mac_dma_t<LOGR>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    else if (pf == 1)
         mac_t<LOGR, ROWS, 1>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+    else if (pf == 4)
+        mac_t<LOGR, ROWS, 4>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
     else if (pf == 3)
         mac_t<LOGR, ROWS, 3>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
     else if (pf == 2)
@@ -845,16 +998,6 @@ __device__ __forceinline__ bool p_garner(const double* r, int K, int L, int T, d
     return gt;
 }
 
-// centred(x_P) mod q_i from the Garner digits and the centring flag
-__device__ __forceinline__ double p_centred_mod_q(const double* m, bool gt, int K, int L, int i, double q,
-                                                  double qinv, const double* __restrict__ pmodq,
-                                                  const double* __restrict__ pmod_aux) {
-    double cv = reduce(m[K - 1], q, qinv);
-    for (int k = K - 2; k >= 0; k--) cv = reduce(mulmod(cv, pmod_aux[(long long)k * L + i], q, qinv) + m[k], q, qinv);
-    if (gt) cv -= pmodq[i];
-    return cv;
-}
-
 // Once per coefficient of the aux part: pg[z][k][c] = Garner digit m_k (k < K), pg[z][K][c] = 1 when
 // x_P > floor(P/2) else 0.  accP: [Z][K][N] canonical residues.
 __global__ void k_p_garner(const uint64_t* __restrict__ accP, double* __restrict__ pg, int n, int K, int L, int T,
@@ -871,14 +1014,12 @@ __global__ void k_p_garner(const uint64_t* __restrict__ accP, double* __restrict
     o[(long long)K * n] = gt ? 1.0 : 0.0;
 }
 
-template <int LOGR>
+template <int LOGR, int KT>
 __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ acc, long long EN, int nq,
-                                                     const double* __restrict__ pg, int K, int L, int T,
+                                                     const double* __restrict__ pg, int L,
                                                      const uint64_t* __restrict__ add0, const uint64_t* __restrict__ add1,
                                                      long long add_ps, OutPtrs outs, long long P, int logn,
                                                      const DevPrime* __restrict__ primes,
-                                                     const double* __restrict__ invq,
-                                                     const uint32_t* __restrict__ pwords,
                                                      const double* __restrict__ pmodq,
                                                      const double* __restrict__ pinvq,
                                                      const double* __restrict__ pmod_aux) {
@@ -905,21 +1046,38 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
     inv_phase_a<kTwScalar>(v, Pi.iv, 1, q, qinv);
-    const double* pp = pg + (long long)z * (K + 1) * n;
+    // mod-down epilogue in chunks of four positions, each chunk's loads issued together
+    const double* pp = pg + (long long)z * (KT + 1) * n;
     const uint64_t* add = comp == 0 ? add0 : add1;
+    const uint64_t* ad = add ? add + (long long)cb * add_ps + (long long)i * n : nullptr;
     uint64_t* o = outs.p[cb] + (long long)comp * P + (long long)i * n;
-    const double pinv = pinvq[i];
+    const double pinv = pinvq[i], pmq = pmodq[i];
+    double paux[KT > 1 ? KT - 1 : 1];
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        const long long idx = r + (long long)N2 * (tau + TPS * e);
-        const double x = mulmod(v[e], Pi.ninv, q, qinv);  // x_i, |x| <= 0.94 q
-        double mp[kMaxP];
-        for (int k = 0; k < K; k++) mp[k] = pp[(long long)k * n + idx];
-        const bool gt = pp[(long long)K * n + idx] != 0.0;
-        const double cv = p_centred_mod_q(mp, gt, K, L, i, q, qinv, pmodq, pmod_aux);
-        double res = mulmod(reduce(x - cv, q, qinv), pinv, q, qinv);
-        if (add) res += (double)add[(long long)cb * add_ps + (long long)i * n + idx];
-        o[idx] = f64_to_u64(canon(res, q, qinv));
+    for (int kk = 0; kk + 1 < KT; kk++) paux[kk] = pmod_aux[(long long)kk * L + i];
+#pragma unroll
+    for (int e0 = 0; e0 < 16; e0 += 4) {
+        double mp[4][KT + 1], av[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int idx = r + N2 * (tau + TPS * (e0 + e));
+#pragma unroll
+            for (int kk = 0; kk <= KT; kk++) mp[e][kk] = pp[(long long)kk * n + idx];
+            av[e] = ad ? (double)ad[idx] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int idx = r + N2 * (tau + TPS * (e0 + e));
+            const double x = mulmod(v[e0 + e], Pi.ninv, q, qinv);  // x_i, |x| <= 0.94 q
+            // centred(x_P) mod q_i from the aux part's Garner digits m_k and its centring flag:
+            // Horner over m_{K-1} .. m_0 with p_k mod q_i, minus P mod q_i when x_P > floor(P/2)
+            double cv = reduce(mp[e][KT - 1], q, qinv);
+#pragma unroll
+            for (int kk = KT - 2; kk >= 0; kk--) cv = reduce(mulmod(cv, paux[kk], q, qinv) + mp[e][kk], q, qinv);
+            if (mp[e][KT] != 0.0) cv -= pmq;
+            const double res = mulmod(reduce(x - cv, q, qinv), pinv, q, qinv) + av[e];
+            o[idx] = f64_to_u64(canon(res, q, qinv));
+        }
     }
 }
 
@@ -951,15 +1109,23 @@ void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B,
         const int S = 256 / ((1 << n1) / 16);
         const int ncb = (1 << n2) / S;
         dim3 grid(((ncb + 7) / 8) * 8 * nq, 1, 2 * bc);
-#define RCK_MD(LR)                                                                                                \
-    k_inv_cols_md<LR><<<grid, 256, 0, s>>>(accb, EN, nq, accPb, c.K, c.L, c.L + c.K, a0, a1, add_ps, op, P, c.logn, \
-                                           c.d_primes, c.d_invq, c.d_pwords, c.d_pmodq, c.d_pinvq, c.d_pmod_aux)
+#define RCK_MD(LR, KK)                                                                                      \
+    k_inv_cols_md<LR, KK><<<grid, 256, 0, s>>>(accb, EN, nq, accPb, c.L, a0, a1, add_ps, op, P, c.logn, c.d_primes, \
+                                               c.d_pmodq, c.d_pinvq, c.d_pmod_aux)
+#define RCK_MDK(LR)                                     \
+    switch (c.K) {                                      \
+        case 1: RCK_MD(LR, 1); break;                   \
+        case 2: RCK_MD(LR, 2); break;                   \
+        case 3: RCK_MD(LR, 3); break;                   \
+        default: RCK_MD(LR, 4); break;                  \
+    }
         switch (n1) {
-            case 6: RCK_MD(6); break;
-            case 7: RCK_MD(7); break;
-            case 8: RCK_MD(8); break;
+            case 6: RCK_MDK(6); break;
+            case 7: RCK_MDK(7); break;
+            case 8: RCK_MDK(8); break;
             default: RCK_REQUIRE(false, "unsupported ring degree");
         }
+#undef RCK_MDK
 #undef RCK_MD
         RCK_CHECK_HIP(hipGetLastError());
     }

@@ -39,7 +39,9 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 // NTT-form doubles; part: [dsplit][B][2][nq+naux][N] partial sums over digit ranges.  Eval limbs
 // are processed `group` at a time (column-pass scratch of B*D*group limbs).
 // rows: ring rows per MAC workgroup (4 or 16; 8 or 16 with width 8); prefetch: load schedule (0 late
-// keys, 1 early keys, 2 next digit's rows prefetched, 3 keys in groups of four positions).
+// keys, 1 early keys, 2 next digit's rows prefetched, 3 keys in groups of four positions, 4 keys
+// issued before the second radix-16 phase, 5 four ciphertexts per workgroup with the next digit's
+// rows copied into LDS while the current one is transformed; rows 4 only).
 // width: elements per thread in the MAC's row pass (16, or 8: half the registers, more resident waves).
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
                       double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s);

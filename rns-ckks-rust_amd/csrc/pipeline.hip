@@ -194,7 +194,7 @@ int ks_tuned_import(const std::string& text) {
                                     &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &ms, &extra);
         RCK_REQUIRE(got == 11, "malformed tuning table line: " + line);
         RCK_REQUIRE(v[1] >= 1 && v[2] >= v[1] && v[3] >= 1 && v[4] >= 1, "tuning table entry out of range: " + line);
-        const bool w16 = v[9] == 16 && (v[7] == 4 || v[7] == 16) && v[8] >= 0 && v[8] <= 3;
+        const bool w16 = v[9] == 16 && (v[7] == 4 || v[7] == 16) && v[8] >= 0 && (v[8] <= 4 || (v[8] == 5 && v[7] == 4));
         const bool w8 = v[9] == 8 && (v[7] == 8 || v[7] == 16) && v[8] == 0;
         RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (w16 || w8),
                     "tuning table launch shape out of range: " + line);
@@ -242,11 +242,11 @@ std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
         for (int ds : {1, 2, 4}) {
             if (ds > D) continue;
             for (int rows : {4, 16})
-                for (int pf : {0, 1, 2, 3}) out.push_back({g, ds, rows, pf, 16});
+                for (int pf : {0, 1, 2, 3, 4}) out.push_back({g, ds, rows, pf, 16});
+            if (B >= 2) out.push_back({g, ds, 4, 5, 16});  // four ciphertexts per workgroup, LDS-streamed rows
             if (g == E && ds <= 2)
                 for (int rows : {8, 16}) out.push_back({g, ds, rows, 0, 8});
         }
-    (void)B;
     return out;
 }
 

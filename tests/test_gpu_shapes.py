@@ -20,7 +20,8 @@ def uniform(moduli, n, rs, polys):
 
 SHAPES = [  # group, dsplit, rows, prefetch, width
     (None, 1, 16, 0, 16), (None, 2, 4, 1, 16), (None, 1, 4, 2, 16), (None, 1, 16, 3, 16), (None, 1, 8, 0, 8),
-    (None, 2, 16, 0, 8), (4, 1, 8, 0, 8), (1, 1, 16, 3, 16),
+    (None, 2, 16, 0, 8), (4, 1, 8, 0, 8), (1, 1, 16, 3, 16), (None, 1, 4, 4, 16), (None, 2, 16, 4, 16),
+    (None, 1, 4, 5, 16), (None, 2, 4, 5, 16), (3, 1, 4, 5, 16),
 ]
 
 
@@ -44,3 +45,39 @@ def test_mac_shapes_bit_identical(logn, limbs):
         out.upload(np.zeros_like(want))
         ctx.relinearize(quad, limbs, 80, k, out)
         assert np.array_equal(out.numpy(), want), (grp, ds, rows, pf, width)
+
+
+@pytest.mark.parametrize("logn,limbs", [(12, 4), (13, 5), (16, 6)])
+def test_mac_streamed_batch_bit_identical(logn, limbs):
+    """The four-ciphertexts-per-workgroup MAC (rows copied into LDS while the previous digit is
+    transformed) on a batch of 5 mult+relin (one workgroup with a single active wave) gives the
+    per-op results."""
+    n, B = 1 << logn, 5
+    p = PR.config_batched(n=1 << 16, limbs=limbs, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    rs = np.random.default_rng(100 + logn)
+    D = ctx.num_decomp_digits(limbs)
+    E = limbs + 2
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    k = ctx.ksk_create(key, limbs - 1, 2)
+    P = limbs * n
+    lhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B).reshape(B, 2, limbs, n))
+    rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B).reshape(B, 2, limbs, n))
+    quad = G.DeviceBuffer((3, limbs, n))
+    one = G.DeviceBuffer((2, limbs, n))
+    want = []
+    for b in range(B):
+        ctx.mult(lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), limbs, 40, 40, quad)
+        ctx.relinearize(quad, limbs, 80, k, one)
+        want.append(one.numpy())
+    for ds in (1, 2):
+        G.tune_table_import(f"{logn} {limbs} {E} {D} {B} {E} {ds} 4 5 16 1.0\n")
+        out = G.DeviceBuffer((B, 2, limbs, n))
+        bt = G.Batch(ctx)
+        for b in range(B):
+            bt.add(G.OP_MULT_RELIN, limbs, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), k, out.offset(b * 2 * P))
+        bt.compile(tune=True)
+        assert bt.report()["groups"][0]["load_schedule"] == 5
+        got = out.numpy()
+        for b in range(B):
+            assert np.array_equal(got[b], want[b]), (ds, b)
