@@ -331,7 +331,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
                                                                     double* __restrict__ part, int logn, PrimeMap pm,
                                                                     const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 16;
-    __shared__ double lds[ROWS * (R + R / 16 + 1)];
+    __shared__ double lds[ROWS * row_pitch<R>()];
     const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / ROWS;
     const int g = blockIdx.x, xcd = g & 7, slot = g >> 3;
     const int cb = slot % B;
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
 //   * the row-pass twiddles of the tile are staged in LDS once and shared by the four waves;
 //   * the key rows are read in groups after the second phase; the four waves share them through
 //     the CU's L1 and L2.
-// Each wave's LDS buffer holds its rows at the padded pitch of row_lds (R + R/16 + 1): the copy
+// Each wave's LDS buffer holds its rows at the padded pitch of row_lds (R + R/16): the copy
 // lands a row linearly, phase A reads it linearly and writes its outputs back in place at the
 // padded positions, which phase B reads without bank conflicts.
 
@@ -465,17 +465,65 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
 constexpr int kWaitVm0 = 0x0F70;
 constexpr int kWaitLgkm0 = 0xC07F;
 
+// Row-pass stages with the twiddles staged in LDS in lane-friendly layouts:
+//   phase A (stages 0..3, the same 15 entries for every lane of a ring row):
+//     twa[k * ROWS + row], k = (1 << s) - 1 + t  (the ROWS rows of one k sit side by side);
+//   phase B (stages 4..LOGR-1, 16 >> (LOGR - s) entries of stage s per lane):
+//     twb[k * 64 + lane], k = off_s + t  (the 64 lanes of one k sit side by side).
+// Entry t of stage s for ring row b is fw[((N1 + b) << s) + t'] with t' = t (phase A) or
+// t' = (16 tau >> (LOGR - s)) + t (phase B), as in fwd_phase_a / fwd_phase_b.
+template <int LOGR>
+struct MacTw {
+    static constexpr int R = 1 << LOGR, TPS = R / 16, ROWS = 64 / TPS;
+    static constexpr int countb() {
+        int c = 0;
+        for (int s = 4; s < LOGR; s++) c += 16 >> (LOGR - s);
+        return c;
+    }
+    static constexpr int SIZE = 15 * ROWS + countb() * 64;
+};
+
+template <int LOGR>
+__device__ __forceinline__ void mac_phase_a(double (&v)[16], const double* twa, double q, double qinv) {
+    constexpr int ROWS = MacTw<LOGR>::ROWS;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int half = 8 >> s;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if ((e & half) == 0) ct_bfly(v[e], v[e + half], twa[((1 << s) - 1 + (e >> (4 - s))) * ROWS], q, qinv);
+        }
+    }
+    reduce_all(v, q, qinv);
+}
+
+template <int LOGR>
+__device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, double q, double qinv) {
+    constexpr int R = 1 << LOGR;
+    int off = 0;
+#pragma unroll
+    for (int s = 4; s < LOGR; s++) {
+        const int half = R >> (s + 1), m = LOGR - s;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            if ((e & half) == 0) ct_bfly(v[e], v[e + half], twb[(off + (e >> m)) * 64], q, qinv);
+        }
+        off += 16 >> m;
+    }
+    reduce_all(v, q, qinv);
+}
+
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict__ cols, int D, int G, int B,
                                                        int nbq, int dsplit, int ntiles,
                                                        const double* __restrict__ key, int E, int j0,
                                                        double* __restrict__ part, int logn, PrimeMap pm,
                                                        const DevPrime* __restrict__ primes) {
-    constexpr int R = 1 << LOGR, TPS = R / 16, ROWS = 64 / TPS, PITCH = R + R / 16 + 1;
+    constexpr int R = 1 << LOGR, TPS = R / 16, ROWS = 64 / TPS, PITCH = row_pitch<R>();
     // copy granule per lane: 16 bytes (1 KiB per instruction) when a row is at least 1 KiB, else 4
     constexpr int SZ = R * 8 >= 1024 ? 16 : 4, PER_ROW = R * 8 / (64 * SZ);
-    __shared__ double twl[ROWS * R];          // twl[row * R + (1 << s) + t]: stage-s twiddle t of the row
-    __shared__ double bufl[4][ROWS * PITCH];  // per wave: the digit's rows
+    __shared__ double twl[MacTw<LOGR>::SIZE];  // [phase A: 15 x ROWS][phase B: countb x 64]
+    __shared__ double bufl[4][ROWS * PITCH];   // per wave: the digit's rows
     const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / ROWS;
     const int g = blockIdx.x, xcd = g & 7, slot = g >> 3;
     const int cq = slot % nbq;
@@ -512,16 +560,27 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
         }
     };
     if (active) dma(tile0);
-    // twiddles of the tile's rows: stage s < LOGR, t < 2^s
-    for (int t = tid; t < ROWS * R; t += 256) {
-        const int r = t / R, k = t % R;
-        if (k == 0) continue;
-        const int st = 31 - __builtin_clz(k);
-        twl[t] = tw_load<kTwGlobal>(P.fw, ((N1 + b0 + r) << st) + (k - (1 << st)));
+    // stage the tile's twiddles (layouts at MacTw)
+    for (int t = tid; t < MacTw<LOGR>::SIZE; t += 256) {
+        int idx;
+        if (t < 15 * ROWS) {
+            const int k = t / ROWS, r = t % ROWS, st = 31 - __builtin_clz(k + 1);
+            idx = ((N1 + b0 + r) << st) + (k + 1 - (1 << st));
+        } else {
+            const int u = t - 15 * ROWS, k = u / 64, ln = u % 64, r = ln / TPS, ta = ln % TPS;
+            int st = 4, off = 0;
+            while (k >= off + (16 >> (LOGR - st))) {
+                off += 16 >> (LOGR - st);
+                st++;
+            }
+            idx = ((N1 + b0 + r) << st) + ((16 * ta) >> (LOGR - st)) + (k - off);
+        }
+        twl[t] = tw_load<kTwGlobal>(P.fw, idx);
     }
     __syncthreads();
     if (!active) return;
-    const double* tw = twl + row * R;
+    const double* twa = twl + row;
+    const double* twb = twl + 15 * ROWS + lane;
     double a0[16], a1[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) a0[e] = a1[e] = 0.0;
@@ -532,7 +591,7 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
         double v[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = buf[row * PITCH + tau + TPS * e];
-        fwd_phase_a<kTwFlat>(v, tw, 1, q, qinv);
+        mac_phase_a<LOGR>(v, twa, q, qinv);
 #pragma unroll
         for (int e = 0; e < 16; e++) buf[row_lds<R>(row, tau + TPS * e)] = v[e];
 #pragma unroll
@@ -541,7 +600,7 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
         asm volatile("" ::: "memory");
         if (i + 1 < i1) dma(src + col_stride);
         __builtin_amdgcn_sched_barrier(0);
-        fwd_phase_b<LOGR, kTwFlat>(v, tw, 1, tau, q, qinv);
+        mac_phase_b<LOGR>(v, twb, q, qinv);
         // key rows in four groups of four positions (issued after the copy: by the time they are
         // waited for, the copy has had the whole second phase to land)
         const double* k0 = key + (2LL * i) * EN + (long long)j * n + (long long)b * R + tau;
@@ -594,7 +653,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 8) void k_digit_mac8(const doubl
                                                                      const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 8, T16 = R / 16, M = LOGR - 6;
     static_assert(LOGR >= 6 && LOGR <= 8, "row length 64 .. 256");
-    __shared__ double lds[ROWS * (R + R / 16 + 1)];
+    __shared__ double lds[ROWS * row_pitch<R>()];
     const int n = 1 << logn, N1 = n >> LOGR, nrb = N1 / ROWS;
     const int g = blockIdx.x, xcd = g & 7, slot = g >> 3;
     const int cb = slot % B;

@@ -54,7 +54,7 @@ template <int LOGR>
 __global__ __launch_bounds__(256) void k_fwd_rows(Batch<double> data, int logn, PrimeMap pm,
                                                   const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
-    __shared__ double lds[S * (R + R / 16 + 1)];
+    __shared__ double lds[S * row_pitch<R>()];
     const int N1 = 1 << (logn - LOGR);
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
     const int b = blockIdx.x * S + row;
@@ -83,7 +83,7 @@ template <int LOGR>
 __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, PrimeMap pm,
                                                   const DevPrime* __restrict__ primes, int nsum, long long sstride) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
-    __shared__ double lds[S * (R + R / 16 + 1)];
+    __shared__ double lds[S * row_pitch<R>()];
     const int N1 = 1 << (logn - LOGR);
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
     const int b = blockIdx.x * S + row;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_mult_rows(const double* __restrict__ xl
                                                    double* __restrict__ d, long long P, int logn, PrimeMap pm,
                                                    const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
-    __shared__ double lds[S * (R + R / 16 + 1)];
+    __shared__ double lds[S * row_pitch<R>()];
     const int n = 1 << logn, N1 = n >> LOGR;
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
     const int b = blockIdx.x * S + row;

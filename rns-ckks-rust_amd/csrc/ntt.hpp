@@ -66,9 +66,15 @@ __device__ __forceinline__ double tw_load(const double* tbl, int i) {
 }
 
 // LDS index of a row tile with one pad word per 16 (conflict-free for the x = 16 tau + e reads).
+// The row pitch R + R/16 is 16, 8, 4 doubles past a multiple of 32 for R = 256, 128, 64, so the
+// rows that one half-wave touches (2, 4, 8 of them) land on disjoint banks as well.
+template <int R>
+constexpr int row_pitch() {
+    return R + R / 16;
+}
 template <int R>
 __device__ __forceinline__ int row_lds(int row, int x) {
-    return row * (R + R / 16 + 1) + x + (x >> 4);
+    return row * row_pitch<R>() + x + (x >> 4);
 }
 
 template <int E>
