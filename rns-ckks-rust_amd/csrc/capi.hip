@@ -410,10 +410,8 @@ int rck_mult_relinearized(rck_context* ctx, const uint64_t* lhs, const uint64_t*
         RCK_REQUIRE(lhs_scale == rhs_scale, "ciphertexts must share the same scale to multiply");
         RCK_REQUIRE(limbs > 1, "rescale requires at least two levels");
         const long long P = (long long)limbs * c.n;
-        uint64_t* quad = static_cast<uint64_t*>(c.workspace(2, (size_t)3 * P * 8));
         uint64_t* rel = static_cast<uint64_t*>(c.workspace(3, (size_t)2 * P * 8));
-        mult_impl(c, lhs, rhs, limbs, quad, S(stream));
-        keyswitch_impl(c, quad + 2 * P, limbs, key, quad, quad + P, rel, S(stream));
+        mult_relin_batch(c, lhs, rhs, 0, 1, limbs, key, {rel}, S(stream));
         rescale_impl(c, rel, limbs, 2, out, S(stream));
         if (out_scale) {
             uint32_t s = lhs_scale + rhs_scale, b = floor_log2(c.hp[limbs - 1].q);

@@ -79,9 +79,14 @@ __global__ __launch_bounds__(256) void k_fwd_rows(Batch<double> data, int logn, 
 // Row pass, inverse (in place).
 // nsum > 1: the input is the sum of nsum partial operands at element stride sstride (digit-split
 // keyswitch accumulators); the result is written over the first one.
+// addn (nullable): for the first nq limbs, P * addn[z >> 1][z & 1] (NTT form, P mod q_y = pmodq[y]) is
+// added too: the relinearization terms folded into keyswitch accumulators [B][2][E][N] (mod-down by P
+// then returns keyswitch + addend exactly).
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, PrimeMap pm,
-                                                  const DevPrime* __restrict__ primes, int nsum, long long sstride) {
+                                                  const DevPrime* __restrict__ primes, int nsum, long long sstride,
+                                                  const double* __restrict__ addn, long long addn_bs,
+                                                  long long addn_cs, int nq, const double* __restrict__ pmodq) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[S * row_pitch<R>()];
     const int N1 = 1 << (logn - LOGR);
@@ -94,12 +99,20 @@ __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, 
     double v[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = p[e * TPS + tau];  // NTT-domain row order (ntt.hpp)
-    if (nsum > 1) {
+    const bool fold = addn != nullptr && y + pm.yoff < nq;
+    if (nsum > 1 || fold) {
         for (int k = 1; k < nsum; k++) {
 #pragma unroll
             for (int e = 0; e < 16; e++) v[e] += p[k * sstride + e * TPS + tau];
         }
-        reduce_all(v, q, qinv);  // |sum| <= nsum * 0.51q < 2^53 for nsum <= 8
+        if (fold) {
+            const double* an = addn + (long long)(z >> 1) * addn_bs + (long long)(z & 1) * addn_cs +
+                               (long long)(y + pm.yoff) * (1LL << logn) + (long long)b * R;
+            const double pm_q = pmodq[y + pm.yoff];
+#pragma unroll
+            for (int e = 0; e < 16; e++) v[e] += mulmod(an[e * TPS + tau], pm_q, q, qinv);
+        }
+        reduce_all(v, q, qinv);  // |sum| <= nsum * 0.51q + 0.94q < 2^53 for nsum <= 8
     }
     inv_phase_b<LOGR>(v, P.iv, N1 + b, tau, q, qinv);
 #pragma unroll
@@ -150,7 +163,9 @@ __device__ __forceinline__ void row_inv_store(double (&v)[16], double* p, double
     for (int e = 0; e < 16; e++) p[tau + TPS * e] = v[e];
 }
 
-template <int LOGR>
+// FOLD: d0 and d1 are stored in NTT form (NTT-domain row order) for a keyswitch that folds them into
+// its accumulators; only d2 gets the inverse row stages.
+template <int LOGR, bool FOLD>
 __global__ __launch_bounds__(256) void k_mult_rows(const double* __restrict__ xl, const double* __restrict__ xr,
                                                    double* __restrict__ d, long long P, int logn, PrimeMap pm,
                                                    const DevPrime* __restrict__ primes) {
@@ -178,8 +193,16 @@ __global__ __launch_bounds__(256) void k_mult_rows(const double* __restrict__ xl
         a0[e] = reduce(mulmod(a0[e], b0[e], q, qinv), q, qinv);
         a1[e] = reduce(mulmod(a1[e], b1[e], q, qinv), q, qinv);
     }
-    row_inv_store<LOGR>(a0, dz, lds, row, tau, N1 + b, Pr.iv, q, qinv);
-    row_inv_store<LOGR>(t, dz + P, lds, row, tau, N1 + b, Pr.iv, q, qinv);
+    if constexpr (FOLD) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            dz[e * TPS + tau] = a0[e];
+            dz[P + e * TPS + tau] = t[e];
+        }
+    } else {
+        row_inv_store<LOGR>(a0, dz, lds, row, tau, N1 + b, Pr.iv, q, qinv);
+        row_inv_store<LOGR>(t, dz + P, lds, row, tau, N1 + b, Pr.iv, q, qinv);
+    }
     row_inv_store<LOGR>(a1, dz + 2 * P, lds, row, tau, N1 + b, Pr.iv, q, qinv);
 }
 
@@ -317,11 +340,12 @@ static void fwd_rows_t(const Context& c, Batch<double> d, int limbs, int polys, 
 
 template <int LOGR>
 static void inv_rows_t(const Context& c, Batch<double> d, int limbs, int polys, PrimeMap pm, hipStream_t s, int nsum,
-                       long long sstride) {
+                       long long sstride, const double* addn, long long addn_bs, long long addn_cs, int nq) {
     constexpr int S = 256 / ((1 << LOGR) / 16);
     const int n1 = 1 << (c.logn - LOGR);
     ProfScope ps(s, "ntt_inv_rows");
-    k_inv_rows<LOGR><<<dim3(n1 / S, limbs, polys), 256, 0, s>>>(d, c.logn, pm, c.d_primes, nsum, sstride);
+    k_inv_rows<LOGR><<<dim3(n1 / S, limbs, polys), 256, 0, s>>>(d, c.logn, pm, c.d_primes, nsum, sstride, addn,
+                                                                addn_bs, addn_cs, nq, c.d_pmodq);
     RCK_LAUNCH_CHECK();
 }
 
@@ -343,25 +367,28 @@ static bool small_ring(const Context& c) { return c.logn <= 11; }
 
 template <int LOGR>
 static void mult_rows_t(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
-                        hipStream_t s) {
+                        hipStream_t s, bool fold) {
     constexpr int S = 256 / ((1 << LOGR) / 16);
     const int n1 = 1 << (c.logn - LOGR);
     ProfScope ps(s, "mult_rows");
-    k_mult_rows<LOGR><<<dim3(n1 / S, limbs, B), 256, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm,
-                                                              c.d_primes);
+    const dim3 grid(n1 / S, limbs, B);
+    if (fold)
+        k_mult_rows<LOGR, true><<<grid, 256, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
+    else
+        k_mult_rows<LOGR, false><<<grid, 256, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
     RCK_LAUNCH_CHECK();
 }
 
 void launch_mult_rows(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
-                      hipStream_t s) {
+                      hipStream_t s, bool fold) {
     check_dims(c, limbs, B);
     RCK_REQUIRE(!small_ring(c), "fused tensor rows need N >= 2^12");
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
     switch (n2) {
-        case 6: mult_rows_t<6>(c, xl, xr, d, limbs, B, pm, s); break;
-        case 7: mult_rows_t<7>(c, xl, xr, d, limbs, B, pm, s); break;
-        case 8: mult_rows_t<8>(c, xl, xr, d, limbs, B, pm, s); break;
+        case 6: mult_rows_t<6>(c, xl, xr, d, limbs, B, pm, s, fold); break;
+        case 7: mult_rows_t<7>(c, xl, xr, d, limbs, B, pm, s, fold); break;
+        case 8: mult_rows_t<8>(c, xl, xr, d, limbs, B, pm, s, fold); break;
         default: RCK_REQUIRE(false, "unsupported ring degree");
     }
 }
@@ -397,16 +424,18 @@ void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps,
 }
 
 void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
-                     hipStream_t s, int nsum, long long sstride) {
+                     hipStream_t s, int nsum, long long sstride, const double* addn, long long addn_bs,
+                     long long addn_cs, int nq) {
     check_dims(c, limbs, polys);
     RCK_REQUIRE(nsum >= 1 && nsum <= 8, "at most 8 partial sums");
+    RCK_REQUIRE(addn == nullptr || (polys % 2 == 0 && nq <= limbs && nq <= (int)c.L), "bad folded addend");
     Batch<double> b{data, ls, ps};
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
     switch (n2) {
-        case 6: inv_rows_t<6>(c, b, limbs, polys, pm, s, nsum, sstride); break;
-        case 7: inv_rows_t<7>(c, b, limbs, polys, pm, s, nsum, sstride); break;
-        case 8: inv_rows_t<8>(c, b, limbs, polys, pm, s, nsum, sstride); break;
+        case 6: inv_rows_t<6>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq); break;
+        case 7: inv_rows_t<7>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq); break;
+        case 8: inv_rows_t<8>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq); break;
         default: RCK_REQUIRE(false, "unsupported ring degree");
     }
 }

@@ -212,12 +212,15 @@ void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps,
 void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, int out_kind, void* out,
                 long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, int nsum = 1,
                 long long sstride = 0);
+// addn (nullable): for polys = 2B keyswitch accumulators [B][2][limbs][N], adds P * addn (NTT form, ciphertext
+// stride addn_bs, component stride addn_cs, limb stride N) to the first nq limbs
 void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
-                     hipStream_t s, int nsum = 1, long long sstride = 0);
+                     hipStream_t s, int nsum = 1, long long sstride = 0, const double* addn = nullptr,
+                     long long addn_bs = 0, long long addn_cs = 0, int nq = 0);
 // xl: B x [a0 a1], xr: B x [b0 b1] after the forward column pass (limbs limbs each) -> d: B x [d0 d1 d2]
-// after the inverse row pass (tensor fused between the row passes)
+// after the inverse row pass (tensor fused between the row passes); fold: d0, d1 stay in NTT form
 void launch_mult_rows(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
-                      hipStream_t s);
+                      hipStream_t s, bool fold = false);
 void launch_inv_cols(const Context& c, const double* in, long long in_ls, long long in_ps, int out_kind, void* out,
                      long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
 

@@ -12,6 +12,8 @@
 
 namespace rck {
 
+enum { kSlotLhs = 10, kSlotRhs = 11, kSlotQuad = 12, kSlotTmp = 13 };
+
 // negacyclic product of `polys` polynomial pairs a[z] * b[z or 0] -> out[z] (NTT resident)
 void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
                      long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
@@ -25,21 +27,25 @@ void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, 
     launch_inv(c, t, n, P, kOutU64, out, n, out_ps, limbs, polys, pm, s);
 }
 
-// B ciphertext products lhs[b] x rhs[b] (ciphertext strides in_ps) -> quadratic out[b] (stride out_ps)
-void mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
-                       uint64_t* out, long long out_ps, hipStream_t s) {
+// B ciphertext products lhs[b] x rhs[b] (ciphertext strides in_ps) -> quadratic out[b] (stride out_ps).
+// fold_ntt (two-pass rings): out[b] receives only d2 (coefficient form); d0, d1 stay in NTT form in the
+// returned buffer ([B][3][limbs][N] doubles, d0 at +0 and d1 at +limbs*N of each ciphertext) for a
+// keyswitch that folds them into its accumulators.
+double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
+                   uint64_t* out, long long out_ps, hipStream_t s, bool fold_ntt) {
     const long long n = c.n, P = (long long)limbs * n;
     PrimeMap pm{limbs, (int)c.L, 0};
     double* x = static_cast<double*>(c.workspace(0, (size_t)B * 4 * P * 8));
     double* d = static_cast<double*>(c.workspace(1, (size_t)B * 3 * P * 8));
     if (c.logn <= 11) {
+        RCK_REQUIRE(!fold_ntt, "the folded product needs the two-pass NTT");
         for (int b = 0; b < B; b++) {
             launch_fwd(c, kInU64, lhs + b * in_ps, n, P, x, n, P, limbs, 2, pm, s);
             launch_fwd(c, kInU64, rhs + b * in_ps, n, P, x + 2 * P, n, P, limbs, 2, pm, s);
             launch_tensor(c, x, d, limbs, pm, s);
             launch_inv(c, d, n, P, kOutU64, out + b * out_ps, n, P, limbs, 3, pm, s);
         }
-        return;
+        return d;
     }
     // forward column pass: lhs[b] -> xl[b] = x[b][0..1], rhs[b] -> xr[b] = x[B + b][0..1]
     double* xl = x;
@@ -53,7 +59,12 @@ void mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long 
             launch_fwd_cols(c, kInU64, rhs + b * in_ps, n, P, xr + b * 2 * P, n, P, limbs, 2, pm, s);
         }
     }
-    launch_mult_rows(c, xl, xr, d, limbs, B, pm, s);
+    launch_mult_rows(c, xl, xr, d, limbs, B, pm, s, fold_ntt);
+    if (fold_ntt) {
+        // inverse column pass of d2 only: d[b][2] -> out[b]
+        launch_inv_cols(c, d + 2 * P, n, 3 * P, kOutU64, out, n, out_ps, limbs, B, pm, s);
+        return d;
+    }
     // inverse column pass of every d[b][k] -> out[b][k]
     if (B == 1 || out_ps == 3 * P) {
         launch_inv_cols(c, d, n, P, kOutU64, out, n, P, limbs, 3 * B, pm, s);
@@ -61,10 +72,29 @@ void mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long 
         for (int b = 0; b < B; b++)
             launch_inv_cols(c, d + b * 3 * P, n, P, kOutU64, out + b * out_ps, n, P, limbs, 3, pm, s);
     }
+    return d;
 }
 
 void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, uint64_t* out, hipStream_t s) {
     mult_batch(c, lhs, rhs, 0, 1, limbs, out, 0, s);
+}
+
+bool mult_relin_folds(const Context& c, const KeyDev* key) { return c.logn >= 12 && key && key->aux > 0; }
+
+void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
+                      const KeyDev* key, const std::vector<uint64_t*>& outs, hipStream_t s, KsShape shape) {
+    const long long P = (long long)limbs * c.n;
+    if (mult_relin_folds(c, key)) {
+        // relinearize = (d0, d1) + keyswitch(d2): d0, d1 are added to the Q limbs of the keyswitch
+        // accumulators in NTT form, times P, so mod-down by P returns keyswitch + (d0, d1) exactly
+        uint64_t* c2 = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * P * 8));
+        const double* dn = mult_batch(c, lhs, rhs, in_ps, B, limbs, c2, P, s, true);
+        keyswitch_batch(c, c2, P, B, limbs, key, nullptr, nullptr, 0, outs, s, shape, dn, 3 * P);
+        return;
+    }
+    uint64_t* quad = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * 3 * P * 8));
+    mult_batch(c, lhs, rhs, in_ps, B, limbs, quad, 3 * P, s);
+    keyswitch_batch(c, quad + 2 * P, 3 * P, B, limbs, key, quad, quad + P, 3 * P, outs, s, shape);
 }
 
 void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t* out, hipStream_t s) {
@@ -83,7 +113,6 @@ static long long common_stride(const std::vector<P>& ptrs) {
     return st;
 }
 
-enum { kSlotLhs = 10, kSlotRhs = 11, kSlotQuad = 12, kSlotTmp = 13 };
 
 void run_op_group(Context& c, int kind, int l, const KeyDev* key, const std::vector<const uint64_t*>& as,
                   const std::vector<const uint64_t*>& bs, const std::vector<uint64_t*>& outs, hipStream_t s,
@@ -107,15 +136,13 @@ void run_op_group(Context& c, int kind, int l, const KeyDev* key, const std::vec
                 rhs = rb;
                 st = 2 * P;
             }
-            uint64_t* quad = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * 3 * P * 8));
-            mult_batch(c, lhs, rhs, st, B, l, quad, 3 * P, s);
             if (kind == RCK_OP_MULT_RELIN) {
-                keyswitch_batch(c, quad + 2 * P, 3 * P, B, l, key, quad, quad + P, 3 * P, outs, s, shape);
+                mult_relin_batch(c, lhs, rhs, st, B, l, key, outs, s, shape);
             } else {
                 uint64_t* rel = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
                 std::vector<uint64_t*> rels;
                 for (int i = 0; i < B; i++) rels.push_back(rel + i * 2 * P);
-                keyswitch_batch(c, quad + 2 * P, 3 * P, B, l, key, quad, quad + P, 3 * P, rels, s, shape);
+                mult_relin_batch(c, lhs, rhs, st, B, l, key, rels, s, shape);
                 for (int i = 0; i < B; i++) rescale_impl(c, rels[i], l, 2, outs[i], s);
             }
             break;
@@ -256,7 +283,7 @@ std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
 // keyswitch_rns_hybrid (:159-194).
 void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, KsShape shape) {
+                     hipStream_t s, KsShape shape, const double* addn, long long addn_ps) {
     RCK_REQUIRE((int)outs.size() == B, "one output per ciphertext");
     check_ptr(key, "key");
     const long long n = c.n, P = (long long)limbs * n;
@@ -274,6 +301,7 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
         levels = active;  // keyswitching.rs:170
     }
     const int naux = key->aux;
+    RCK_REQUIRE(addn == nullptr || (naux > 0 && c.logn >= 12), "folded addends need hybrid keys and the two-pass NTT");
     const int E = limbs + naux;
     const long long EN = (long long)E * n;
     const KsShape aut = ks_auto_shape(c, limbs, E, levels, B);
@@ -292,7 +320,7 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     if (naux && c.logn >= 12) {
         // inverse rows of every limb; inverse columns of the aux limbs to coefficient form; the Q-limb
         // inverse columns run fused with mod-down and the adds
-        launch_inv_rows(c, part, n, EN, E, 2 * B, pm, s, dsplit, sstride);
+        launch_inv_rows(c, part, n, EN, E, 2 * B, pm, s, dsplit, sstride, addn, addn_ps, P, limbs);
         uint64_t* accP = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * naux * n * 8));
         launch_inv_cols(c, part + (long long)limbs * n, n, EN, kOutU64, accP, n, (long long)naux * n, naux, 2 * B,
                         PrimeMap{limbs, (int)c.L, limbs}, s);

@@ -28,16 +28,21 @@ inline void check_ptr(const void* p, const char* what) {
 // negacyclic products a[z] * b[z] (b_ps == 0: the same b for every z) of `polys` polynomials
 void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
               long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
-// B products lhs[b] x rhs[b] (ciphertext stride in_ps) -> quadratic out[b] (stride out_ps)
-void mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
-                uint64_t* out, long long out_ps, hipStream_t s);
+// B products lhs[b] x rhs[b] (ciphertext stride in_ps) -> quadratic out[b] (stride out_ps).  fold_ntt:
+// out[b] gets d2 only and d0, d1 stay in NTT form in the returned buffer ([B][3][limbs][N] doubles)
+double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
+                   uint64_t* out, long long out_ps, hipStream_t s, bool fold_ntt = false);
 void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, uint64_t* out, hipStream_t s);
 void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t* out, hipStream_t s);
 // keyswitch of B polynomials (stride in_ps) with one key, adding add0/add1 (stride add_ps, nullable)
-// into outs[b] = [c0][c1]
+// into outs[b] = [c0][c1].  addn (hybrid keys, two-pass rings; nullable): NTT-form addends
+// addn[b * addn_ps + {0, limbs*N}] folded into the accumulators (times P) instead of add0/add1.
 void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, KsShape shape = {});
+                     hipStream_t s, KsShape shape = {}, const double* addn = nullptr, long long addn_ps = 0);
+// mult + relinearize of B ciphertext pairs (stride in_ps) at `limbs` limbs: outs[b] = [c0][c1]
+void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
+                      const KeyDev* key, const std::vector<uint64_t*>& outs, hipStream_t s, KsShape shape = {});
 void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const KeyDev* key, const uint64_t* add0,
                     const uint64_t* add1, uint64_t* out, hipStream_t s);
 // B same-kind operations at one level sharing a key (RCK_OP_* kinds of the C ABI): in0 as[b], in1 bs[b],
