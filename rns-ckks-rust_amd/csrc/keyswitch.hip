@@ -513,12 +513,15 @@ __device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, 
     reduce_all(v, q, qinv);
 }
 
-template <int LOGR>
+// INV (single digit range): the epilogue also runs the inverse row stages of both accumulators (after
+// adding P * fold[cb][comp] on the Q limbs, when fold is given) and stores them in coefficient row
+// order, so the accumulators go straight to the inverse column pass.
+template <int LOGR, bool INV>
 __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict__ cols, int D, int G, int B,
                                                        int nbq, int dsplit, int ntiles,
                                                        const double* __restrict__ key, int E, int j0,
                                                        double* __restrict__ part, int logn, PrimeMap pm,
-                                                       const DevPrime* __restrict__ primes) {
+                                                       const DevPrime* __restrict__ primes, MacFold fold) {
     constexpr int R = 1 << LOGR, TPS = R / 16, ROWS = 64 / TPS, PITCH = row_pitch<R>();
     // copy granule per lane: 16 bytes (1 KiB per instruction) when a row is at least 1 KiB, else 4
     constexpr int SZ = R * 8 >= 1024 ? 16 : 4, PER_ROW = R * 8 / (64 * SZ);
@@ -629,10 +632,35 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
     reduce_all(a1, q, qinv);
     double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + (long long)b * R + tau;
     double* o1 = o0 + EN;
+    if constexpr (INV) {
+        const bool fq = fold.addn != nullptr && j < fold.nq;
+        const double pmq = fq ? fold.pmodq[j] : 0.0;
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        o0[e * TPS] = a0[e];
-        o1[e * TPS] = a1[e];
+        for (int comp = 0; comp < 2; comp++) {
+            double(&acc)[16] = comp == 0 ? a0 : a1;
+            if (fq) {
+                const double* an = fold.addn + cb * fold.bs + comp * fold.cs + (long long)j * n + (long long)b * R + tau;
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[e] = reduce(acc[e] + mulmod(an[e * TPS], pmq, q, qinv), q, qinv);
+            }
+            inv_phase_b<LOGR>(acc, P.iv, N1 + b, tau, q, qinv);
+#pragma unroll
+            for (int e = 0; e < 16; e++) buf[row_lds<R>(row, 16 * tau + e)] = acc[e];
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[e] = buf[row_lds<R>(row, tau + TPS * e)];
+            inv_phase_a(acc, P.iv, N1 + b, q, qinv);
+            double* o = comp == 0 ? o0 : o1;
+#pragma unroll
+            for (int e = 0; e < 16; e++) o[TPS * e] = acc[e];
+            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // this component's reads are done before the next writes
+            asm volatile("" ::: "memory");
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            o0[e * TPS] = a0[e];
+            o1[e * TPS] = a1[e];
+        }
     }
 }
 
@@ -768,15 +796,21 @@ static void mac8_t(const Context& c, const double* cols, int D, int G, int B, in
 
 template <int LOGR>
 static void mac_dma_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
-                      int j0, double* part, PrimeMap pm, hipStream_t s) {
+                      int j0, double* part, PrimeMap pm, hipStream_t s, const MacFold* inv) {
     constexpr int ROWS = 64 / ((1 << LOGR) / 16);
     const int nrb = (1 << (c.logn - LOGR)) / ROWS;
     const int ntiles = nrb * G * dsplit;
     const int nbq = (B + 3) / 4;
     const int blocks = ((ntiles + 7) / 8) * 8 * nbq;
     ProfScope ps(s, "digit_mac");
-    k_digit_mac_dma<LOGR><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E, j0, part, c.logn,
-                                                       pm, c.d_primes);
+    if (inv) {
+        RCK_REQUIRE(dsplit == 1, "the fused inverse rows need a single digit range");
+        k_digit_mac_dma<LOGR, true><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E, j0, part,
+                                                                 c.logn, pm, c.d_primes, *inv);
+    } else {
+        k_digit_mac_dma<LOGR, false><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E, j0,
+                                                                  part, c.logn, pm, c.d_primes, MacFold{});
+    }
     RCK_CHECK_HIP(hipGetLastError());
 }
 
@@ -795,9 +829,10 @@ static void mac_t(const Context& c, const double* cols, int D, int G, int B, int
 
 template <int LOGR, int ROWS>
 static void mac_r(const Context& c, int pf, const double* cols, int D, int G, int B, int dsplit, const double* key,
-                  int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
+                  int E, int j0, double* part, PrimeMap pm, hipStream_t s, const MacFold* inv) {
+    RCK_REQUIRE(inv == nullptr || pf == 5, "the fused inverse rows need the streamed MAC");
     if (pf == 5)
-        mac_dma_t<LOGR>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+        mac_dma_t<LOGR>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s, inv);
     else if (pf == 1)
         mac_t<LOGR, ROWS, 1>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
     else if (pf == 4)
@@ -812,7 +847,8 @@ static void mac_r(const Context& c, int pf, const double* cols, int D, int G, in
 
 template <int LOGR>
 static void mac_v(const Context& c, int rows, int pf, int width, const double* cols, int D, int G, int B, int dsplit,
-                  const double* key, int E, int j0, double* part, PrimeMap pm, hipStream_t s) {
+                  const double* key, int E, int j0, double* part, PrimeMap pm, hipStream_t s, const MacFold* inv) {
+    RCK_REQUIRE(inv == nullptr || (rows <= 4 && width == 16), "the fused inverse rows need the streamed MAC");
     if (width == 8) {
         if (rows <= 8)
             mac8_t<LOGR, 8>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
@@ -821,9 +857,9 @@ static void mac_v(const Context& c, int rows, int pf, int width, const double* c
         return;
     }
     if (rows <= 4)
-        mac_r<LOGR, 4>(c, pf, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+        mac_r<LOGR, 4>(c, pf, cols, D, G, B, dsplit, key, E, j0, part, pm, s, inv);
     else
-        mac_r<LOGR, 16>(c, pf, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
+        mac_r<LOGR, 16>(c, pf, cols, D, G, B, dsplit, key, E, j0, part, pm, s, inv);
 }
 
 // Small rings: plain forward NTT of every digit for every prime, then pointwise MAC.
@@ -850,7 +886,8 @@ __global__ void k_mac_small(const double* __restrict__ dn, int D, const double* 
 }
 
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
-                      double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s) {
+                      double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s,
+                      const MacFold* inv) {
     const int E = nq + naux;
     const long long n = c.n, EN = (long long)E * n;
     PrimeMap pm{nq, (int)c.L, 0};
@@ -897,9 +934,9 @@ void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, con
             RCK_CHECK_HIP(hipStreamWaitEvent(s, cc.event(3 + slot), 0));
         }
         switch (n2) {
-            case 6: mac_v<6>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
-            case 7: mac_v<7>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
-            case 8: mac_v<8>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s); break;
+            case 6: mac_v<6>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s, inv); break;
+            case 7: mac_v<7>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s, inv); break;
+            case 8: mac_v<8>(c, rows, prefetch, width, cols, D, g, B, dsplit, key, E, j0, part, pm, s, inv); break;
             default: RCK_REQUIRE(false, "unsupported ring degree");
         }
         if (ngroups > 1) RCK_CHECK_HIP(hipEventRecord(cc.event(1 + slot), s));

@@ -43,8 +43,18 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 // issued before the second radix-16 phase, 5 four ciphertexts per workgroup with the next digit's
 // rows copied into LDS while the current one is transformed; rows 4 only).
 // width: elements per thread in the MAC's row pass (16, or 8: half the registers, more resident waves).
+// inv (load schedule 5, rows 4, one digit range only): the MAC's epilogue also runs the inverse row
+// stages of the accumulators, first adding P * addn on the Q limbs when inv->addn is set; part is then
+// in coefficient row order, ready for the inverse column pass.
+struct MacFold {
+    const double* addn = nullptr;  // NTT-form addends: addn + b * bs + comp * cs + limb * N
+    long long bs = 0, cs = 0;
+    int nq = 0;
+    const double* pmodq = nullptr;  // centred P mod q_i
+};
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
-                      double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s);
+                      double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s,
+                      const MacFold* inv = nullptr);
 // mod_down_eval_to_q (keyswitching.rs:278-322) on coefficient-form accumulators over Q-prefix
 // nq || P (2 polys, u64), optionally adding add0 / add1 (nq limbs each) to the two results.
 void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64_t* add0, const uint64_t* add1,

@@ -313,14 +313,17 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     double* part = static_cast<double*>(c.workspace(6, (size_t)dsplit * B * 2 * EN * 8));
     launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps);
     const int width = shape.group > 0 || shape.rows > 0 ? shape.width : aut.width;
+    // the streamed MAC with one digit range also runs the accumulators' inverse row stages (and the fold)
+    const bool fuse_inv = naux && c.logn >= 12 && prefetch == 5 && rows == 4 && dsplit == 1;
+    MacFold mf{addn, addn_ps, P, limbs, c.d_pmodq};
     launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, rows, prefetch,
-                     rows == 4 ? 16 : width, s);
+                     rows == 4 ? 16 : width, s, fuse_inv ? &mf : nullptr);
     PrimeMap pm{limbs, (int)c.L, 0};
     const long long sstride = (long long)B * 2 * EN;
     if (naux && c.logn >= 12) {
         // inverse rows of every limb; inverse columns of the aux limbs to coefficient form; the Q-limb
         // inverse columns run fused with mod-down and the adds
-        launch_inv_rows(c, part, n, EN, E, 2 * B, pm, s, dsplit, sstride, addn, addn_ps, P, limbs);
+        if (!fuse_inv) launch_inv_rows(c, part, n, EN, E, 2 * B, pm, s, dsplit, sstride, addn, addn_ps, P, limbs);
         uint64_t* accP = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * naux * n * 8));
         launch_inv_cols(c, part + (long long)limbs * n, n, EN, kOutU64, accP, n, (long long)naux * n, naux, 2 * B,
                         PrimeMap{limbs, (int)c.L, limbs}, s);
