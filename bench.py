@@ -46,10 +46,12 @@ def time_it(G, fn, steps, stream=None):
     return e0.elapsed_ms(e1)
 
 
-def mac_bytes(n, D, G_, B, dsplit):
+def mac_bytes(n, D, G_, B, dsplit, fold_limbs=0):
     """Algorithmic HBM bytes of one digit-MAC launch: the column-pass output of B*D digits for its
-    G_ eval limbs, the key rows (2 per digit per limb, read once) and the partial sums written."""
-    return (B * D * G_ + 2 * D * G_ + dsplit * B * 2 * G_) * n * 8
+    G_ eval limbs, the key rows (2 per digit per limb, read once), the partial sums written and, when
+    the relinearization terms are folded into the accumulators, their NTT-form rows (2 per ciphertext
+    for each of the launch's fold_limbs Q limbs)."""
+    return (B * D * G_ + 2 * D * G_ + dsplit * B * 2 * G_ + 2 * B * fold_limbs) * n * 8
 
 
 def measured_traffic(mode, n, L, K, D, B):
@@ -63,6 +65,22 @@ def measured_traffic(mode, n, L, K, D, B):
     if not rec or rec.get("workload") != f"N=2^{n.bit_length() - 1} L={L} K={K} D={D} B={B}":
         return None
     return rec["fetch_bytes"] + rec["write_bytes"]
+
+
+def brief(rec, keep=4):
+    """A tuning report with each candidate list cut to its `keep` fastest entries (and their count)."""
+    if not isinstance(rec, dict):
+        return rec
+    out = {}
+    for k, v in rec.items():
+        if k == "candidates" and isinstance(v, list):
+            out["candidates_timed"] = len(v)
+            out[k] = sorted(v, key=lambda c: c.get("ms", 0))[:keep]
+        elif isinstance(v, list):
+            out[k] = [brief(x, keep) for x in v]
+        else:
+            out[k] = v
+    return out
 
 
 def per_op_entry(table, n, limbs, E, D):
@@ -226,7 +244,10 @@ def main():
         ds = report["groups"][0]["digit_splits"] if report else tune1["digit_splits"]
         if ds is None:  # automatic shape without tuning: the library's heuristic for one ciphertext
             ds = max(1, min(4, -(-2048 // (64 * (L + K) * B))))
-        alg = mac_bytes(n, D, G_, B, ds)
+        # the streamed MAC (load schedule 5) with one digit range reads the folded d0, d1 rows of a
+        # batched mult+relinearize (per-op mode calls relinearize on a coefficient-form quad: no fold)
+        folded = report is not None and report["groups"][0]["load_schedule"] == 5 and ds == 1
+        alg = int(mac_bytes(n, D, G_, B, ds, L / launches if folded else 0))
         achieved = alg / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -246,7 +267,7 @@ def main():
                                + (f", {B} ciphertext pairs per step sharing one relin key" if B > 1 else ""),
                    "mode": args.mode, "batch": B, "per_op_ops_s": round(per_op_value, 2),
                    "batched_vs_per_op": round(value / per_op_value, 3)},
-        "roofline": roofline, "cpu_baseline": cpu, "tuning": {"per_op": tune1, "batched": report},
+        "roofline": roofline, "cpu_baseline": cpu, "tuning": {"per_op": brief(tune1), "batched": brief(report)},
         "kernels_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in top},
     }
     print(json.dumps(line), flush=True)
