@@ -244,9 +244,9 @@ def main():
         ds = report["groups"][0]["digit_splits"] if report else tune1["digit_splits"]
         if ds is None:  # automatic shape without tuning: the library's heuristic for one ciphertext
             ds = max(1, min(4, -(-2048 // (64 * (L + K) * B))))
-        # the streamed MAC (load schedule 5) with one digit range reads the folded d0, d1 rows of a
+        # the streamed MAC (load schedule 5 or 6) with one digit range reads the folded d0, d1 rows of a
         # batched mult+relinearize (per-op mode calls relinearize on a coefficient-form quad: no fold)
-        folded = report is not None and report["groups"][0]["load_schedule"] == 5 and ds == 1
+        folded = report is not None and report["groups"][0]["load_schedule"] in (5, 6) and ds == 1
         alg = int(mac_bytes(n, D, G_, B, ds, L / launches if folded else 0))
         achieved = alg / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

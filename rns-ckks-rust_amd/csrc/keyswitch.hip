@@ -516,7 +516,10 @@ __device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, 
 // INV (single digit range): the epilogue also runs the inverse row stages of both accumulators (after
 // adding P * fold[cb][comp] on the Q limbs, when fold is given) and stores them in coefficient row
 // order, so the accumulators go straight to the inverse column pass.
-template <int LOGR, bool INV>
+// SPLIT: the four waves take one ciphertext and four consecutive parts of its digit range (for small
+// batches: four times the waves, the staged twiddles still shared); the four partial accumulators
+// are summed through LDS and waves 0 and 1 finish components 0 and 1.
+template <int LOGR, bool INV, bool SPLIT>
 __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict__ cols, int D, int G, int B,
                                                        int nbq, int dsplit, int ntiles,
                                                        const double* __restrict__ key, int E, int j0,
@@ -534,14 +537,15 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
     if (tile >= ntiles) return;
     const int rb = tile % nrb, rest = tile / nrb, jl = rest % G, sp = rest / G;
     const int j = j0 + jl;
-    const int i0 = (int)((long long)D * sp / dsplit), i1 = (int)((long long)D * (sp + 1) / dsplit);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r0 = (int)((long long)D * sp / dsplit), r1 = (int)((long long)D * (sp + 1) / dsplit);
+    const int i0 = SPLIT ? r0 + (r1 - r0) * wave / 4 : r0, i1 = SPLIT ? r0 + (r1 - r0) * (wave + 1) / 4 : r1;
     const int tau = lane % TPS, row = lane / TPS;
     const int b0 = rb * ROWS, b = b0 + row;
     const DevPrime P = primes[pm(j)];
     const double q = P.q, qinv = P.qinv;
-    const int cb = cq * 4 + wave;
-    const bool active = cb < B;
+    const int cb = SPLIT ? cq : cq * 4 + wave;
+    const bool active = cb < B && i0 < i1;
     const long long EN = (long long)E * n;
     const long long col_stride = (long long)G * n;
     double* buf = bufl[wave];
@@ -581,7 +585,7 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
         twl[t] = tw_load<kTwGlobal>(P.fw, idx);
     }
     __syncthreads();
-    if (!active) return;
+    if (!SPLIT && !active) return;
     const double* twa = twl + row;
     const double* twb = twl + 15 * ROWS + lane;
     double a0[16], a1[16];
@@ -631,14 +635,12 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
     reduce_all(a0, q, qinv);
     reduce_all(a1, q, qinv);
     double* o0 = part + (((long long)sp * B + cb) * 2) * EN + (long long)j * n + (long long)b * R + tau;
-    double* o1 = o0 + EN;
-    if constexpr (INV) {
-        const bool fq = fold.addn != nullptr && j < fold.nq;
-        const double pmq = fq ? fold.pmodq[j] : 0.0;
-#pragma unroll
-        for (int comp = 0; comp < 2; comp++) {
-            double(&acc)[16] = comp == 0 ? a0 : a1;
-            if (fq) {
+    // one component's finish: optional fold + inverse row stages (INV), then the store
+    auto finish = [&](int comp, double(&acc)[16]) {
+        double* o = o0 + comp * EN;
+        if constexpr (INV) {
+            if (fold.addn != nullptr && j < fold.nq) {
+                const double pmq = fold.pmodq[j];
                 const double* an = fold.addn + cb * fold.bs + comp * fold.cs + (long long)j * n + (long long)b * R + tau;
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[e] = reduce(acc[e] + mulmod(an[e * TPS], pmq, q, qinv), q, qinv);
@@ -649,18 +651,38 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[e] = buf[row_lds<R>(row, tau + TPS * e)];
             inv_phase_a(acc, P.iv, N1 + b, q, qinv);
-            double* o = comp == 0 ? o0 : o1;
 #pragma unroll
             for (int e = 0; e < 16; e++) o[TPS * e] = acc[e];
-            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // this component's reads are done before the next writes
+            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // the buffer's reads are done before it is reused
             asm volatile("" ::: "memory");
-        }
-    } else {
+        } else {
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            o0[e * TPS] = a0[e];
-            o1[e * TPS] = a1[e];
+            for (int e = 0; e < 16; e++) o[e * TPS] = acc[e];
         }
+    };
+    if constexpr (SPLIT) {
+        // sum the four waves' partial accumulators: component k goes through the buffers, wave k sums it
+        // (|sum| <= 4 * 0.51q), then waves 0 and 1 finish their component in their own buffer
+#pragma unroll
+        for (int comp = 0; comp < 2; comp++) {
+            double(&acc)[16] = comp == 0 ? a0 : a1;
+#pragma unroll
+            for (int e = 0; e < 16; e++) buf[e * 64 + lane] = acc[e];
+            __syncthreads();
+            if (wave == comp) {
+#pragma unroll
+                for (int e = 0; e < 16; e++)
+                    acc[e] = reduce(bufl[0][e * 64 + lane] + bufl[1][e * 64 + lane] + bufl[2][e * 64 + lane] +
+                                        bufl[3][e * 64 + lane],
+                                    q, qinv);
+            }
+            __syncthreads();
+        }
+        if (wave == 0) finish(0, a0);
+        if (wave == 1) finish(1, a1);
+    } else {
+        finish(0, a0);
+        finish(1, a1);
     }
 }
 
@@ -794,22 +816,22 @@ static void mac8_t(const Context& c, const double* cols, int D, int G, int B, in
     RCK_CHECK_HIP(hipGetLastError());
 }
 
-template <int LOGR>
+template <int LOGR, bool SPLIT>
 static void mac_dma_t(const Context& c, const double* cols, int D, int G, int B, int dsplit, const double* key, int E,
                       int j0, double* part, PrimeMap pm, hipStream_t s, const MacFold* inv) {
     constexpr int ROWS = 64 / ((1 << LOGR) / 16);
     const int nrb = (1 << (c.logn - LOGR)) / ROWS;
     const int ntiles = nrb * G * dsplit;
-    const int nbq = (B + 3) / 4;
+    const int nbq = SPLIT ? B : (B + 3) / 4;
     const int blocks = ((ntiles + 7) / 8) * 8 * nbq;
     ProfScope ps(s, "digit_mac");
     if (inv) {
         RCK_REQUIRE(dsplit == 1, "the fused inverse rows need a single digit range");
-        k_digit_mac_dma<LOGR, true><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E, j0, part,
-                                                                 c.logn, pm, c.d_primes, *inv);
+        k_digit_mac_dma<LOGR, true, SPLIT><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E, j0,
+                                                                        part, c.logn, pm, c.d_primes, *inv);
     } else {
-        k_digit_mac_dma<LOGR, false><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E, j0,
-                                                                  part, c.logn, pm, c.d_primes, MacFold{});
+        k_digit_mac_dma<LOGR, false, SPLIT><<<dim3(blocks), 256, 0, s>>>(cols, D, G, B, nbq, dsplit, ntiles, key, E,
+                                                                         j0, part, c.logn, pm, c.d_primes, MacFold{});
     }
     RCK_CHECK_HIP(hipGetLastError());
 }
@@ -830,9 +852,11 @@ static void mac_t(const Context& c, const double* cols, int D, int G, int B, int
 template <int LOGR, int ROWS>
 static void mac_r(const Context& c, int pf, const double* cols, int D, int G, int B, int dsplit, const double* key,
                   int E, int j0, double* part, PrimeMap pm, hipStream_t s, const MacFold* inv) {
-    RCK_REQUIRE(inv == nullptr || pf == 5, "the fused inverse rows need the streamed MAC");
+    RCK_REQUIRE(inv == nullptr || pf == 5 || pf == 6, "the fused inverse rows need the streamed MAC");
     if (pf == 5)
-        mac_dma_t<LOGR>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s, inv);
+        mac_dma_t<LOGR, false>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s, inv);
+    else if (pf == 6)
+        mac_dma_t<LOGR, true>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s, inv);
     else if (pf == 1)
         mac_t<LOGR, ROWS, 1>(c, cols, D, G, B, dsplit, key, E, j0, part, pm, s);
     else if (pf == 4)

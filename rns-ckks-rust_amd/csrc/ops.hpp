@@ -41,9 +41,10 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 // rows: ring rows per MAC workgroup (4 or 16; 8 or 16 with width 8); prefetch: load schedule (0 late
 // keys, 1 early keys, 2 next digit's rows prefetched, 3 keys in groups of four positions, 4 keys
 // issued before the second radix-16 phase, 5 four ciphertexts per workgroup with the next digit's
-// rows copied into LDS while the current one is transformed; rows 4 only).
+// rows copied into LDS while the current one is transformed, 6 the same with four digit ranges of
+// one ciphertext per workgroup; 5 and 6 with rows 4 only).
 // width: elements per thread in the MAC's row pass (16, or 8: half the registers, more resident waves).
-// inv (load schedule 5, rows 4, one digit range only): the MAC's epilogue also runs the inverse row
+// inv (load schedule 5 or 6, rows 4, one digit range only): the MAC's epilogue also runs the inverse row
 // stages of the accumulators, first adding P * addn on the Q limbs when inv->addn is set; part is then
 // in coefficient row order, ready for the inverse column pass.
 struct MacFold {

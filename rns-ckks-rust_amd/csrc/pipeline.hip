@@ -221,7 +221,7 @@ int ks_tuned_import(const std::string& text) {
                                     &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &ms, &extra);
         RCK_REQUIRE(got == 11, "malformed tuning table line: " + line);
         RCK_REQUIRE(v[1] >= 1 && v[2] >= v[1] && v[3] >= 1 && v[4] >= 1, "tuning table entry out of range: " + line);
-        const bool w16 = v[9] == 16 && (v[7] == 4 || v[7] == 16) && v[8] >= 0 && (v[8] <= 4 || (v[8] == 5 && v[7] == 4));
+        const bool w16 = v[9] == 16 && (v[7] == 4 || v[7] == 16) && v[8] >= 0 && (v[8] <= 4 || ((v[8] == 5 || v[8] == 6) && v[7] == 4));
         const bool w8 = v[9] == 8 && (v[7] == 8 || v[7] == 16) && v[8] == 0;
         RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (w16 || w8),
                     "tuning table launch shape out of range: " + line);
@@ -271,6 +271,7 @@ std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
             for (int rows : {4, 16})
                 for (int pf : {0, 1, 2, 3, 4}) out.push_back({g, ds, rows, pf, 16});
             if (B >= 2) out.push_back({g, ds, 4, 5, 16});  // four ciphertexts per workgroup, LDS-streamed rows
+            out.push_back({g, ds, 4, 6, 16});              // four digit ranges of one ciphertext per workgroup
             if (g == E && ds <= 2)
                 for (int rows : {8, 16}) out.push_back({g, ds, rows, 0, 8});
         }
@@ -314,7 +315,7 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps);
     const int width = shape.group > 0 || shape.rows > 0 ? shape.width : aut.width;
     // the streamed MAC with one digit range also runs the accumulators' inverse row stages (and the fold)
-    const bool fuse_inv = naux && c.logn >= 12 && prefetch == 5 && rows == 4 && dsplit == 1;
+    const bool fuse_inv = naux && c.logn >= 12 && (prefetch == 5 || prefetch == 6) && rows == 4 && dsplit == 1;
     MacFold mf{addn, addn_ps, P, limbs, c.d_pmodq};
     launch_digit_mac(c, digits, B, levels, key->data, limbs, naux, part, dsplit, group, rows, prefetch,
                      rows == 4 ? 16 : width, s, fuse_inv ? &mf : nullptr);

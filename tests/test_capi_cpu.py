@@ -69,7 +69,7 @@ def test_tune_table_round_trip_and_validation():
     for line in rows.splitlines():
         assert line in table.splitlines()
     for bad in ("9 5 7 9 3 7 2 8 1 16 1.0\n", "9 5 7 9 3 8 2 16 1 16 1.0\n", "9 5 7 9\n",
-                "9 5 7 9 3 7 2 4 0 8 1.0\n", "9 5 7 9 3 7 2 16 1 1.0\n", "9 5 7 9 3 7 2 16 5 16 1.0\n", "9 5 7 9 3 7 2 16 6 16 1.0\n"):
+                "9 5 7 9 3 7 2 4 0 8 1.0\n", "9 5 7 9 3 7 2 16 1 1.0\n", "9 5 7 9 3 7 2 16 5 16 1.0\n", "9 5 7 9 3 7 2 16 6 16 1.0\n", "9 5 7 9 3 7 2 4 7 16 1.0\n"):
         with pytest.raises(G.RckError):
             G.tune_table_import(bad)
         assert G.tune_table_export() == table

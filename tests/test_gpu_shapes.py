@@ -22,6 +22,7 @@ SHAPES = [  # group, dsplit, rows, prefetch, width
     (None, 1, 16, 0, 16), (None, 2, 4, 1, 16), (None, 1, 4, 2, 16), (None, 1, 16, 3, 16), (None, 1, 8, 0, 8),
     (None, 2, 16, 0, 8), (4, 1, 8, 0, 8), (1, 1, 16, 3, 16), (None, 1, 4, 4, 16), (None, 2, 16, 4, 16),
     (None, 1, 4, 5, 16), (None, 2, 4, 5, 16), (3, 1, 4, 5, 16),
+    (None, 1, 4, 6, 16), (None, 2, 4, 6, 16), (2, 1, 4, 6, 16),
 ]
 
 
@@ -49,9 +50,10 @@ def test_mac_shapes_bit_identical(logn, limbs):
 
 @pytest.mark.parametrize("logn,limbs", [(12, 4), (13, 5), (16, 6)])
 def test_mac_streamed_batch_bit_identical(logn, limbs):
-    """The four-ciphertexts-per-workgroup MAC (rows copied into LDS while the previous digit is
-    transformed) on a batch of 5 mult+relin (one workgroup with a single active wave) gives the
-    per-op results."""
+    """The streamed MACs (rows copied into LDS while the previous digit is transformed): four
+    ciphertexts per workgroup (load schedule 5; a batch of 5 leaves one workgroup with a single
+    active wave) and four digit ranges of one ciphertext per workgroup (6), with one or two digit
+    splits, on a batch of 5 mult+relin give the per-op results."""
     n, B = 1 << logn, 5
     p = PR.config_batched(n=1 << 16, limbs=limbs, aux=2)
     ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
@@ -70,14 +72,14 @@ def test_mac_streamed_batch_bit_identical(logn, limbs):
         ctx.mult(lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), limbs, 40, 40, quad)
         ctx.relinearize(quad, limbs, 80, k, one)
         want.append(one.numpy())
-    for ds in (1, 2):
-        G.tune_table_import(f"{logn} {limbs} {E} {D} {B} {E} {ds} 4 5 16 1.0\n")
+    for ds, pf in ((1, 5), (2, 5), (1, 6), (2, 6)):
+        G.tune_table_import(f"{logn} {limbs} {E} {D} {B} {E} {ds} 4 {pf} 16 1.0\n")
         out = G.DeviceBuffer((B, 2, limbs, n))
         bt = G.Batch(ctx)
         for b in range(B):
             bt.add(G.OP_MULT_RELIN, limbs, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), k, out.offset(b * 2 * P))
         bt.compile(tune=True)
-        assert bt.report()["groups"][0]["load_schedule"] == 5
+        assert bt.report()["groups"][0]["load_schedule"] == pf
         got = out.numpy()
         for b in range(B):
-            assert np.array_equal(got[b], want[b]), (ds, b)
+            assert np.array_equal(got[b], want[b]), (ds, pf, b)
