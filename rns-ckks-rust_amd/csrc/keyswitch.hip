@@ -32,9 +32,8 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
                                                       const double* __restrict__ invq, int T,
                                                       const uint32_t* __restrict__ qw, const uint32_t* __restrict__ hw,
                                                       int32_t* __restrict__ digits) {
-    // LDS: [W + 1][kDecBS] words of X, preceded (NQM == 0) by [nq][kDecBS] Garner digits
+    // LDS (NQM == 0 only): [nq][kDecBS] Garner digits
     extern __shared__ double lds_m[];
-    uint32_t* xs = reinterpret_cast<uint32_t*>(lds_m + (NQM == 0 ? nq * kDecBS : 0));
     const int tid = threadIdx.x;
     const long long c = (long long)blockIdx.x * kDecBS + tid;
     if (c >= n) return;
@@ -129,25 +128,27 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
             borrow = (t >> 32) & 1;
         }
     }
-    // balanced digits of X (two's complement, sign-extended over W words).  X is parked in this
-    // thread's LDS column so the digit loop can index its words without a private stack array.
-#pragma unroll
-    for (int k = 0; k < W; k++) xs[k * kDecBS + tid] = x[k];
-    xs[W * kDecBS + tid] = (x[W - 1] >> 31) ? 0xffffffffu : 0u;
+    // balanced digits of X (two's complement, sign-extended past word W - 1).  The digit loop is
+    // unrolled over the most digits W words can hold, so every word index is a constant and X stays
+    // in registers; `levels` (uniform) guards the tail.
+    const uint32_t sx = (x[W - 1] >> 31) ? 0xffffffffu : 0u;
     int carry = 0;
-    for (int i = 0; i < levels; i++) {
-        const int bit = 30 * i, w = bit >> 5, sh = bit & 31;
-        const uint64_t lo = xs[w * kDecBS + tid];
-        const uint64_t hi = xs[min(w + 1, W) * kDecBS + tid];
-        const int raw = (int)(((lo | (hi << 32)) >> sh) & 0x3fffffffu);
-        int d = raw + carry;
-        if (d >= (1 << 29)) {
-            d -= (1 << 30);
-            carry = 1;
-        } else {
-            carry = 0;
+#pragma unroll
+    for (int i = 0; i < (32 * W) / 30 + 1; i++) {
+        if (i < levels) {
+            const int bit = 30 * i, w = bit >> 5, sh = bit & 31;
+            const uint64_t lo = w < W ? x[w] : sx;
+            const uint64_t hi = w + 1 < W ? x[w + 1] : sx;
+            const int raw = (int)(((lo | (hi << 32)) >> sh) & 0x3fffffffu);
+            int d = raw + carry;
+            if (d >= (1 << 29)) {
+                d -= (1 << 30);
+                carry = 1;
+            } else {
+                carry = 0;
+            }
+            digits[(long long)i * n + c] = d;
         }
-        digits[(long long)i * n + c] = d;
     }
 }
 
@@ -156,7 +157,7 @@ static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, i
                    long long ps) {
     const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords;
     ProfScope prof(s, "decompose");
-    const size_t lds = (size_t)(W + 1) * kDecBS * 4 + (NQM == 0 ? (size_t)nq * kDecBS * 8 : 0);
+    const size_t lds = NQM == 0 ? (size_t)nq * kDecBS * 8 : 0;
     k_decompose<W, NQM><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, lds, s>>>(
         poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords, digits);
     RCK_CHECK_HIP(hipGetLastError());

@@ -139,3 +139,46 @@ def test_batch_rejects_bad_ops():
     bt.add(G.OP_ROTATE, 4, a, None, k, G.DeviceBuffer((2, 4, n)))
     with pytest.raises(G.RckError, match="rotation key level must match the ciphertext level"):
         bt.compile()
+
+
+def test_headline_workload_batched_matches_per_op_and_oracle():
+    """The headline workload itself: 16 mult+relinearize at N=2^16, L=24 primes below 2^50, K=2,
+    D=40, sharing one hybrid key, through the batch executor with both streamed MAC shapes (the
+    relinearization terms folded into the accumulators, inverse rows in the MAC epilogue): every
+    ciphertext bit-exact against the per-op calls, the first against the oracle."""
+    p = PR.config_per_op_gpu()
+    n, L, K, B = p.poly_degree, len(p.primes), len(p.aux_primes), 16
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    rs = np.random.default_rng(2024)
+    D = ctx.num_decomp_digits(L)
+    assert D == 40
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    k = ctx.ksk_create(key, L - 1, K)
+    P = L * n
+    lhs_h = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
+    rhs_h = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
+    lhs, rhs = dev(lhs_h), dev(rhs_h)
+    quad = G.DeviceBuffer((3, L, n))
+    one = G.DeviceBuffer((2, L, n))
+    want = []
+    for b in range(B):
+        ctx.mult(lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), L, 40, 40, quad)
+        ctx.relinearize(quad, L, 80, k, one)
+        want.append(one.numpy())
+    qb = O.Basis(n, p.primes, p.roots)
+    eb = O.Basis(n, p.primes + p.aux_primes, p.roots + p.aux_roots)
+    O.set_threads(16)
+    assert np.array_equal(want[0], O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs_h[0], rhs_h[0]), key))
+    for pf in (5, 6):
+        G.tune_table_import(f"16 {L} {L + K} {D} {B} {L + K} 1 4 {pf} 16 1.0\n")
+        out = G.DeviceBuffer((B, 2, L, n))
+        bt = G.Batch(ctx)
+        for b in range(B):
+            bt.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), k, out.offset(b * 2 * P))
+        bt.compile(tune=True)
+        assert bt.report()["groups"][0]["load_schedule"] == pf
+        bt.run()
+        bt.synchronize()
+        got = out.numpy()
+        for b in range(B):
+            assert np.array_equal(got[b], want[b]), (pf, b)
