@@ -237,22 +237,17 @@ KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B) {
     KsShape r;
     if (ks_tuned_find(c, limbs, E, D, B, &r, nullptr)) return r;
     r.group = c.tune_group > 0 ? c.tune_group : E;
-    int n1, n2;
-    split_logn(c.logn, &n1, &n2);
-    r.rows = (1 << n2) / 16 >= 16 ? 4 : 16;  // at least one full wave per MAC workgroup
     r.prefetch = 0;
     if (c.logn <= 11) {
+        r.rows = 16;
         r.dsplit = 1;
         return r;
     }
-    if (c.tune_dsplit > 0) {
-        r.dsplit = std::min(c.tune_dsplit, std::min(8, D));
-        return r;
-    }
-    const int nrb = (1 << n1) / r.rows;
-    const long long tiles = (long long)nrb * E * B;
-    const int ds = (int)((2048 + tiles - 1) / tiles);
-    r.dsplit = std::max(1, std::min(ds, std::min(4, D)));
+    // the streamed MAC: four ciphertexts per workgroup from four up, four digit ranges of one
+    // ciphertext below; one digit range, so its epilogue also runs the inverse rows (and the fold)
+    r.rows = 4;
+    r.prefetch = B >= 4 ? 5 : 6;
+    r.dsplit = c.tune_dsplit > 0 ? std::min(c.tune_dsplit, std::min(8, D)) : 1;
     return r;
 }
 
