@@ -440,11 +440,7 @@ int rck_rotate(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t sc
         check_ptr(ct, "ct");
         check_ptr(out, "out");
         RCK_REQUIRE((int)key->level == (int)limbs - 1, "rotation key level must match the ciphertext level");
-        const long long n = c.n, P = (long long)limbs * n;
-        uint64_t* rot = static_cast<uint64_t*>(c.workspace(3, (size_t)2 * P * 8));
-        launch_galois(c, {ct, n, P}, {rot, n, P}, limbs, 2, key->galois, PrimeMap{(int)limbs, (int)c.L, 0},
-                      S(stream));
-        keyswitch_impl(c, rot + P, limbs, key, rot, nullptr, out, S(stream));
+        rotate_batch(c, {ct}, (int)limbs, key, {out}, S(stream));
         if (out_scale) *out_scale = scale;
     });
 }

@@ -25,13 +25,15 @@ constexpr int kDecBS = 64;
 // NQM (>= nq, a multiple of 8): bound of the limb count; the Garner digits m_0 .. m_{nq-1} live in
 // registers (loops fully unrolled over NQM with limb guards), so the workgroup's LDS holds only the
 // parked words of X and more waves stay resident.  NQM = 0: the digits live in LDS (any nq).
+// ginv != 0: the input is read through the automorphism X -> X^g (g * ginv = 1 mod 2N), i.e. the
+// decomposition of sigma_g(poly): coefficient c is (+/-) poly[c * ginv mod 2N] (as k_galois).
 template <int W, int NQM>
 __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict__ poly, long long ps, int n, int nq,
                                                       int levels,
                                                       const DevPrime* __restrict__ primes,
                                                       const double* __restrict__ invq, int T,
                                                       const uint32_t* __restrict__ qw, const uint32_t* __restrict__ hw,
-                                                      int32_t* __restrict__ digits) {
+                                                      int32_t* __restrict__ digits, unsigned long long ginv) {
     // LDS (NQM == 0 only): [nq][kDecBS] Garner digits
     extern __shared__ double lds_m[];
     const int tid = threadIdx.x;
@@ -39,6 +41,18 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     if (c >= n) return;
     poly += (long long)blockIdx.y * ps;
     digits += (long long)blockIdx.y * levels * n;
+    long long src = c;
+    bool neg = false;
+    if (ginv) {
+        const unsigned long long k = ((unsigned long long)c * ginv) & (2ull * n - 1);
+        neg = k >= (unsigned long long)n;
+        src = neg ? (long long)k - n : (long long)k;
+    }
+    // residue of limb i of the (rotated) coefficient, as a double in [0, q_i)
+    auto rd = [&](int i, double q) -> double {
+        const uint64_t x = poly[(long long)i * n + src];
+        return neg && x != 0 ? q - (double)x : (double)x;
+    };
     // Garner (row order as in the reference: m_i from r_i and m_0 .. m_{i-1})
     double m[NQM > 0 ? NQM : 1];
     if constexpr (NQM > 0) {
@@ -47,7 +61,7 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
             m[i] = 0.0;
             if (i < nq) {
                 const double q = primes[i].q, qinv = primes[i].qinv;
-                double v = (double)poly[(long long)i * n + c];
+                double v = rd(i, q);
 #pragma unroll
                 for (int j = 0; j < i; j++) {
                     double a = reduce(v - m[j], q, qinv);
@@ -59,7 +73,7 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     } else {
         for (int i = 0; i < nq; i++) {
             const double q = primes[i].q, qinv = primes[i].qinv;
-            double v = (double)poly[(long long)i * n + c];
+            double v = rd(i, q);
             for (int j = 0; j < i; j++) {
                 double a = reduce(v - lds_m[j * kDecBS + tid], q, qinv);
                 v = mulmod(a, invq[(long long)i * T + j], q, qinv);
@@ -154,12 +168,12 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
 
 template <int W, int NQM>
 static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
-                   long long ps) {
+                   long long ps, unsigned long long ginv) {
     const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords;
     ProfScope prof(s, "decompose");
     const size_t lds = NQM == 0 ? (size_t)nq * kDecBS * 8 : 0;
     k_decompose<W, NQM><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, lds, s>>>(
-        poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords, digits);
+        poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords, digits, ginv);
     RCK_CHECK_HIP(hipGetLastError());
 }
 
@@ -167,35 +181,35 @@ static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, i
 // larger limb counts keep the digits in LDS (NQM = 0).
 template <int W>
 static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
-                  long long ps) {
+                  long long ps, unsigned long long ginv) {
     if (nq <= 8)
-        dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps);
+        dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (nq <= 16)
-        dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps);
+        dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (nq <= 24)
-        dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps);
+        dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else
-        dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps);
+        dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv);
 }
 
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
-                      long long ps) {
+                      long long ps, unsigned long long ginv) {
     RCK_REQUIRE(nq >= 1 && nq <= (int)c.L, "bad limb count for decomposition");
     const int need = std::max((int)(c.qbits[nq] + 32) / 32 + 1, (30 * levels + 31) / 32 + 1);
     if (need <= 8)
-        dec_t<8>(c, poly, nq, levels, digits, s, B, ps);
+        dec_t<8>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 16)
-        dec_t<16>(c, poly, nq, levels, digits, s, B, ps);
+        dec_t<16>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 24)
-        dec_t<24>(c, poly, nq, levels, digits, s, B, ps);
+        dec_t<24>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 32)
-        dec_t<32>(c, poly, nq, levels, digits, s, B, ps);
+        dec_t<32>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 40)
-        dec_t<40>(c, poly, nq, levels, digits, s, B, ps);
+        dec_t<40>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 48)
-        dec_t<48>(c, poly, nq, levels, digits, s, B, ps);
+        dec_t<48>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 64)
-        dec_t<64>(c, poly, nq, levels, digits, s, B, ps);
+        dec_t<64>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else
         RCK_REQUIRE(false, "decomposition needs more than 2048 bits");
 }
@@ -1135,11 +1149,13 @@ __global__ void k_p_garner(const uint64_t* __restrict__ accP, double* __restrict
     o[(long long)K * n] = gt ? 1.0 : 0.0;
 }
 
+// ginv != 0: add0 is read through the automorphism (the rotated c0 of a rotation, as k_galois).
 template <int LOGR, int KT>
 __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ acc, long long EN, int nq,
                                                      const double* __restrict__ pg, int L,
                                                      const uint64_t* __restrict__ add0, const uint64_t* __restrict__ add1,
-                                                     long long add_ps, OutPtrs outs, long long P, int logn,
+                                                     long long add_ps, unsigned long long ginv, OutPtrs outs,
+                                                     long long P, int logn,
                                                      const DevPrime* __restrict__ primes,
                                                      const double* __restrict__ pmodq,
                                                      const double* __restrict__ pinvq,
@@ -1148,9 +1164,22 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
     __shared__ double lds[R * S];
     const int n = 1 << logn, N2 = n >> LOGR, ncb = N2 / S;
     // XCD-aware order: the nq limb workgroups of one column block run back to back on one XCD
-    // (workgroup w -> XCD w % 8), so that XCD's L2 serves the block's aux-limb values to all of them
-    const int w = blockIdx.x, k = w >> 3, i = k % nq, cbk = (k / nq) * 8 + (w & 7);
-    if (cbk >= ncb) return;
+    // (workgroup w -> XCD w % 8), so that XCD's L2 serves the block's aux-limb values to all of them.
+    // With fewer than 8 column blocks (N <= 2^14) each block is spread over 8 / ncb XCDs, each taking
+    // every (8 / ncb)-th limb, so that every XCD has work.
+    const int w = blockIdx.x;
+    int i, cbk;
+    if (ncb >= 8) {
+        const int k = w >> 3;
+        i = k % nq;
+        cbk = (k / nq) * 8 + (w & 7);
+        if (cbk >= ncb) return;
+    } else {
+        const int x = w & 7, rep = 8 / ncb;
+        cbk = x % ncb;
+        i = x / ncb + rep * (w >> 3);
+        if (i >= nq) return;
+    }
     const int tid = threadIdx.x, col = tid % S, tau = tid / S;
     const int r = cbk * S + col;
     const int z = blockIdx.z, cb = z >> 1, comp = z & 1;
@@ -1184,7 +1213,15 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
             const int idx = r + N2 * (tau + TPS * (e0 + e));
 #pragma unroll
             for (int kk = 0; kk <= KT; kk++) mp[e][kk] = pp[(long long)kk * n + idx];
-            av[e] = ad ? (double)ad[idx] : 0.0;
+            if (!ad) {
+                av[e] = 0.0;
+            } else if (comp == 0 && ginv) {
+                const unsigned long long k = ((unsigned long long)idx * ginv) & (2ull * n - 1);
+                const uint64_t x = ad[k >= (unsigned long long)n ? (long long)k - n : (long long)k];
+                av[e] = k >= (unsigned long long)n && x != 0 ? q - (double)x : (double)x;
+            } else {
+                av[e] = (double)ad[idx];
+            }
         }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -1204,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
 
 void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B, const uint64_t* accP,
                              const uint64_t* add0, const uint64_t* add1, long long add_ps,
-                             const std::vector<uint64_t*>& outs, hipStream_t s) {
+                             const std::vector<uint64_t*>& outs, hipStream_t s, unsigned long long ginv) {
     RCK_REQUIRE(c.K > 0 && c.logn >= 12, "fused mod-down needs an aux basis and the two-pass NTT");
     const long long n = c.n, EN = (long long)(nq + c.K) * n, P = (long long)nq * n;
     int n1, n2;
@@ -1229,9 +1266,11 @@ void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B,
         ProfScope ps(s, "inv_cols_moddown");
         const int S = 256 / ((1 << n1) / 16);
         const int ncb = (1 << n2) / S;
-        dim3 grid(((ncb + 7) / 8) * 8 * nq, 1, 2 * bc);
+        // see k_inv_cols_md for the order (ncb is a power of two)
+        const int nw = ncb >= 8 ? ((ncb + 7) / 8) * 8 * nq : 8 * ((nq + 8 / ncb - 1) / (8 / ncb));
+        dim3 grid(nw, 1, 2 * bc);
 #define RCK_MD(LR, KK)                                                                                      \
-    k_inv_cols_md<LR, KK><<<grid, 256, 0, s>>>(accb, EN, nq, accPb, c.L, a0, a1, add_ps, op, P, c.logn, c.d_primes, \
+    k_inv_cols_md<LR, KK><<<grid, 256, 0, s>>>(accb, EN, nq, accPb, c.L, a0, a1, add_ps, ginv, op, P, c.logn, c.d_primes, \
                                                c.d_pmodq, c.d_pinvq, c.d_pmod_aux)
 #define RCK_MDK(LR)                                     \
     switch (c.K) {                                      \

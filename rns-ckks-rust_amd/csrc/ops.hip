@@ -162,12 +162,17 @@ void launch_addsub(const Context& c, Batch<const uint64_t> a, Batch<const uint64
     CHK();
 }
 
-void launch_galois(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs, int polys,
-                   uint64_t galois, PrimeMap pm, hipStream_t s) {
+uint64_t galois_inverse(const Context& c, uint64_t galois) {
     uint64_t ginv;
     RCK_REQUIRE(galois % 2 == 1, "galois automorphisms require an odd exponent");
     RCK_REQUIRE(galois < 2ull * c.n, "galois element must be reduced modulo 2N");
     RCK_REQUIRE(h_invmod(galois, 2ull * c.n, &ginv), "galois element must be invertible modulo 2N");
+    return ginv;
+}
+
+void launch_galois(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs, int polys,
+                   uint64_t galois, PrimeMap pm, hipStream_t s) {
+    const uint64_t ginv = galois_inverse(c, galois);
     ProfScope ps(s, "galois");
     k_galois<<<grid_for(c.n, limbs, polys), 256, 0, s>>>(in, out, c.n, ginv, pm, c.d_primes);
     CHK();

@@ -10,6 +10,8 @@ void launch_addsub(const Context& c, Batch<const uint64_t> a, Batch<const uint64
                    int polys, int sub, PrimeMap pm, hipStream_t s);
 void launch_galois(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs, int polys,
                    uint64_t galois, PrimeMap pm, hipStream_t s);
+// g^{-1} mod 2N for a galois element g (odd, reduced, invertible; the crate's error texts otherwise)
+uint64_t galois_inverse(const Context& c, uint64_t galois);
 void launch_rescale(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs_in, int polys,
                     hipStream_t s);
 void launch_mul_add_u64(const Context& c, Batch<const uint64_t> a, Batch<const uint64_t> b, Batch<const uint64_t> cc,
@@ -32,8 +34,9 @@ void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // Balanced base-2^30 decomposition of a coefficient-form polynomial with nq Q-limbs into
 // `levels` signed digits per coefficient: digits[i * N + c] (keyswitching.rs:196-236, 381-467).
 // B polynomials at poly stride `ps` elements -> digits [B][levels][N].
+// ginv != 0: decompose sigma_g(poly) instead (coefficients gathered through X -> X^g, g * ginv = 1 mod 2N).
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s,
-                      int B = 1, long long ps = 0);
+                      int B = 1, long long ps = 0, unsigned long long ginv = 0);
 // For B ciphertexts sharing one key: acc_{0,1}[j] = sum_i NTT_j(digit_i) * key[i][{0,1}][j] for every eval
 // prime j (Q-prefix nq, then naux aux primes), NTT form.  digits: [B][D][N]; key: [>=D][2][nq+naux][N]
 // NTT-form doubles; part: [dsplit][B][2][nq+naux][N] partial sums over digit ranges.  Eval limbs
@@ -64,8 +67,9 @@ void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64
 // Inverse column pass of the Q-limbs of B x 2 keyswitch accumulators (acc: [B][2][nq+K][N] after the
 // inverse row pass) fused with mod-down by P (aux limbs already in coefficient form in
 // accP: [B][2][K][N]) and the optional adds add0/add1 (+ b * add_ps); writes outs[b] = [c0][c1].
+// ginv != 0: add0 is read through the automorphism X -> X^g (g * ginv = 1 mod 2N).
 void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B, const uint64_t* accP,
                              const uint64_t* add0, const uint64_t* add1, long long add_ps,
-                             const std::vector<uint64_t*>& outs, hipStream_t s);
+                             const std::vector<uint64_t*>& outs, hipStream_t s, unsigned long long ginv = 0);
 
 }  // namespace rck

@@ -79,6 +79,42 @@ void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, 
     mult_batch(c, lhs, rhs, 0, 1, limbs, out, 0, s);
 }
 
+// Are ptr(op_0) .. ptr(op_{B-1}) equally spaced?  Returns the stride (elements) or -1.
+template <typename P>
+static long long common_stride(const std::vector<P>& ptrs) {
+    if (ptrs.size() < 2) return 0;
+    const long long st = (long long)(ptrs[1] - ptrs[0]);
+    if (st <= 0) return -1;
+    for (size_t i = 2; i < ptrs.size(); i++)
+        if ((long long)(ptrs[i] - ptrs[i - 1]) != st) return -1;
+    return st;
+}
+
+
+void rotate_batch(Context& c, const std::vector<const uint64_t*>& as, int limbs, const KeyDev* key,
+                  const std::vector<uint64_t*>& outs, hipStream_t s, KsShape shape) {
+    check_ptr(key, "key");
+    const int B = (int)as.size();
+    const long long n = c.n, P = (long long)limbs * n;
+    const long long st = B > 1 ? common_stride(as) : 2 * P;
+    if (c.logn >= 12 && key->aux > 0 && st > 0) {
+        // rotate = (sigma(c0), 0) + keyswitch(sigma(c1)): the decomposition reads c1 and the mod-down
+        // epilogue reads c0 through the automorphism, so no rotated copy is written
+        const uint64_t ginv = galois_inverse(c, key->galois);
+        keyswitch_batch(c, as[0] + P, st, B, limbs, key, as[0], nullptr, st, outs, s, shape, nullptr, 0, ginv);
+        return;
+    }
+    uint64_t* rot = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
+    PrimeMap pm{limbs, (int)c.L, 0};
+    if (B > 1 && st == 2 * P) {  // contiguous inputs: one launch for all B
+        launch_galois(c, {as[0], n, P}, {rot, n, P}, limbs, 2 * B, key->galois, pm, s);
+    } else {
+        for (int i = 0; i < B; i++)
+            launch_galois(c, {as[i], n, P}, {rot + i * 2 * P, n, P}, limbs, 2, key->galois, pm, s);
+    }
+    keyswitch_batch(c, rot + P, 2 * P, B, limbs, key, rot, nullptr, 2 * P, outs, s, shape);
+}
+
 bool mult_relin_folds(const Context& c, const KeyDev* key) { return c.logn >= 12 && key && key->aux > 0; }
 
 void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
@@ -101,18 +137,6 @@ void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t
     const long long n = c.n;
     launch_rescale(c, {in, n, (long long)limbs * n}, {out, n, (long long)(limbs - 1) * n}, limbs, polys, s);
 }
-
-// Are ptr(op_0) .. ptr(op_{B-1}) equally spaced?  Returns the stride (elements) or -1.
-template <typename P>
-static long long common_stride(const std::vector<P>& ptrs) {
-    if (ptrs.size() < 2) return 0;
-    const long long st = (long long)(ptrs[1] - ptrs[0]);
-    if (st <= 0) return -1;
-    for (size_t i = 2; i < ptrs.size(); i++)
-        if ((long long)(ptrs[i] - ptrs[i - 1]) != st) return -1;
-    return st;
-}
-
 
 void run_op_group(Context& c, int kind, int l, const KeyDev* key, const std::vector<const uint64_t*>& as,
                   const std::vector<const uint64_t*>& bs, const std::vector<uint64_t*>& outs, hipStream_t s,
@@ -148,15 +172,7 @@ void run_op_group(Context& c, int kind, int l, const KeyDev* key, const std::vec
             break;
         }
         case RCK_OP_ROTATE: {
-            uint64_t* rot = static_cast<uint64_t*>(c.workspace(kSlotTmp, (size_t)B * 2 * P * 8));
-            PrimeMap pm{l, (int)c.L, 0};
-            if (B > 1 && common_stride(as) == 2 * P) {  // contiguous inputs: one launch for all B
-                launch_galois(c, {as[0], n, P}, {rot, n, P}, l, 2 * B, key->galois, pm, s);
-            } else {
-                for (int i = 0; i < B; i++)
-                    launch_galois(c, {as[i], n, P}, {rot + i * 2 * P, n, P}, l, 2, key->galois, pm, s);
-            }
-            keyswitch_batch(c, rot + P, 2 * P, B, l, key, rot, nullptr, 2 * P, outs, s, shape);
+            rotate_batch(c, as, l, key, outs, s, shape);
             break;
         }
         case RCK_OP_MULT_PLAIN: {
@@ -279,7 +295,7 @@ std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
 // keyswitch_rns_hybrid (:159-194).
 void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, KsShape shape, const double* addn, long long addn_ps) {
+                     hipStream_t s, KsShape shape, const double* addn, long long addn_ps, unsigned long long ginv) {
     RCK_REQUIRE((int)outs.size() == B, "one output per ciphertext");
     check_ptr(key, "key");
     const long long n = c.n, P = (long long)limbs * n;
@@ -298,6 +314,7 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     }
     const int naux = key->aux;
     RCK_REQUIRE(addn == nullptr || (naux > 0 && c.logn >= 12), "folded addends need hybrid keys and the two-pass NTT");
+    RCK_REQUIRE(ginv == 0 || (naux > 0 && c.logn >= 12), "the fused automorphism needs hybrid keys and the two-pass NTT");
     const int E = limbs + naux;
     const long long EN = (long long)E * n;
     const KsShape aut = ks_auto_shape(c, limbs, E, levels, B);
@@ -307,7 +324,7 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     const int prefetch = shape.prefetch >= 0 ? shape.prefetch : aut.prefetch;
     int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
     double* part = static_cast<double*>(c.workspace(6, (size_t)dsplit * B * 2 * EN * 8));
-    launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps);
+    launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps, ginv);
     const int width = shape.group > 0 || shape.rows > 0 ? shape.width : aut.width;
     // the streamed MAC with one digit range also runs the accumulators' inverse row stages (and the fold)
     const bool fuse_inv = naux && c.logn >= 12 && (prefetch == 5 || prefetch == 6) && rows == 4 && dsplit == 1;
@@ -323,7 +340,7 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
         uint64_t* accP = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * naux * n * 8));
         launch_inv_cols(c, part + (long long)limbs * n, n, EN, kOutU64, accP, n, (long long)naux * n, naux, 2 * B,
                         PrimeMap{limbs, (int)c.L, limbs}, s);
-        launch_inv_cols_moddown(c, part, limbs, B, accP, add0, add1, add_ps, outs, s);
+        launch_inv_cols_moddown(c, part, limbs, B, accP, add0, add1, add_ps, outs, s, ginv);
     } else if (naux) {
         uint64_t* accu = static_cast<uint64_t*>(c.workspace(4, (size_t)B * 2 * EN * 8));
         launch_inv(c, part, n, EN, kOutU64, accu, n, EN, E, 2 * B, pm, s, dsplit, sstride);

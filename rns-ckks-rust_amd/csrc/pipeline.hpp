@@ -37,9 +37,15 @@ void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t
 // keyswitch of B polynomials (stride in_ps) with one key, adding add0/add1 (stride add_ps, nullable)
 // into outs[b] = [c0][c1].  addn (hybrid keys, two-pass rings; nullable): NTT-form addends
 // addn[b * addn_ps + {0, limbs*N}] folded into the accumulators (times P) instead of add0/add1.
+// ginv != 0 (hybrid keys, two-pass rings): poly and add0 are read through the automorphism X -> X^g.
 void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, KsShape shape = {}, const double* addn = nullptr, long long addn_ps = 0);
+                     hipStream_t s, KsShape shape = {}, const double* addn = nullptr, long long addn_ps = 0,
+                     unsigned long long ginv = 0);
+// rotate (RnsCkksContext::rotate, context/rotation.rs:104-127) of B ciphertexts at `limbs` limbs with one
+// rotation key
+void rotate_batch(Context& c, const std::vector<const uint64_t*>& as, int limbs, const KeyDev* key,
+                  const std::vector<uint64_t*>& outs, hipStream_t s, KsShape shape = {});
 // mult + relinearize of B ciphertext pairs (stride in_ps) at `limbs` limbs: outs[b] = [c0][c1]
 void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
                       const KeyDev* key, const std::vector<uint64_t*>& outs, hipStream_t s, KsShape shape = {});
