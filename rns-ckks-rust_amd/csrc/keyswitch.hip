@@ -219,7 +219,8 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
 // eval limbs jl = y * jper .. min(g, (y + 1) * jper) - 1 of a group (prime pm(jl)), written to
 // out[z][jl][N].  A digit is prime-independent, so a thread loads its 16 int32 values once and
 // transforms them under every prime of its range; the stores of one prime drain while the next
-// prime's stages run.  Same thread mapping and LDS exchange as k_fwd_cols.
+// prime's stages run (streaming stores: the 8 bytes per residue are read once, by the MAC, much
+// later).  Same thread mapping and LDS exchange as k_fwd_cols.
 // ---------------------------------------------------------------------------------------
 // Phase-B twiddles of the column pass for one prime (stages 4..LOGR-1 need 16 >> (LOGR - s)
 // entries of stage s, starting at (1 << s) + (16 tau >> (LOGR - s))).  They are loaded into
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(256) void k_digit_cols(const int32_t* __restrict__ 
         col_phase_b<LOGR>(v, tw, q, qinv);
         if (jl + 1 < je) tw.load(primes[pm(jl + 1)].fw, tau);
 #pragma unroll
-        for (int e = 0; e < 16; e++) (dst + (long long)N2 * e)[lane] = v[e];
+        for (int e = 0; e < 16; e++) __builtin_nontemporal_store(v[e], &(dst + (long long)N2 * e)[lane]);
     }
 }
 
