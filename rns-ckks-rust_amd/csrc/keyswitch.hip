@@ -466,13 +466,16 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
     }
 }
 
-// Streaming variant: a workgroup is four waves, one ciphertext each, on the same key tile
-// (ring rows b0 .. b0 + ROWS - 1 of one eval limb, ROWS = 64 / (R / 16) rows per wave).
+// Streaming variant: a workgroup is four waves on one tile (ring rows b0 .. b0 + ROWS - 1 of one
+// eval limb, ROWS = 64 / (R / 16) rows per wave): one ciphertext per wave, or (SPLIT) one
+// ciphertext and a quarter of its digit range per wave.
 //   * the column-pass rows of digit i + 1 are copied HBM -> LDS by global_load_lds while digit i
 //     is transformed (no registers held; the copy is waited for when digit i + 1 starts);
 //   * the row-pass twiddles of the tile are staged in LDS once and shared by the four waves;
-//   * the key rows are read in groups after the second phase; the four waves share them through
-//     the CU's L1 and L2.
+//   * the key rows are read in groups after the second phase (with one ciphertext per wave the
+//     four waves read the same rows, served by the CU's L1 and L2).  Vector loads complete in issue
+//     order, so the copy is issued once the exchange has been read and before the key loads: waiting
+//     for the keys then includes the copy, which by then has had the whole second phase to land.
 // Each wave's LDS buffer holds its rows at the padded pitch of row_lds (R + R/16): the copy
 // lands a row linearly, phase A reads it linearly and writes its outputs back in place at the
 // padded positions, which phase B reads without bank conflicts.
