@@ -190,6 +190,7 @@ def main():
     log(f"[bench] per-op: {per_op_ms:.3f} ms/op = {per_op_value:.1f} ops/s")
 
     report = None
+    value_rescale = None
     if args.mode == "batched":
         bt = G.Batch(ctx)
         for b in range(B):
@@ -216,6 +217,21 @@ def main():
         log(f"[bench] batched: {ms_per_step:.3f} ms per batch of {B} = {value:.1f} ops/s "
             f"({value / per_op_value:.2f}x per-op)")
         prof_fn = bt.run_eager
+
+        # also reported: the same batch with the rescale included (mult_relinearized,
+        # context/ops.rs:124-133); it shares the key-switch launch shape tuned above
+        Pr = (L - 1) * n
+        out_r = G.DeviceBuffer((B, 2, L - 1, n))
+        btr = G.Batch(ctx)
+        for b in range(B):
+            btr.add(G.OP_MULT_RELIN_RESCALE, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ksk,
+                    out_r.offset(b * 2 * Pr))
+        btr.compile(tune=not args.no_tune)
+        for _ in range(args.warmup):
+            btr.run(stream)
+        ms_r = time_it(G, lambda: btr.run(stream), args.steps, stream) / args.steps
+        value_rescale = B * 1000.0 / ms_r
+        log(f"[bench] batched with rescale: {ms_r:.3f} ms per batch of {B} = {value_rescale:.1f} ops/s")
     else:
         ms_per_step, value = per_op_ms, per_op_value
         prof_fn = per_op_step
@@ -266,7 +282,8 @@ def main():
         "config": {"workload": f"{args.mode} mult+relinearize, N=2^16, L=24 primes < 2^50, K=2 aux, D={D} digits"
                                + (f", {B} ciphertext pairs per step sharing one relin key" if B > 1 else ""),
                    "mode": args.mode, "batch": B, "per_op_ops_s": round(per_op_value, 2),
-                   "batched_vs_per_op": round(value / per_op_value, 3)},
+                   "batched_vs_per_op": round(value / per_op_value, 3),
+                   "batched_with_rescale_ops_s": round(value_rescale, 2) if value_rescale else None},
         "roofline": roofline, "cpu_baseline": cpu, "tuning": {"per_op": brief(tune1), "batched": brief(report)},
         "kernels_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in top},
     }
