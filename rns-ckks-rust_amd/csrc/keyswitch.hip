@@ -472,7 +472,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
 //   * the column-pass rows of digit i + 1 are copied HBM -> LDS by global_load_lds while digit i
 //     is transformed (no registers held; the copy is waited for when digit i + 1 starts);
 //   * the row-pass twiddles of the tile are staged in LDS once and shared by the four waves;
-//   * the key rows are read in groups after the second phase (with one ciphertext per wave the
+//   * the key rows are read in two groups after the second phase (with one ciphertext per wave the
 //     four waves read the same rows, served by the CU's L1 and L2).  Vector loads complete in issue
 //     order, so the copy is issued once the exchange has been read and before the key loads: waiting
 //     for the keys then includes the copy, which by then has had the whole second phase to land.
@@ -627,24 +627,23 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
         if (i + 1 < i1) dma(src + col_stride);
         __builtin_amdgcn_sched_barrier(0);
         mac_phase_b<LOGR>(v, twb, q, qinv);
-        // key rows in four groups of four positions (issued after the copy: by the time they are
+        // key rows in two groups of eight positions (issued after the copy: by the time they are
         // waited for, the copy has had the whole second phase to land)
         const double* k0 = key + (2LL * i) * EN + (long long)j * n + (long long)b * R + tau;
         const double* k1 = k0 + EN;
 #pragma unroll
-        for (int g4 = 0; g4 < 4; g4++) {
-            double x0[4], x1[4];
+        for (int g8 = 0; g8 < 2; g8++) {
+            double x0[8], x1[8];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                x0[e] = k0[(4 * g4 + e) * TPS];
-                x1[e] = k1[(4 * g4 + e) * TPS];
+            for (int e = 0; e < 8; e++) {
+                x0[e] = k0[(8 * g8 + e) * TPS];
+                x1[e] = k1[(8 * g8 + e) * TPS];
             }
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                a0[4 * g4 + e] += mulmod(v[4 * g4 + e], x0[e], q, qinv);
-                a1[4 * g4 + e] += mulmod(v[4 * g4 + e], x1[e], q, qinv);
+            for (int e = 0; e < 8; e++) {
+                a0[8 * g8 + e] += mulmod(v[8 * g8 + e], x0[e], q, qinv);
+                a1[8 * g8 + e] += mulmod(v[8 * g8 + e], x1[e], q, qinv);
             }
-            __builtin_amdgcn_sched_barrier(0);
         }
         if (((i - i0) & 3) == 3) {
             reduce_all(a0, q, qinv);
