@@ -311,3 +311,31 @@ def test_config2_full_size_mult_relin_bit_exact():
     for comp in range(2):
         for l in range(L):
             assert np.array_equal(got[comp, l], want[comp, l]), f"c{comp} limb {l}"
+
+
+@pytest.mark.parametrize("n,L,K", [(4096, 5, 1), (4096, 4, 3), (1 << 14, 5, 4), (1 << 13, 6, 1)])
+def test_hybrid_aux_prime_counts_match_oracle(n, L, K):
+    """Hybrid relinearize, rotate and mult_relinearized with K = 1, 3, 4 aux primes (the mod-down's
+    Garner over P, its centring and the per-K specialised epilogue), uniform seeded key rows."""
+    p = PR.config_batched(n=n, limbs=L, aux=K)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    qb = O.Basis(n, p.primes, p.roots)
+    eb = O.Basis(n, p.primes + p.aux_primes, p.roots + p.aux_roots)
+    D = ctx.num_decomp_digits(L)
+    rs = np.random.default_rng(100 * K + L)
+    key = np.stack([rand_poly(eb.moduli, n, rs, 2) for _ in range(D)])
+    dk = ctx.ksk_create(key, L - 1, K)
+    quad = rand_poly(qb.moduli, n, rs, 3)
+    out = empty((2, L, n))
+    ctx.relinearize(dev(quad), L, 80, dk, out)
+    assert np.array_equal(out.numpy(), O.relinearize(qb, eb, quad, key))
+    gel = O.galois_element_for_rotation(n, 5)
+    drk = ctx.ksk_create(key, L - 1, K, gel)
+    ct = rand_poly(qb.moduli, n, rs, 2)
+    ctx.rotate(dev(ct), L, 40, drk, out)
+    assert np.array_equal(out.numpy(), O.rotate(qb, eb, ct, gel, key))
+    lhs, rhs = rand_poly(qb.moduli, n, rs, 2), rand_poly(qb.moduli, n, rs, 2)
+    prod = empty((2, L - 1, n))
+    ctx.mult_relinearized(dev(lhs), dev(rhs), L, 40, 40, dk, prod)
+    want = O.rescale(qb, O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs, rhs), key))
+    assert np.array_equal(prod.numpy(), want)
