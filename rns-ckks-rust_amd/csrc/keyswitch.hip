@@ -516,6 +516,9 @@ __device__ __forceinline__ void mac_phase_a(double (&v)[16], const double* twa, 
     reduce_all(v, q, qinv);
 }
 
+// The outputs feed only the key products (mulmod takes |x| <= 3.5q), so instead of reducing all
+// 16 outputs, the 8 upper inputs of the last stage (|u| <= 0.51q + 3 * 0.94q) are reduced before
+// it: every output is then |u| + |t| <= (0.51 + 0.94)q.
 template <int LOGR>
 __device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, double q, double qinv) {
     constexpr int R = 1 << LOGR;
@@ -523,13 +526,17 @@ __device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, 
 #pragma unroll
     for (int s = 4; s < LOGR; s++) {
         const int half = R >> (s + 1), m = LOGR - s;
+        if (s == LOGR - 1) {
+#pragma unroll
+            for (int e = 0; e < 16; e++)
+                if ((e & half) == 0) v[e] = reduce(v[e], q, qinv);
+        }
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) ct_bfly(v[e], v[e + half], twb[(off + (e >> m)) * 64], q, qinv);
         }
         off += 16 >> m;
     }
-    reduce_all(v, q, qinv);
 }
 
 // INV (single digit range): the epilogue also runs the inverse row stages of both accumulators (after
