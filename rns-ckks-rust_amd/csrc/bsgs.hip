@@ -48,7 +48,7 @@ uint32_t bsgs_baby_step(uint32_t dim) {
 // per coefficient (l, x) a (giants x bs) by (bs x vz) product mod q_l.  Lanes run over x (coalesced
 // loads and stores); each thread owns a TG x TV register tile of (giant, vz) outputs, so every loaded
 // diagonal value feeds TV products and every baby value TG products.  Absent diagonals are stored as
-// zeros and contribute exact zeros.  Each product is reduced (|.| <= 0.94q) and the running sums
+// zeros and contribute exact zeros.  Each product is reduced (|.| <= 0.97q) and the running sums
 // every second term (|sum| <= 2.4q < 2^53 for q < 2^51: every addition stays exact).
 template <int TG, int TV>
 __global__ __launch_bounds__(256) void k_diag_mac(const double* __restrict__ baby, const double* __restrict__ diag,

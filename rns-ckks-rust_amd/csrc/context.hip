@@ -230,7 +230,9 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
         const uint32_t T = L + K;
         c->hp.resize(T);
         for (uint32_t i = 0; i < T; i++) {
-            RCK_REQUIRE(all[i] < (1ull << 51), "moduli must be below 2^51 (FP64-exact arithmetic)");
+            // the FP64-exact products need |a*w/q| < 2^51 with lazily reduced inputs (modarith.hpp)
+            RCK_REQUIRE(all[i] < (1ull << 50) + (1ull << 46),
+                        "moduli must be below 2^50 + 2^46 (FP64-exact arithmetic)");
             std::string err = plan_check(n, all[i], roots[i], &c->hp[i]);
             RCK_REQUIRE(err.empty(), "invalid RNS prime " + std::to_string(all[i]) + ": " + err);
         }

@@ -517,8 +517,8 @@ __device__ __forceinline__ void mac_phase_a(double (&v)[16], const double* twa, 
 }
 
 // The outputs feed only the key products (mulmod takes |x| <= 3.5q), so instead of reducing all
-// 16 outputs, the 8 upper inputs of the last stage (|u| <= 0.51q + 3 * 0.94q) are reduced before
-// it: every output is then |u| + |t| <= (0.51 + 0.94)q.
+// 16 outputs, the 8 upper inputs of the last stage (|u| <= 0.51q + 3 * 0.97q) are reduced before
+// it: every output is then |u| + |t| <= (0.51 + 0.97)q.
 template <int LOGR>
 __device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, double q, double qinv) {
     constexpr int R = 1 << LOGR;
@@ -1236,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int idx = r + N2 * (tau + TPS * (e0 + e));
-            const double x = mulmod(v[e0 + e], Pi.ninv, q, qinv);  // x_i, |x| <= 0.94 q
+            const double x = mulmod(v[e0 + e], Pi.ninv, q, qinv);  // x_i, |x| <= 0.97 q
             // centred(x_P) mod q_i from the aux part's Garner digits m_k and its centring flag:
             // Horner over m_{K-1} .. m_0 with p_k mod q_i, minus P mod q_i when x_P > floor(P/2)
             double cv = reduce(mp[e][KT - 1], q, qinv);

@@ -1,4 +1,4 @@
-// Exact modular arithmetic for primes q < 2^51 on the FP64 pipes of CDNA4.
+// Exact modular arithmetic for primes q < 2^50 + 2^46 on the FP64 pipes of CDNA4.
 //
 // MI355X issues FP64 FMA at full rate (64 lanes/clk/CU) while 32-bit integer multiplies run
 // at quarter rate, so residues live in registers as integer-valued doubles and every modular
@@ -8,11 +8,11 @@
 //   r = (h - k*q) + l  (both steps exact because the results are integers below 2^53).
 //
 // Representation: a residue x mod q is any integer-valued double; "reduced" means |x| <= q/2+1.
-// Exactness conditions (q < 2^50.1, all operands integer-valued):
-//   mulmod(a, w): |a| <= 3.5q and |w| <= q/2 + 1 ->  |result| <= 0.94 q
-//                 (|a*w/q| < 2^51 keeps the shift rounding exact; the quotient estimate is
-//                  off by < 0.44, so |a*w - k*q| <= 0.94q; h - k*q and its sum with l are
-//                  integers below 2^53 and therefore exact).
+// Exactness conditions (q < 2^50 + 2^46 < 2^50.1, enforced by the context; operands integer-valued):
+//   mulmod(a, w): |a| <= 3.5q and |w| <= q/2 + 1 ->  |result| <= 0.97 q
+//                 (|a*w/q| <= 1.75q < 2^51 keeps the shift rounding exact; the computed quotient
+//                  is off by < 1.75q * 2^-52 < 0.47, so |a*w - k*q| <= 0.97q; h - k*q and its sum
+//                  with l are integers below 2^53 and therefore exact).
 //   reduce(x):    |x| < 2^53                     ->  |result| <= q/2 + 1.
 // Outputs crossing the C ABI are canonical u64 in [0, q).  Nothing here is approximate: each
 // function returns the exact representative described above.
@@ -27,7 +27,7 @@ constexpr double kShift = 6755399441055744.0;
 
 __device__ __forceinline__ double round_int(double x) { return (x + kShift) - kShift; }
 
-// a*w mod q as a signed representative with |r| <= 0.86q (see header for input bounds).
+// a*w mod q as a signed representative with |r| <= 0.97q (see header for input bounds).
 __device__ __forceinline__ double mulmod(double a, double w, double q, double qinv) {
     double h = a * w;
     double l = __builtin_fma(a, w, -h);

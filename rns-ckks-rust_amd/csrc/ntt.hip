@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, 
 #pragma unroll
             for (int e = 0; e < 16; e++) v[e] += mulmod(an[e * TPS + tau], pm_q, q, qinv);
         }
-        reduce_all(v, q, qinv);  // |sum| <= nsum * 0.51q + 0.94q < 2^53 for nsum <= 8
+        reduce_all(v, q, qinv);  // |sum| <= nsum * 0.51q + 0.97q < 2^53 for nsum <= 8
     }
     inv_phase_b<LOGR>(v, P.iv, N1 + b, tau, q, qinv);
 #pragma unroll

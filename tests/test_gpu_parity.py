@@ -339,3 +339,16 @@ def test_hybrid_aux_prime_counts_match_oracle(n, L, K):
     ctx.mult_relinearized(dev(lhs), dev(rhs), L, 40, 40, dk, prod)
     want = O.rescale(qb, O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs, rhs), key))
     assert np.array_equal(prod.numpy(), want)
+
+
+def test_context_rejects_invalid_chains():
+    """RnsContext / NttPlan::new validation (math/ntt.rs:17-60) with the crate's error texts, and the
+    modulus bound of the FP64-exact arithmetic (2^50 + 2^46)."""
+    with pytest.raises(G.RckError, match="invalid RNS prime 41"):
+        G.Context(poly_degree=8, primes=[41], roots=[6])
+    big = (1 << 50) + (1 << 46) + 17
+    with pytest.raises(G.RckError, match="moduli must be below 2"):
+        G.Context(poly_degree=8, primes=[big], roots=[3])
+    q, g = chain("above_2^50", 1 << 16, 1)
+    ctx = G.Context(poly_degree=8, primes=q, roots=g)  # primes just above 2^50 are accepted
+    assert ctx is not None
