@@ -502,6 +502,12 @@ struct MacTw {
     static constexpr int SIZE = 15 * ROWS + countb() * 64;
 };
 
+// Value bounds of the row pass (multiples of q).  With |w| <= q/2 + 1 and |a| <= A q, A <= 3.75,
+// mulmod returns |a w - k q| <= (0.5 + 0.133 A) q: the quotient estimate is off by at most
+// |a w / q| 2^-52 <= A q 2^-53 < 0.133 A for q < 2^50 + 2^46 (modarith.hpp), and A <= 3.75 keeps
+// |a w / q| below 2^51.  A butterfly whose lower input is bounded by L and upper input by U
+// outputs at most L + 0.5 + 0.133 U.  Phase A takes the column pass's reduced outputs
+// (|x| <= q/2 + 1) through four stages without reducing: 0.5 -> 1.067 -> 1.709 -> 2.436 -> 3.26.
 template <int LOGR>
 __device__ __forceinline__ void mac_phase_a(double (&v)[16], const double* twa, double q, double qinv) {
     constexpr int ROWS = MacTw<LOGR>::ROWS;
@@ -513,12 +519,16 @@ __device__ __forceinline__ void mac_phase_a(double (&v)[16], const double* twa, 
             if ((e & half) == 0) ct_bfly(v[e], v[e + half], twa[((1 << s) - 1 + (e >> (4 - s))) * ROWS], q, qinv);
         }
     }
-    reduce_all(v, q, qinv);
 }
 
-// The outputs feed only the key products (mulmod takes |x| <= 3.5q), so instead of reducing all
-// 16 outputs, the 8 upper inputs of the last stage (|u| <= 0.51q + 3 * 0.97q) are reduced before
-// it: every output is then |u| + |t| <= (0.51 + 0.97)q.
+// Phase B starts from |v| <= 3.26q.  Every output is a chain of lower butterfly inputs plus one
+// twiddle product per stage, so only the chains' lower inputs are reduced: before the first stage
+// its 8 lower inputs (every value is then <= 0.5 + (0.5 + 0.133 * 3.26) = 1.433 after it, then
+// 2.124 after the second stage), and with four stages, before the third stage the 4 lower inputs
+// that feed the last stage's lower inputs (v[e], e % 4 == 0): those pairs give <= 1.282, the other
+// pairs <= 2.906, and the last stage <= 1.282 + 0.5 + 0.133 * 2.906 = 2.17.  With three stages the
+// last one gives <= 2.906.  Multiplied inputs stay <= 3.26 and the outputs feed the key products
+// (|a| <= 3.75q) directly: 12 reductions instead of 16.
 template <int LOGR>
 __device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, double q, double qinv) {
     constexpr int R = 1 << LOGR;
@@ -526,10 +536,14 @@ __device__ __forceinline__ void mac_phase_b(double (&v)[16], const double* twb, 
 #pragma unroll
     for (int s = 4; s < LOGR; s++) {
         const int half = R >> (s + 1), m = LOGR - s;
-        if (s == LOGR - 1) {
+        if (s == 4) {
 #pragma unroll
             for (int e = 0; e < 16; e++)
                 if ((e & half) == 0) v[e] = reduce(v[e], q, qinv);
+        }
+        if (s == 6 && LOGR == 8) {
+#pragma unroll
+            for (int e = 0; e < 16; e += 4) v[e] = reduce(v[e], q, qinv);
         }
 #pragma unroll
         for (int e = 0; e < 16; e++) {
@@ -618,6 +632,7 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
 #pragma unroll
     for (int e = 0; e < 16; e++) a0[e] = a1[e] = 0.0;
     const double* src = tile0;
+    int pend = 0;  // products added since the accumulators were last reduced
     for (int i = i0; i < i1; i++, src += col_stride) {
         __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this digit's rows are in LDS
         asm volatile("" ::: "memory");
@@ -652,7 +667,10 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
                 a1[8 * g8 + e] += mulmod(v[8 * g8 + e], x1[e], q, qinv);
             }
         }
-        if (((i - i0) & 3) == 3) {
+        // each product is <= 0.94q (inputs <= 3.26q, mac_phase_a), so six of them on a reduced
+        // accumulator stay below 6.2q < 2^53 / q: exact integers
+        if (++pend == 6) {
+            pend = 0;
             reduce_all(a0, q, qinv);
             reduce_all(a1, q, qinv);
         }
