@@ -178,6 +178,20 @@ static std::vector<uint32_t> half_words(std::vector<uint32_t> w) {
     return w;
 }
 
+// 32-bit little-endian words -> 30-bit little-endian words (at most kWords30)
+static std::vector<uint32_t> words30(const std::vector<uint32_t>& w) {
+    std::vector<uint32_t> out;
+    const size_t bits = w.size() * 32;
+    for (size_t b = 0; b < bits; b += 30) {
+        uint32_t v = 0;
+        for (int k = 0; k < 30 && b + k < bits; k++) v |= ((w[(b + k) / 32] >> ((b + k) % 32)) & 1u) << k;
+        out.push_back(v);
+    }
+    while (!out.empty() && out.back() == 0) out.pop_back();
+    RCK_REQUIRE(out.size() <= (size_t)kWords30, "ciphertext modulus too large for the decomposition kernel");
+    return out;
+}
+
 static uint32_t word_bits(const std::vector<uint32_t>& w) {
     size_t n = w.size();
     while (n > 1 && w[n - 1] == 0) n--;
@@ -263,7 +277,7 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
         for (uint32_t i = 0; i < T; i++) {
             const HostPrime& h = c->hp[i];
             double* base = c->d_tables + (size_t)i * 4 * n;
-            dp[i] = DevPrime{(double)h.q, 1.0 / (double)h.q, h_centred(h.n_inv, h.q), 0.0, base, base + n,
+            dp[i] = DevPrime{(double)h.q, 1.0 / (double)h.q, h_centred(h.n_inv, h.q), h.q, base, base + n,
                              base + 2 * n, base + 3 * n};
         }
         c->d_primes = upload(dp);
@@ -277,9 +291,9 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
                 invq[(size_t)i * T + j] = h_centred(v, all[i]);
             }
         c->d_invq = upload(invq);
-        // Q_l words for l = 0..L
+        // Q_l words for l = 0..L (30-bit words on the device)
         // [l][0][:] = words of Q_l, [l][1][:] = words of floor(Q_l / 2)
-        std::vector<uint32_t> qw((size_t)(L + 1) * 2 * kWords, 0);
+        std::vector<uint32_t> qw((size_t)(L + 1) * 2 * kWords30, 0);
         c->qwords.resize(L + 1);
         c->qbits.resize(L + 1);
         for (uint32_t l = 0; l <= L; l++) {
@@ -287,9 +301,9 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
             RCK_REQUIRE(w.size() <= (size_t)kWords - 2, "ciphertext modulus too large for the decomposition kernel");
             c->qwords[l] = w;
             c->qbits[l] = word_bits(w);
-            std::memcpy(&qw[(size_t)l * 2 * kWords], w.data(), w.size() * 4);
-            auto h = half_words(w);
-            std::memcpy(&qw[(size_t)l * 2 * kWords + kWords], h.data(), h.size() * 4);
+            auto w30 = words30(w), h30 = words30(half_words(w));
+            std::memcpy(&qw[(size_t)l * 2 * kWords30], w30.data(), w30.size() * 4);
+            std::memcpy(&qw[(size_t)l * 2 * kWords30 + kWords30], h30.data(), h30.size() * 4);
         }
         c->d_qwords = upload(qw);
         if (K > 0) {

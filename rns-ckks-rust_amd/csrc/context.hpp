@@ -19,7 +19,7 @@ struct DevPrime {
     double q;          // modulus
     double qinv;       // fl(1/q)
     double ninv;       // N^{-1} mod q, centred
-    double pad0;
+    uint64_t qu;       // q as an integer (wave-uniform integer arithmetic)
     const double* fw;  // forward twiddles: fw[k] = psi^{bitrev_logN(k)}, centred, k < N
     const double* iv;  // inverse twiddles: iv[k] = psi^{-bitrev_logN(k)}, centred
     const double* tw;  // natural-order psi^{-j} (cyclic-mode pre-twist), centred
@@ -30,6 +30,7 @@ struct DevPrime {
 constexpr int kMaxQ = 64;     // Q primes supported by the decomposition kernels
 constexpr int kMaxP = 4;      // aux primes supported by mod-down
 constexpr int kWords = 64;    // 32-bit words of the multiword scratch (2048 bits)
+constexpr int kWords30 = 72;  // 30-bit words of the decomposition's Q_l tables (2160 bits)
 
 struct HostPrime {
     uint64_t q, g, root, root_inv, psi, psi_inv, n_inv;
@@ -44,7 +45,8 @@ struct Context {
     double* d_tables = nullptr;
     // inv_q[i * (L+K) + j] = q_j^{-1} mod q_i (centred), for i != j over the whole chain
     double* d_invq = nullptr;
-    // qwords[l * kWords + w]: 32-bit words of Q_l = q_0 ... q_{l-1}, l = 0..L
+    // d_qwords[(l * 2 + h) * kWords30 + w]: 30-bit words of Q_l = q_0 ... q_{l-1} (h = 0) and of
+    // floor(Q_l / 2) (h = 1), l = 0..L (the decomposition's base-2^30 digits are these words)
     uint32_t* d_qwords = nullptr;
     std::vector<std::vector<uint32_t>> qwords;  // host copy
     std::vector<uint32_t> qbits;                // bits(Q_l)

@@ -92,38 +92,45 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
             return lds_m[i * kDecBS + tid];
         }
     };
-    // Horner: X = m_{nq-1}; X = X * q_i + m_i
+    // Horner over 30-bit words (the digits' base): X = 0; X = X * q_i + m_i for i = nq-1 .. 0.
+    // With q = qhi 2^30 + qlo (qhi < 2^21), word k of X * q + m is the low 30 bits of
+    //   S_k = x_k qlo + x_{k-1} qhi + C_k,  C_0 = m < 2^51,  C_{k+1} = S_k >> 30,
+    // and S_k < 2^60 + 2^51 + 2^51 never overflows: two 32x32+64 multiply-adds, a mask and a shift
+    // per word.  After step i, X < q_i ... q_{nq-1} < 2^{51 (nq - i)}, so words at or beyond
+    // (51 (nq - i) + 29) / 30 + 1 stay zero and are skipped.
     uint32_t x[W];
 #pragma unroll
     for (int k = 0; k < W; k++) x[k] = 0;
-    {
-        uint64_t top = (uint64_t)mdig(nq - 1);
-        x[0] = (uint32_t)top;
-        x[1] = (uint32_t)(top >> 32);
-    }
-    for (int i = nq - 2; i >= 0; i--) {
-        const uint64_t q = (uint64_t)primes[i].q;
-        const uint32_t qlo = (uint32_t)q, qhi = (uint32_t)(q >> 32);
-        // X * q_i + m_i < q_i ... q_{nq-1} < 2^{51 (nq - i)}: words beyond `len` stay zero
-        // (len is wave-uniform, so the skipped words cost no issue slots).
-        // One pass, two carry chains: t = x_k qlo + c0 (< 2^64 - 2^32), u = lo32(t) + x_{k-1} qhi + c1
-        // (< 2^53); the new word is lo32(u).
-        const int len = min(W, (51 * (nq - i) + 31) / 32 + 1);
-        uint64_t c0 = 0, c1 = (uint64_t)mdig(i);
+    auto step = [&](int k_len, uint64_t q, double mi) {
+        const uint32_t qlo = (uint32_t)q & 0x3fffffffu, qhi = (uint32_t)(q >> 30);
+        // mi is an integer in [0, 2^51): the low 52 bits of mi + 2^52 are mi itself
+        uint64_t C = (uint64_t)__double_as_longlong(mi + 4503599627370496.0) & ((1ull << 52) - 1);
         uint32_t prev = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            if (k < len) {
-                const uint64_t t = (uint64_t)x[k] * qlo + c0;
-                const uint64_t u = (t & 0xffffffffull) + (uint64_t)prev * qhi + c1;
+            if (k < k_len) {
+                const uint64_t S = (uint64_t)x[k] * qlo + ((uint64_t)prev * qhi + C);
                 prev = x[k];
-                x[k] = (uint32_t)u;
-                c0 = t >> 32;
-                c1 = u >> 32;
+                x[k] = (uint32_t)S & 0x3fffffffu;
+                C = S >> 30;
             }
         }
+    };
+    if constexpr (NQM > 0) {
+        // fully unrolled and branch-free: every Garner digit and word index is a constant, the word
+        // count of step i is bounded with NQM in place of nq (the extra words compute zeros), and the
+        // steps i >= nq multiply by 1 and add 0 (a branch around them would make the compiler copy
+        // every word at each join)
+#pragma clang loop unroll(full)
+        for (int i = NQM - 1; i >= 0; i--) {
+            const bool on = i < nq;
+            step(min(W, (51 * (NQM - i) + 29) / 30 + 1), on ? primes[i].qu : 1ull, on ? m[i] : 0.0);
+        }
+    } else {
+        for (int i = nq - 1; i >= 0; i--) step(min(W, (51 * (nq - i) + 29) / 30 + 1), primes[i].qu, mdig(i));
     }
-    // centring: X > floor(Q/2)  ->  X - Q
+    // centring: X > floor(Q/2)  ->  X - Q, two's complement over the W words (30 W >= bits(Q): the
+    // centred value lies in (-Q/2, Q/2))
     bool gt = false, decided = false;
 #pragma unroll
     for (int k = W - 1; k >= 0; k--) {
@@ -134,68 +141,64 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
         }
     }
     if (gt) {
-        uint64_t borrow = 0;
+        uint32_t borrow = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            uint64_t t = (uint64_t)x[k] - qw[k] - borrow;
-            x[k] = (uint32_t)t;
-            borrow = (t >> 32) & 1;
+            const uint32_t t = x[k] - qw[k] - borrow;  // in (-2^31, 2^30): bit 31 is the borrow
+            x[k] = t & 0x3fffffffu;
+            borrow = t >> 31;
         }
     }
-    // balanced digits of X (two's complement, sign-extended past word W - 1).  The digit loop is
-    // unrolled over the most digits W words can hold, so every word index is a constant and X stays
-    // in registers; `levels` (uniform) guards the tail.
-    const uint32_t sx = (x[W - 1] >> 31) ? 0xffffffffu : 0u;
+    // balanced base-2^30 digits: digit i is word i plus the carry (words past W - 1 are the sign
+    // extension); d >= 2^29 becomes d - 2^30 with a carry into the next digit
+    const uint32_t sx = (x[W - 1] >> 29) ? 0x3fffffffu : 0u;
     int carry = 0;
 #pragma unroll
-    for (int i = 0; i < (32 * W) / 30 + 1; i++) {
+    for (int i = 0; i < W; i++) {
         if (i < levels) {
-            const int bit = 30 * i, w = bit >> 5, sh = bit & 31;
-            const uint64_t lo = w < W ? x[w] : sx;
-            const uint64_t hi = w + 1 < W ? x[w + 1] : sx;
-            const int raw = (int)(((lo | (hi << 32)) >> sh) & 0x3fffffffu);
-            int d = raw + carry;
-            if (d >= (1 << 29)) {
-                d -= (1 << 30);
-                carry = 1;
-            } else {
-                carry = 0;
-            }
-            digits[(long long)i * n + c] = d;
+            int d = (int)x[i] + carry;
+            carry = d >= (1 << 29);
+            digits[(long long)i * n + c] = carry ? d - (1 << 30) : d;
         }
+    }
+    for (int i = W; i < levels; i++) {
+        int d = (int)sx + carry;
+        carry = d >= (1 << 29);
+        digits[(long long)i * n + c] = carry ? d - (1 << 30) : d;
     }
 }
 
 template <int W, int NQM>
 static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
                    long long ps, unsigned long long ginv) {
-    const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords;
+    const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords30;
     ProfScope prof(s, "decompose");
     const size_t lds = NQM == 0 ? (size_t)nq * kDecBS * 8 : 0;
     k_decompose<W, NQM><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, lds, s>>>(
-        poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords, digits, ginv);
+        poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords30, digits, ginv);
     RCK_CHECK_HIP(hipGetLastError());
 }
 
 // Register-resident Garner digits up to 24 limbs (beyond that the unrolled digit array would spill);
-// larger limb counts keep the digits in LDS (NQM = 0).
+// larger limb counts keep the digits in LDS (NQM = 0).  W words hold at most 30 W bits of Q, so
+// NQM limbs of at most 51 bits need W >= 51 NQM / 30: smaller W only pair with fewer limbs.
 template <int W>
 static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
                   long long ps, unsigned long long ginv) {
-    if (nq <= 8)
-        dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps, ginv);
-    else if (nq <= 16)
-        dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps, ginv);
-    else if (nq <= 24)
-        dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps, ginv);
-    else
-        dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv);
+    if constexpr (W <= 48) {
+        if (nq <= 8) return dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        if (nq <= 16) return dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        if (nq <= 24) return dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps, ginv);
+    }
+    dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv);
 }
 
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
                       long long ps, unsigned long long ginv) {
     RCK_REQUIRE(nq >= 1 && nq <= (int)c.L, "bad limb count for decomposition");
-    const int need = std::max((int)(c.qbits[nq] + 32) / 32 + 1, (30 * levels + 31) / 32 + 1);
+    RCK_REQUIRE(levels >= 1, "decomposition needs at least one digit");
+    // 30-bit words holding Q_nq (and the centred value in two's complement)
+    const int need = std::max(1, (int)(c.qbits[nq] + 29) / 30);
     if (need <= 8)
         dec_t<8>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 16)
@@ -210,8 +213,10 @@ void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels
         dec_t<48>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else if (need <= 64)
         dec_t<64>(c, poly, nq, levels, digits, s, B, ps, ginv);
+    else if (need <= kWords30)
+        dec_t<kWords30>(c, poly, nq, levels, digits, s, B, ps, ginv);
     else
-        RCK_REQUIRE(false, "decomposition needs more than 2048 bits");
+        RCK_REQUIRE(false, "decomposition needs more than 2160 bits");
 }
 
 // ---------------------------------------------------------------------------------------
@@ -245,12 +250,20 @@ struct ColTw {
     }
 };
 
+// Phase A runs on digits (|d| < 2^29, far below q) without reducing: its outputs are <= 2.44q
+// (stage bounds 0.5, 1.07, 1.71, 2.44; see mac_phase_a for the per-stage rule), so phase B reduces
+// only its first stage's 8 lower inputs (then 1.32, 2.0, 2.77, 3.63) before the final reduction.
 template <int LOGR>
 __device__ __forceinline__ void col_phase_b(double (&v)[16], const ColTw<LOGR>& t, double q, double qinv) {
     int k = 0;
 #pragma unroll
     for (int s = 4; s < LOGR; s++) {
         const int half = (1 << LOGR) >> (s + 1);
+        if (s == 4) {
+#pragma unroll
+            for (int e = 0; e < 16; e++)
+                if ((e & half) == 0) v[e] = reduce(v[e], q, qinv);
+        }
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) ct_bfly(v[e], v[e + half], t.w[k + (e >> (LOGR - s))], q, qinv);
@@ -296,7 +309,7 @@ __global__ __launch_bounds__(256) void k_digit_cols(const int32_t* __restrict__ 
             asm volatile("" : "+v"(t));
             v[e] = (double)t;
         }
-        fwd_phase_a<kTwScalar>(v, P.fw, 1, q, qinv);
+        fwd_phase_a<kTwScalar, false>(v, P.fw, 1, q, qinv);
         lds_barrier();  // the previous prime's LDS reads are done
 #pragma unroll
         for (int e = 0; e < 16; e++) lds[(tau + TPS * e) * S + col] = v[e];

@@ -85,8 +85,9 @@ __device__ __forceinline__ void reduce_all(double (&v)[E], double q, double qinv
 
 // Phase A of a 2^LOGR-point sub-transform: the thread holds x = tau + (R/16) e, e < 16, and
 // runs stages 0..3 (distance bits LOGR-1 .. LOGR-4).  Twiddle of stage s, group i:
-// tbl[(base << s) + i].  Inputs |v| <= 0.51q; outputs reduced.  MODE: how the twiddles are read.
-template <int MODE = kTwGlobal>
+// tbl[(base << s) + i].  Inputs |v| <= 0.51q; outputs reduced (REDUCE) or, without the final
+// reduction, |v| <= 3.26q (2.44q for inputs far below q).  MODE: how the twiddles are read.
+template <int MODE = kTwGlobal, bool REDUCE = true>
 __device__ __forceinline__ void fwd_phase_a(double (&v)[16], const double* __restrict__ tbl, int base, double q,
                                             double qinv) {
 #pragma unroll
@@ -100,7 +101,7 @@ __device__ __forceinline__ void fwd_phase_a(double (&v)[16], const double* __res
             }
         }
     }
-    reduce_all(v, q, qinv);
+    if constexpr (REDUCE) reduce_all(v, q, qinv);
 }
 
 // Phase B: the thread holds x = 16 tau + e and runs stages 4..LOGR-1.

@@ -341,6 +341,31 @@ def test_hybrid_aux_prime_counts_match_oracle(n, L, K):
     assert np.array_equal(prod.numpy(), want)
 
 
+@pytest.mark.parametrize("kind", ["below_2^50", "above_2^50"])
+def test_relinearize_26_limbs_matches_oracle(kind):
+    """26 ciphertext primes (more than the decomposition keeps in registers: its Garner digits go
+    through LDS) and 2 aux primes at N=4096, 44 digits: relinearize and rotate == oracle."""
+    n = 4096
+    q, g = chain(kind, n, 28)
+    ctx = G.Context(poly_degree=n, primes=q[:26], roots=g[:26], aux_primes=q[26:], aux_roots=g[26:])
+    qb = O.Basis(n, q[:26], g[:26])
+    eb = O.Basis(n, q, g)
+    D = ctx.num_decomp_digits(26)
+    assert D == O.num_decomp_digits(qb.total_modulus_bits())
+    rs = np.random.default_rng(26)
+    key = np.stack([rand_poly(eb.moduli, n, rs, 2) for _ in range(D)])
+    dk = ctx.ksk_create(key, 25, 2)
+    quad = rand_poly(qb.moduli, n, rs, 3)
+    out = empty((2, 26, n))
+    ctx.relinearize(dev(quad), 26, 80, dk, out)
+    assert np.array_equal(out.numpy(), O.relinearize(qb, eb, quad, key))
+    gel = O.galois_element_for_rotation(n, -3)
+    drk = ctx.ksk_create(key, 25, 2, gel)
+    ct = rand_poly(qb.moduli, n, rs, 2)
+    ctx.rotate(dev(ct), 26, 40, drk, out)
+    assert np.array_equal(out.numpy(), O.rotate(qb, eb, ct, gel, key))
+
+
 def test_context_rejects_invalid_chains():
     """RnsContext / NttPlan::new validation (math/ntt.rs:17-60) with the crate's error texts, and the
     modulus bound of the FP64-exact arithmetic (2^50 + 2^46)."""
