@@ -296,6 +296,11 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
         std::vector<uint32_t> qw((size_t)(L + 1) * 2 * kWords30, 0);
         c->qwords.resize(L + 1);
         c->qbits.resize(L + 1);
+        {
+            uint64_t qmin = all[0], qmax = all[0];
+            for (uint32_t i = 1; i < L; i++) qmin = std::min(qmin, all[i]), qmax = std::max(qmax, all[i]);
+            c->garner_lazy = (double)qmax <= 2.5 * (double)qmin;
+        }
         for (uint32_t l = 0; l <= L; l++) {
             auto w = product_words(all.data(), l);
             RCK_REQUIRE(w.size() <= (size_t)kWords - 2, "ciphertext modulus too large for the decomposition kernel");
@@ -388,6 +393,7 @@ Context* Context::scratch_clone() const {
     c->d_qwords = d_qwords;
     c->qwords = qwords;
     c->qbits = qbits;
+    c->garner_lazy = garner_lazy;
     c->d_pwords = d_pwords;
     c->d_pmodq = d_pmodq;
     c->d_pinvq = d_pinvq;

@@ -50,6 +50,7 @@ struct Context {
     uint32_t* d_qwords = nullptr;
     std::vector<std::vector<uint32_t>> qwords;  // host copy
     std::vector<uint32_t> qbits;                // bits(Q_l)
+    bool garner_lazy = false;  // every Q prime is below 2.5x the smallest (decomposition skips reductions)
     // mod-down constants (aux basis P): pwords = words of P, halfp = floor(P/2)
     uint32_t* d_pwords = nullptr;
     double* d_pmodq = nullptr;     // [L] P mod q_i (centred)

@@ -64,8 +64,10 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
                 double v = rd(i, q);
 #pragma unroll
                 for (int j = 0; j < i; j++) {
-                    double a = reduce(v - m[j], q, qinv);
-                    v = mulmod(a, invq[(long long)i * T + j], q, qinv);
+                    // no reduction: every Q prime is below 2.5x the smallest (the launcher routes other
+                    // chains to NQM = 0), so |v| <= q_i and 0 <= m_j < 2.5 q_i give |v - m_j| < 3.5 q_i,
+                    // a valid mulmod input, and the product is <= 0.97 q_i
+                    v = mulmod(v - m[j], invq[(long long)i * T + j], q, qinv);
                 }
                 m[i] = i == 0 ? v : canon(v, q, qinv);
             }
@@ -179,13 +181,15 @@ static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, i
     RCK_CHECK_HIP(hipGetLastError());
 }
 
-// Register-resident Garner digits up to 24 limbs (beyond that the unrolled digit array would spill);
-// larger limb counts keep the digits in LDS (NQM = 0).  W words hold at most 30 W bits of Q, so
-// NQM limbs of at most 51 bits need W >= 51 NQM / 30: smaller W only pair with fewer limbs.
+// Register-resident Garner digits up to 24 limbs (beyond that the unrolled digit array would spill)
+// for chains whose Q primes lie within a factor 2.5 of each other (their Garner skips reductions);
+// larger limb counts and other chains keep the digits in LDS (NQM = 0).  W words hold at most 30 W
+// bits of Q, so NQM limbs of at most 51 bits need W >= 51 NQM / 30: W > 48 only pairs with NQM = 0.
 template <int W>
 static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
                   long long ps, unsigned long long ginv) {
     if constexpr (W <= 48) {
+        if (!c.garner_lazy) return dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv);
         if (nq <= 8) return dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps, ginv);
         if (nq <= 16) return dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps, ginv);
         if (nq <= 24) return dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps, ginv);

@@ -366,6 +366,29 @@ def test_relinearize_26_limbs_matches_oracle(kind):
     assert np.array_equal(out.numpy(), O.rotate(qb, eb, ct, gel, key))
 
 
+def test_relinearize_mixed_size_primes_matches_oracle():
+    """A chain mixing ~2^50 and ~2^30 primes (the decomposition's Garner then keeps its reductions
+    and its digits in LDS): relinearize == oracle at N=4096."""
+    n = 4096
+    big, small = PR.primes_below(50, n, 4), PR.primes_below(30, n, 2)
+    q = [big[0], small[0], big[1], small[1]]
+    aux = big[2:]
+    roots = [PR.smallest_primitive_root(x) for x in q]
+    aroots = [PR.smallest_primitive_root(x) for x in aux]
+    ctx = G.Context(poly_degree=n, primes=q, roots=roots, aux_primes=aux, aux_roots=aroots)
+    qb = O.Basis(n, q, roots)
+    eb = O.Basis(n, q + aux, roots + aroots)
+    D = ctx.num_decomp_digits(4)
+    assert D == O.num_decomp_digits(qb.total_modulus_bits())
+    rs = np.random.default_rng(7)
+    key = np.stack([rand_poly(eb.moduli, n, rs, 2) for _ in range(D)])
+    dk = ctx.ksk_create(key, 3, 2)
+    quad = rand_poly(qb.moduli, n, rs, 3)
+    out = empty((2, 4, n))
+    ctx.relinearize(dev(quad), 4, 80, dk, out)
+    assert np.array_equal(out.numpy(), O.relinearize(qb, eb, quad, key))
+
+
 def test_context_rejects_invalid_chains():
     """RnsContext / NttPlan::new validation (math/ntt.rs:17-60) with the crate's error texts, and the
     modulus bound of the FP64-exact arithmetic (2^50 + 2^46)."""
