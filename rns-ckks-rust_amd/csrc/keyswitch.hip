@@ -277,10 +277,6 @@ __device__ __forceinline__ void col_phase_b(double (&v)[16], const ColTw<LOGR>& 
     reduce_all(v, q, qinv);
 }
 
-// Workgroup barrier ordering LDS only.  __syncthreads() also waits for every outstanding global
-// store of the wave (a release fence), which would serialise one prime's stores with the next
-// prime's transform.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_digit_cols(const int32_t* __restrict__ digits, double* __restrict__ out,

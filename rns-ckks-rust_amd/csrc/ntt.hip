@@ -138,10 +138,10 @@ __device__ __forceinline__ void row_fwd(double (&v)[16], const double* p, double
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
     fwd_phase_a(v, fw, base, q, qinv);
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int e = 0; e < 16; e++) lds[row_lds<R>(row, tau + TPS * e)] = v[e];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
     fwd_phase_b<LOGR>(v, fw, base, tau, q, qinv);
@@ -152,10 +152,10 @@ __device__ __forceinline__ void row_inv_store(double (&v)[16], double* p, double
                                               const double* __restrict__ iv, double q, double qinv) {
     constexpr int R = 1 << LOGR, TPS = R / 16;
     inv_phase_b<LOGR>(v, iv, base, tau, q, qinv);
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int e = 0; e < 16; e++) lds[row_lds<R>(row, 16 * tau + e)] = v[e];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, tau + TPS * e)];
     inv_phase_a(v, iv, base, q, qinv);

@@ -65,6 +65,12 @@ __device__ __forceinline__ double tw_load(const double* tbl, int i) {
     }
 }
 
+// Workgroup barrier ordering LDS only.  __syncthreads() also waits for every outstanding global
+// load and store of the wave (a release fence), which would serialise one operand's or prime's
+// memory traffic with the next one's transform.  Values loaded from global memory still get their
+// own waits before use.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // LDS index of a row tile with one pad word per 16 (conflict-free for the x = 16 tau + e reads).
 // The row pitch R + R/16 is 16, 8, 4 doubles past a multiple of 32 for R = 256, 128, 64, so the
 // rows that one half-wave touches (2, 4, 8 of them) land on disjoint banks as well.
