@@ -83,17 +83,6 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
             lds_m[i * kDecBS + tid] = i == 0 ? v : canon(v, q, qinv);
         }
     }
-    auto mdig = [&](int i) -> double {  // Garner digit i (i < nq)
-        if constexpr (NQM > 0) {
-            double r = 0.0;
-#pragma unroll
-            for (int k = 0; k < NQM; k++)
-                if (k == i) r = m[k];
-            return r;
-        } else {
-            return lds_m[i * kDecBS + tid];
-        }
-    };
     // Horner over 30-bit words (the digits' base): X = 0; X = X * q_i + m_i for i = nq-1 .. 0.
     // With q = qhi 2^30 + qlo (qhi < 2^21), word k of X * q + m is the low 30 bits of
     //   S_k = x_k qlo + x_{k-1} qhi + C_k,  C_0 = m < 2^51,  C_{k+1} = S_k >> 30,
@@ -129,7 +118,8 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
             step(min(W, (51 * (NQM - i) + 29) / 30 + 1), on ? primes[i].qu : 1ull, on ? m[i] : 0.0);
         }
     } else {
-        for (int i = nq - 1; i >= 0; i--) step(min(W, (51 * (nq - i) + 29) / 30 + 1), primes[i].qu, mdig(i));
+        for (int i = nq - 1; i >= 0; i--)
+            step(min(W, (51 * (nq - i) + 29) / 30 + 1), primes[i].qu, lds_m[i * kDecBS + tid]);
     }
     // centring: X > floor(Q/2)  ->  X - Q, two's complement over the W words (30 W >= bits(Q): the
     // centred value lies in (-Q/2, Q/2))

@@ -1,0 +1,83 @@
+"""The device's FP64-exact modular product and reduction, restated with Python floats (IEEE double
+products) and an exactly rounded fma (a rational result rounded once), checked at the extremes
+that the kernels' reduction schedules rely on.  CPU only.
+
+Lemma (rns-ckks-rust_amd/csrc/modarith.hpp): for q < 2^50 + 2^46, |w| <= q/2 + 1 and
+|a| <= A q with A <= 3.75,
+    mulmod(a, w) = (a w - k q) + l  is congruent to a w mod q  and  |mulmod(a, w)| <= (0.5 + 0.133 A) q;
+reduce(x) is congruent to x and |reduce(x)| <= q/2 + 1 for |x| < 2^53.
+The NTT row pass, the digit column pass and the decomposition's Garner skip reductions on the
+strength of the A-dependent bound, so a violation at any A used there (0.5 .. 3.75) fails here.
+"""
+import random
+from fractions import Fraction
+
+import pytest
+
+SHIFT = 6755399441055744.0  # 1.5 * 2^52
+
+
+def fma(x, y, z):
+    return float(Fraction(x) * Fraction(y) + Fraction(z))  # one rounding, to nearest even
+
+
+def mulmod(a, w, q, qinv):
+    h = a * w
+    l = fma(a, w, -h)
+    k = fma(h, qinv, SHIFT) - SHIFT
+    r = fma(-k, q, h)
+    return r + l
+
+
+def reduce(x, q, qinv):
+    k = fma(x, qinv, SHIFT) - SHIFT
+    return fma(-k, q, x)
+
+
+QS = [(1 << 50) + (1 << 46) - 1, (1 << 50) + (1 << 46) - 8191, (1 << 50) - 1, 1125899906826241, (1 << 30) + 1]
+AS = [0.5, 1.0665, 1.7082, 2.4352, 3.2589, 3.5, 3.75]
+
+
+@pytest.mark.parametrize("q", QS)
+def test_mulmod_bound_and_congruence(q):
+    rng = random.Random(q)
+    qf, qinv = float(q), 1.0 / float(q)
+    wmax = q // 2 + 1
+    for A in AS:
+        amax = int(A * q)
+        cases = [(s1 * amax, s2 * wmax) for s1 in (1, -1) for s2 in (1, -1)]
+        cases += [(rng.randint(-amax, amax), rng.choice((1, -1)) * wmax) for _ in range(300)]
+        cases += [(rng.choice((1, -1)) * amax, rng.randint(-wmax, wmax)) for _ in range(300)]
+        cases += [(rng.randint(-amax, amax), rng.randint(-wmax, wmax)) for _ in range(300)]
+        bound = (0.5 + 0.133 * A) * q
+        for a, w in cases:
+            r = mulmod(float(a), float(w), qf, qinv)
+            assert r == int(r), (q, A, a, w)
+            assert (int(r) - a * w) % q == 0, (q, A, a, w, r)
+            assert abs(r) <= bound, (q, A, a, w, r / q)
+
+
+@pytest.mark.parametrize("q", QS)
+def test_reduce_bound_and_congruence(q):
+    rng = random.Random(~q)
+    qf, qinv = float(q), 1.0 / float(q)
+    xmax = min((1 << 53) - 1, int(7.5 * q))
+    xs = [xmax, -xmax, q // 2, -(q // 2), q, 3 * q // 2 + 1] + [rng.randint(-xmax, xmax) for _ in range(2000)]
+    for x in xs:
+        r = reduce(float(x), qf, qinv)
+        assert (int(r) - x) % q == 0, (q, x, r)
+        assert abs(r) <= q / 2 + 1, (q, x, r / q)
+
+
+def test_bound_rule_stage_chain():
+    """The per-stage chain used by the row passes: from reduced inputs (0.5q) four CT stages stay
+    within the multiply-input limit and end at <= 3.26q; each bound on the way is one of the A
+    values the lemma is checked at above."""
+    b = 0.5
+    chain = [b]
+    for _ in range(4):
+        b = b + 0.5 + 0.133 * b
+        chain.append(b)
+    assert all(x <= 3.75 for x in chain[:-1])
+    assert chain[-1] <= 3.2589 + 1e-3
+    assert all(any(abs(x - A) < 2e-3 for A in AS) for x in chain)  # the lemma is checked at each

@@ -389,6 +389,41 @@ def test_relinearize_mixed_size_primes_matches_oracle():
     assert np.array_equal(out.numpy(), O.relinearize(qb, eb, quad, key))
 
 
+def test_largest_allowed_primes_match_oracle():
+    """Primes just below the FP64-exact bound 2^50 + 2^46 (the largest the context accepts, where the
+    lazily reduced butterflies and products run closest to their limits): NTT round trip, negacyclic
+    product, mult + relinearize + rescale and rotate == oracle at N=4096."""
+    n, step, top = 4096, 8192, (1 << 50) + (1 << 46)
+    q, cand = [], (top - 1) // step * step + 1
+    while len(q) < 6:
+        if PR.is_prime(cand):
+            q.append(cand)
+        cand -= step
+    g = [PR.smallest_primitive_root(x) for x in q]
+    ctx = G.Context(poly_degree=n, primes=q[:4], roots=g[:4], aux_primes=q[4:], aux_roots=g[4:])
+    qb = O.Basis(n, q[:4], g[:4])
+    eb = O.Basis(n, q, g)
+    rs = np.random.default_rng(46)
+    a, b = rand_poly(qb.moduli, n, rs), rand_poly(qb.moduli, n, rs)
+    prod = empty((4, n))
+    ctx.poly_multiply_ntt(dev(a), dev(b), prod, 4)
+    assert np.array_equal(prod.numpy(), O.rns_multiply_ntt(qb, a, b))
+    D = ctx.num_decomp_digits(4)
+    key = np.stack([rand_poly(eb.moduli, n, rs, 2) for _ in range(D)])
+    dk = ctx.ksk_create(key, 3, 2)
+    lhs, rhs = rand_poly(qb.moduli, n, rs, 2), rand_poly(qb.moduli, n, rs, 2)
+    out = empty((2, 3, n))
+    ctx.mult_relinearized(dev(lhs), dev(rhs), 4, 40, 40, dk, out)
+    want = O.rescale(qb, O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs, rhs), key))
+    assert np.array_equal(out.numpy(), want)
+    gel = O.galois_element_for_rotation(n, 7)
+    drk = ctx.ksk_create(key, 3, 2, gel)
+    ct = rand_poly(qb.moduli, n, rs, 2)
+    rot = empty((2, 4, n))
+    ctx.rotate(dev(ct), 4, 40, drk, rot)
+    assert np.array_equal(rot.numpy(), O.rotate(qb, eb, ct, gel, key))
+
+
 def test_context_rejects_invalid_chains():
     """RnsContext / NttPlan::new validation (math/ntt.rs:17-60) with the crate's error texts, and the
     modulus bound of the FP64-exact arithmetic (2^50 + 2^46)."""
