@@ -44,10 +44,19 @@ struct BatchGroup {
 
 }  // namespace
 
+// Independent groups are spread over up to kLanes capture lanes: each lane has its own stream and
+// scratch clone (workspace), and the graph forks into the lanes after its first node and joins at
+// the end, so small groups (few ciphertexts, short launches) run side by side and fill the GPU.
+constexpr int kLanes = 4;
+
 struct rck_batch {
     rck_context* ctx = nullptr;
-    Context* sc = nullptr;  // scratch clone: own workspace, shared tables
+    Context* sc = nullptr;  // scratch clone: own workspace, shared tables (tuning, eager runs)
     hipStream_t stream = nullptr;
+    std::vector<Context*> lane_sc;         // per capture lane: scratch clone
+    std::vector<hipStream_t> lane_stream;  // per capture lane: stream
+    std::vector<std::vector<int>> lane_groups;
+    std::vector<hipEvent_t> events;  // fork + one join per lane (capture dependencies only)
     std::vector<BatchOp> ops;
     std::vector<BatchGroup> groups;
     hipGraph_t graph = nullptr;
@@ -70,7 +79,7 @@ static int bguard(F&& f) {
     }
 }
 
-static void run_group(rck_batch* bt, const BatchGroup& g, hipStream_t s, KsShape shape) {
+static void run_group(rck_batch* bt, const BatchGroup& g, hipStream_t s, KsShape shape, Context* c = nullptr) {
     std::vector<const uint64_t*> as, bs;
     std::vector<uint64_t*> outs;
     for (int i : g.ops) {
@@ -78,7 +87,65 @@ static void run_group(rck_batch* bt, const BatchGroup& g, hipStream_t s, KsShape
         bs.push_back(bt->ops[i].b);
         outs.push_back(bt->ops[i].out);
     }
-    run_op_group(*bt->sc, g.kind, (int)g.limbs, g.key, as, bs, outs, s, shape);
+    run_op_group(c ? *c : *bt->sc, g.kind, (int)g.limbs, g.key, as, bs, outs, s, shape);
+}
+
+static bool has_keyswitch(int kind) { return kind != RCK_OP_MULT_PLAIN; }
+
+// Work estimate of a group for the lane assignment: keyswitch digits x eval limbs x ciphertexts
+// (the digit NTTs and key MAC dominate), or limbs x ciphertexts for a plaintext product.
+static double group_cost(const Context& c, const BatchGroup& g) {
+    const double B = (double)g.ops.size();
+    if (!has_keyswitch(g.kind)) return 0.1 * B * g.limbs;
+    const int E = (int)g.limbs + (int)g.key->aux;
+    const int D = g.key->aux ? (int)((c.qbits[g.limbs] + 29) / 30) : (int)g.key->digits;
+    return B * D * E + 0.05 * D * E * 2;  // + the key stream, read once per group
+}
+
+// Longest-processing-time assignment of the groups to min(kLanes, groups) lanes.
+static void assign_lanes(rck_batch* bt) {
+    const int nl = std::max(1, std::min<int>(kLanes, (int)bt->groups.size()));
+    std::vector<int> order(bt->groups.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::vector<double> cost(bt->groups.size());
+    for (size_t i = 0; i < cost.size(); i++) cost[i] = group_cost(*bt->sc, bt->groups[i]);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    bt->lane_groups.assign(nl, {});
+    std::vector<double> load(nl, 0.0);
+    for (int gi : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        bt->lane_groups[l].push_back(gi);
+        load[l] += cost[gi];
+    }
+    for (int l = 0; l < nl; l++) {
+        std::sort(bt->lane_groups[l].begin(), bt->lane_groups[l].end());  // submission order within a lane
+        bt->lane_sc.push_back(bt->sc->scratch_clone());
+        hipStream_t st;
+        RCK_CHECK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        bt->lane_stream.push_back(st);
+    }
+    for (int e = 0; e <= nl; e++) {
+        hipEvent_t ev;
+        RCK_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        bt->events.push_back(ev);
+    }
+}
+
+// Every lane runs its groups on its own stream with its own scratch clone; the lanes start after
+// the work already queued on `s` and `s` continues after all of them (as graph dependencies when
+// `s` is capturing).
+static void run_lanes(rck_batch* bt, hipStream_t s) {
+    const int nl = (int)bt->lane_stream.size();
+    RCK_CHECK_HIP(hipEventRecord(bt->events[0], s));
+    for (int l = 0; l < nl; l++) {
+        RCK_CHECK_HIP(hipStreamWaitEvent(bt->lane_stream[l], bt->events[0], 0));
+        for (int gi : bt->lane_groups[l]) {
+            const BatchGroup& g = bt->groups[gi];
+            run_group(bt, g, bt->lane_stream[l], g.shape, bt->lane_sc[l]);
+        }
+        RCK_CHECK_HIP(hipEventRecord(bt->events[1 + l], bt->lane_stream[l]));
+        RCK_CHECK_HIP(hipStreamWaitEvent(s, bt->events[1 + l], 0));
+    }
 }
 
 static void check_group(const Context& c, const BatchGroup& g) {
@@ -98,8 +165,6 @@ static std::string shape_json(const KsShape& sh, float ms) {
       << "}";
     return o.str();
 }
-
-static bool has_keyswitch(int kind) { return kind != RCK_OP_MULT_PLAIN; }
 
 static void autotune(rck_batch* bt, BatchGroup& g) {
     Context& c = *bt->sc;
@@ -292,14 +357,16 @@ int rck_batch_compile(rck_batch* bt, int tune) {
         try {
             for (auto& g : bt->groups)
                 if (tune && has_keyswitch(g.kind)) autotune(bt, g);
-            // eager run: sizes the scratch and leaves every output computed
-            for (auto& g : bt->groups) run_group(bt, g, bt->stream, g.shape);
+            assign_lanes(bt);
+            // eager run: sizes every lane's scratch (streams, events, workspace) and leaves every
+            // output computed
+            run_lanes(bt, bt->stream);
             RCK_CHECK_HIP(hipStreamSynchronize(bt->stream));
             auto t0 = std::chrono::steady_clock::now();
-            c.frozen = true;
+            for (Context* lc : bt->lane_sc) lc->frozen = true;
             RCK_CHECK_HIP(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
             try {
-                for (auto& g : bt->groups) run_group(bt, g, bt->stream, g.shape);
+                run_lanes(bt, bt->stream);
             } catch (...) {
                 hipGraph_t tmp;
                 (void)hipStreamEndCapture(bt->stream, &tmp);
@@ -308,11 +375,11 @@ int rck_batch_compile(rck_batch* bt, int tune) {
             }
             RCK_CHECK_HIP(hipStreamEndCapture(bt->stream, &bt->graph));
             RCK_CHECK_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
-            c.frozen = false;
+            for (Context* lc : bt->lane_sc) lc->frozen = false;
             bt->capture_ms =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         } catch (...) {
-            c.frozen = false;
+            for (Context* lc : bt->lane_sc) lc->frozen = false;
             prof_set(prof);
             throw;
         }
@@ -367,8 +434,12 @@ int rck_batch_destroy(rck_batch* bt) {
     return bguard([&] {
         if (!bt) return;
         if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+        for (hipStream_t st : bt->lane_stream) (void)hipStreamSynchronize(st);
         if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
         if (bt->graph) (void)hipGraphDestroy(bt->graph);
+        for (hipEvent_t ev : bt->events) (void)hipEventDestroy(ev);
+        for (hipStream_t st : bt->lane_stream) (void)hipStreamDestroy(st);
+        for (Context* lc : bt->lane_sc) delete lc;
         delete bt->sc;
         if (bt->stream) (void)hipStreamDestroy(bt->stream);
         delete bt;
