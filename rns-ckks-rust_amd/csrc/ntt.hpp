@@ -128,14 +128,30 @@ __device__ __forceinline__ void fwd_phase_b(double (&v)[16], const double* __res
     reduce_all(v, q, qinv);
 }
 
-// Inverse phase B: stages LOGR-1 .. 4 (reduce every two stages: the sums double each stage).
+// Inverse Gentleman-Sande stages on 16 registers take the pair distances 1, 2, 4, 8 in that order.
+// From reduced inputs (|v| <= 0.51q) a sum chain doubles per stage and a product is
+// <= (0.5 + 0.133 |u - v| / q) q (modarith.hpp), so only two sets of values need reducing for every
+// product input to stay <= 3.75q: before distance 4 the values v[e], e % 4 == 0 (each the sum of
+// four inputs, <= 2q), and before distance 8 the values v[e], e % 8 == 1 (<= 2.27q); every value
+// then stays <= 3.07q, and the phase ends with one full reduction (outputs reduced, as before).
+__device__ __forceinline__ void gs_lazy_reduce(double (&v)[16], int half, double q, double qinv) {
+    if (half == 4) {
+#pragma unroll
+        for (int e = 0; e < 16; e += 4) v[e] = reduce(v[e], q, qinv);
+    } else if (half == 8) {
+        v[1] = reduce(v[1], q, qinv);
+        v[9] = reduce(v[9], q, qinv);
+    }
+}
+
+// Inverse phase B: stages LOGR-1 .. 4 (pair distances 1, 2, .. in increasing order).
 template <int LOGR, int MODE = kTwGlobal>
 __device__ __forceinline__ void inv_phase_b(double (&v)[16], const double* __restrict__ tbl, int base, int tau,
                                             double q, double qinv) {
-    int cnt = 0;
 #pragma unroll
     for (int s = LOGR - 1; s >= 4; s--) {
         const int half = (1 << LOGR) >> (s + 1);
+        gs_lazy_reduce(v, half, q, qinv);
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) {
@@ -143,21 +159,19 @@ __device__ __forceinline__ void inv_phase_b(double (&v)[16], const double* __res
                 gs_bfly(v[e], v[e + half], w, q, qinv);
             }
         }
-        if (++cnt == 2) {
-            reduce_all(v, q, qinv);
-            cnt = 0;
-        }
     }
-    if (cnt) reduce_all(v, q, qinv);
+    reduce_all(v, q, qinv);
 }
 
-// Inverse phase A: stages 3 .. 0 on x = tau + (R/16) e.
+// Inverse phase A: stages 3 .. 0 on x = tau + (R/16) e (pair distances 1, 2, 4, 8; reductions as in
+// inv_phase_b).
 template <int MODE = kTwGlobal>
 __device__ __forceinline__ void inv_phase_a(double (&v)[16], const double* __restrict__ tbl, int base, double q,
                                             double qinv) {
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
         const int half = 8 >> s;
+        gs_lazy_reduce(v, half, q, qinv);
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             if ((e & half) == 0) {
@@ -165,8 +179,8 @@ __device__ __forceinline__ void inv_phase_a(double (&v)[16], const double* __res
                 gs_bfly(v[e], v[e + half], w, q, qinv);
             }
         }
-        if (s == 2 || s == 0) reduce_all(v, q, qinv);
     }
+    reduce_all(v, q, qinv);
 }
 
 // Operand addressing for a batch of limbs: element (z, y, idx) at base + z*ps + y*ls + idx.
