@@ -433,10 +433,16 @@ def mixed_main(args, G, PR):
             b = G.DeviceBuffer.from_numpy(uniform(p.primes[:limbs], n, rs, 1)[0])
             bt.add(G.OP_MULT_PLAIN, limbs, a, b, None, out)
         bufs.append((kind, limbs, step, a, b, out))
+    if args.tune_table and os.path.exists(args.tune_table):
+        with open(args.tune_table) as f:
+            log(f"[bench] tuning table {args.tune_table}: {G.tune_table_import(f.read())} entries loaded")
     t = time.perf_counter()
     bt.compile(tune=not args.no_tune)
     rep = bt.report()
     log(f"[bench] compiled {len(rep['groups'])} groups in {time.perf_counter() - t:.1f} s")
+    if args.tune_table:
+        with open(args.tune_table, "w") as f:
+            f.write(G.tune_table_export())
     stream = G.Stream()
     for _ in range(args.warmup):
         bt.run(stream)
