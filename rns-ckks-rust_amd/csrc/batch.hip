@@ -45,16 +45,17 @@ struct BatchGroup {
 }  // namespace
 
 // Independent groups are spread over up to kLanes capture lanes: each lane has its own stream and
-// scratch clone (workspace), and the graph forks into the lanes after its first node and joins at
-// the end, so small groups (few ciphertexts, short launches) run side by side and fill the GPU.
+// scratch clone (workspace; lane 0 uses the batch's own), and the graph forks into the lanes at its
+// start and joins at the end, so small groups (few ciphertexts, short launches) run side by side and
+// fill the GPU.
 constexpr int kLanes = 4;
 
 struct rck_batch {
     rck_context* ctx = nullptr;
     Context* sc = nullptr;  // scratch clone: own workspace, shared tables (tuning, eager runs)
     hipStream_t stream = nullptr;
-    std::vector<Context*> lane_sc;         // per capture lane: scratch clone
-    std::vector<hipStream_t> lane_stream;  // per capture lane: stream
+    std::vector<Context*> lane_sc;         // per capture lane: scratch clone (lane 0: sc itself)
+    std::vector<hipStream_t> lane_stream;  // per capture lane: stream (lane 0: stream itself)
     std::vector<std::vector<int>> lane_groups;
     std::vector<hipEvent_t> events;  // fork + one join per lane (capture dependencies only)
     std::vector<BatchOp> ops;
@@ -119,12 +120,17 @@ static void assign_lanes(rck_batch* bt) {
     }
     for (int l = 0; l < nl; l++) {
         std::sort(bt->lane_groups[l].begin(), bt->lane_groups[l].end());  // submission order within a lane
+        if (l == 0) {  // lane 0 runs on the batch's own stream and scratch
+            bt->lane_sc.push_back(bt->sc);
+            bt->lane_stream.push_back(bt->stream);
+            continue;
+        }
         bt->lane_sc.push_back(bt->sc->scratch_clone());
         hipStream_t st;
         RCK_CHECK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         bt->lane_stream.push_back(st);
     }
-    for (int e = 0; e <= nl; e++) {
+    for (int e = 0; e < nl; e++) {  // the fork, then one join per lane 1 .. nl-1
         hipEvent_t ev;
         RCK_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         bt->events.push_back(ev);
@@ -136,16 +142,17 @@ static void assign_lanes(rck_batch* bt) {
 // `s` is capturing).
 static void run_lanes(rck_batch* bt, hipStream_t s) {
     const int nl = (int)bt->lane_stream.size();
-    RCK_CHECK_HIP(hipEventRecord(bt->events[0], s));
-    for (int l = 0; l < nl; l++) {
+    if (nl > 1) RCK_CHECK_HIP(hipEventRecord(bt->events[0], s));
+    for (int l = 1; l < nl; l++) {
         RCK_CHECK_HIP(hipStreamWaitEvent(bt->lane_stream[l], bt->events[0], 0));
         for (int gi : bt->lane_groups[l]) {
             const BatchGroup& g = bt->groups[gi];
             run_group(bt, g, bt->lane_stream[l], g.shape, bt->lane_sc[l]);
         }
-        RCK_CHECK_HIP(hipEventRecord(bt->events[1 + l], bt->lane_stream[l]));
-        RCK_CHECK_HIP(hipStreamWaitEvent(s, bt->events[1 + l], 0));
+        RCK_CHECK_HIP(hipEventRecord(bt->events[l], bt->lane_stream[l]));
     }
+    for (int gi : bt->lane_groups[0]) run_group(bt, bt->groups[gi], s, bt->groups[gi].shape, bt->lane_sc[0]);
+    for (int l = 1; l < nl; l++) RCK_CHECK_HIP(hipStreamWaitEvent(s, bt->events[l], 0));
 }
 
 static void check_group(const Context& c, const BatchGroup& g) {
@@ -434,12 +441,12 @@ int rck_batch_destroy(rck_batch* bt) {
     return bguard([&] {
         if (!bt) return;
         if (bt->stream) (void)hipStreamSynchronize(bt->stream);
-        for (hipStream_t st : bt->lane_stream) (void)hipStreamSynchronize(st);
+        for (size_t l = 1; l < bt->lane_stream.size(); l++) (void)hipStreamSynchronize(bt->lane_stream[l]);
         if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
         if (bt->graph) (void)hipGraphDestroy(bt->graph);
         for (hipEvent_t ev : bt->events) (void)hipEventDestroy(ev);
-        for (hipStream_t st : bt->lane_stream) (void)hipStreamDestroy(st);
-        for (Context* lc : bt->lane_sc) delete lc;
+        for (size_t l = 1; l < bt->lane_stream.size(); l++) (void)hipStreamDestroy(bt->lane_stream[l]);
+        for (size_t l = 1; l < bt->lane_sc.size(); l++) delete bt->lane_sc[l];
         delete bt->sc;
         if (bt->stream) (void)hipStreamDestroy(bt->stream);
         delete bt;
