@@ -48,8 +48,9 @@ uint32_t bsgs_baby_step(uint32_t dim) {
 // per coefficient (l, x) a (giants x bs) by (bs x vz) product mod q_l.  Lanes run over x (coalesced
 // loads and stores); each thread owns a TG x TV register tile of (giant, vz) outputs, so every loaded
 // diagonal value feeds TV products and every baby value TG products.  Absent diagonals are stored as
-// zeros and contribute exact zeros.  Each product is reduced (|.| <= 0.97q) and the running sums
-// every second term (|sum| <= 2.4q < 2^53 for q < 2^51: every addition stays exact).
+// zeros and contribute exact zeros.  Both operands are reduced (NTT outputs, |.| <= q/2 + 1), so
+// each product is <= 0.57q (modarith.hpp: 0.5 + 0.133 A at A = 0.5) and the running sums are
+// reduced every eighth term (|sum| <= 0.5q + 8 * 0.57q < 5.1q < 2^53 / q: every addition stays exact).
 template <int TG, int TV>
 __global__ __launch_bounds__(256) void k_diag_mac(const double* __restrict__ baby, const double* __restrict__ diag,
                                                   double* __restrict__ inner, int n, int limbs, int vz_count,
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(256) void k_diag_mac(const double* __restrict__ bab
         for (int t = 0; t < TG; t++)
 #pragma unroll
             for (int u = 0; u < TV; u++) acc[t][u] += mulmod(d[t], w[u], P.q, P.qinv);
-        if (b & 1) {
+        if ((b & 7) == 7) {
 #pragma unroll
             for (int t = 0; t < TG; t++)
 #pragma unroll

@@ -453,7 +453,9 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 16) void k_digit_mac(const doubl
                 }
             }
         }
-        if (((i - i0) & 3) == 3) {
+        // reduced rows times reduced keys: products <= 0.57q, so eight of them on a reduced
+        // accumulator stay below 5.1q < 2^53 / q
+        if (((i - i0) & 7) == 7) {
             reduce_all(a0, q, qinv);
             reduce_all(a1, q, qinv);
         }
@@ -832,7 +834,7 @@ __global__ __launch_bounds__(ROWS*(1 << LOGR) / 8) void k_digit_mac8(const doubl
             a0[e] += mulmod(v[e], k0[e * T16], q, qinv);
             a1[e] += mulmod(v[e], k1[e * T16], q, qinv);
         }
-        if (((i - i0) & 3) == 3) {
+        if (((i - i0) & 7) == 7) {  // products <= 0.57q (reduced operands): eight per reduction
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 a0[e] = reduce(a0[e], q, qinv);
@@ -946,7 +948,7 @@ __global__ void k_mac_small(const double* __restrict__ dn, int D, const double* 
         const double v = dn[((long long)cb * D + d) * EN + (long long)y * n + i];
         s0 += mulmod(v, key[(2LL * d) * EN + (long long)y * n + i], P.q, P.qinv);
         s1 += mulmod(v, key[(2LL * d + 1) * EN + (long long)y * n + i], P.q, P.qinv);
-        if ((d & 3) == 3) {
+        if ((d & 7) == 7) {  // products <= 0.57q (reduced operands): eight per reduction
             s0 = reduce(s0, P.q, P.qinv);
             s1 = reduce(s1, P.q, P.qinv);
         }
