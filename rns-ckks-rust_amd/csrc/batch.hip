@@ -103,8 +103,22 @@ static double group_cost(const Context& c, const BatchGroup& g) {
     return B * D * E + 0.05 * D * E * 2;  // + the key stream, read once per group
 }
 
-// Longest-processing-time assignment of the groups to min(kLanes, groups) lanes.
+// Releases the lanes' own streams, scratch clones and events (lane 0 borrows the batch's).
+static void release_lanes(rck_batch* bt) {
+    for (size_t l = 1; l < bt->lane_stream.size(); l++) (void)hipStreamSynchronize(bt->lane_stream[l]);
+    for (hipEvent_t ev : bt->events) (void)hipEventDestroy(ev);
+    for (size_t l = 1; l < bt->lane_stream.size(); l++) (void)hipStreamDestroy(bt->lane_stream[l]);
+    for (size_t l = 1; l < bt->lane_sc.size(); l++) delete bt->lane_sc[l];
+    bt->events.clear();
+    bt->lane_stream.clear();
+    bt->lane_sc.clear();
+    bt->lane_groups.clear();
+}
+
+// Longest-processing-time assignment of the groups to min(kLanes, groups) lanes (a compile retried
+// after a failure starts from scratch).
 static void assign_lanes(rck_batch* bt) {
+    release_lanes(bt);
     const int nl = std::max(1, std::min<int>(kLanes, (int)bt->groups.size()));
     std::vector<int> order(bt->groups.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
@@ -370,6 +384,10 @@ int rck_batch_compile(rck_batch* bt, int tune) {
             run_lanes(bt, bt->stream);
             RCK_CHECK_HIP(hipStreamSynchronize(bt->stream));
             auto t0 = std::chrono::steady_clock::now();
+            if (bt->exec) (void)hipGraphExecDestroy(bt->exec);  // left by a failed earlier compile
+            if (bt->graph) (void)hipGraphDestroy(bt->graph);
+            bt->exec = nullptr;
+            bt->graph = nullptr;
             for (Context* lc : bt->lane_sc) lc->frozen = true;
             RCK_CHECK_HIP(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
             try {
@@ -444,9 +462,7 @@ int rck_batch_destroy(rck_batch* bt) {
         for (size_t l = 1; l < bt->lane_stream.size(); l++) (void)hipStreamSynchronize(bt->lane_stream[l]);
         if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
         if (bt->graph) (void)hipGraphDestroy(bt->graph);
-        for (hipEvent_t ev : bt->events) (void)hipEventDestroy(ev);
-        for (size_t l = 1; l < bt->lane_stream.size(); l++) (void)hipStreamDestroy(bt->lane_stream[l]);
-        for (size_t l = 1; l < bt->lane_sc.size(); l++) delete bt->lane_sc[l];
+        release_lanes(bt);
         delete bt->sc;
         if (bt->stream) (void)hipStreamDestroy(bt->stream);
         delete bt;
