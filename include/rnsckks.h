@@ -62,7 +62,9 @@ int rck_event_elapsed_ms(void* start, void* stop, float* ms);
  * Replaces RnsCkksContext::new(RnsCkksParams) (src/rns/context/construction.rs:9-30) with its
  * validation (src/rns/context/params.rs:48-73, src/rns/basis.rs:32-59, src/math/ntt.rs:17-60;
  * same error messages).  primes/roots: the Q chain (RnsPrime{modulus, primitive_root}),
- * aux_primes/aux_roots: the P chain (may be empty: Q-only keyswitching). */
+ * aux_primes/aux_roots: the P chain (may be empty: Q-only keyswitching).  Beyond the crate's
+ * checks, every modulus must lie below 2^50 + 2^46 (the device's FP64-exact modular products;
+ * RCK_ERR_INVALID "moduli must be below 2^50 + 2^46 (FP64-exact arithmetic)"). */
 int rck_context_create(uint32_t poly_degree, const uint64_t* primes, const uint64_t* roots, uint32_t n_primes,
                        const uint64_t* aux_primes, const uint64_t* aux_roots, uint32_t n_aux, uint32_t scale_bits,
                        rck_context** out);
