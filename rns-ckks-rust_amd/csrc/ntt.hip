@@ -164,12 +164,13 @@ __device__ __forceinline__ void row_inv_store(double (&v)[16], double* p, double
 }
 
 // FOLD: d0 and d1 are stored in NTT form (NTT-domain row order) for a keyswitch that folds them into
-// its accumulators; only d2 gets the inverse row stages.
-template <int LOGR, bool FOLD>
-__global__ __launch_bounds__(256) void k_mult_rows(const double* __restrict__ xl, const double* __restrict__ xr,
-                                                   double* __restrict__ d, long long P, int logn, PrimeMap pm,
-                                                   const DevPrime* __restrict__ primes) {
-    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+// its accumulators; only d2 gets the inverse row stages.  NT threads per workgroup (NT / (R/16) rows):
+// 128 for small batches, so that the grid still covers every CU.
+template <int LOGR, bool FOLD, int NT>
+__global__ __launch_bounds__(NT) void k_mult_rows(const double* __restrict__ xl, const double* __restrict__ xr,
+                                                  double* __restrict__ d, long long P, int logn, PrimeMap pm,
+                                                  const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = NT / TPS;
     __shared__ double lds[S * row_pitch<R>()];
     const int n = 1 << logn, N1 = n >> LOGR;
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
@@ -365,17 +366,29 @@ static void inv_cols_t(const Context& c, Batch<const double> in, int out_kind, B
 
 static bool small_ring(const Context& c) { return c.logn <= 11; }
 
+template <int LOGR, int NT>
+static void mult_rows_nt(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B,
+                         PrimeMap pm, hipStream_t s, bool fold) {
+    constexpr int S = NT / ((1 << LOGR) / 16);
+    const int n1 = 1 << (c.logn - LOGR);
+    const dim3 grid(n1 / S, limbs, B);
+    if (fold)
+        k_mult_rows<LOGR, true, NT><<<grid, NT, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
+    else
+        k_mult_rows<LOGR, false, NT><<<grid, NT, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
+}
+
 template <int LOGR>
 static void mult_rows_t(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
                         hipStream_t s, bool fold) {
     constexpr int S = 256 / ((1 << LOGR) / 16);
     const int n1 = 1 << (c.logn - LOGR);
     ProfScope ps(s, "mult_rows");
-    const dim3 grid(n1 / S, limbs, B);
-    if (fold)
-        k_mult_rows<LOGR, true><<<grid, 256, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
+    // two 256-thread workgroups fit a CU (registers): below 4 per CU, halve the workgroups instead
+    if ((long long)(n1 / S) * limbs * B < 4LL * 256)
+        mult_rows_nt<LOGR, 128>(c, xl, xr, d, limbs, B, pm, s, fold);
     else
-        k_mult_rows<LOGR, false><<<grid, 256, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
+        mult_rows_nt<LOGR, 256>(c, xl, xr, d, limbs, B, pm, s, fold);
     RCK_LAUNCH_CHECK();
 }
 
