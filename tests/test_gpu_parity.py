@@ -313,6 +313,38 @@ def test_config2_full_size_mult_relin_bit_exact():
             assert np.array_equal(got[comp, l], want[comp, l]), f"c{comp} limb {l}"
 
 
+def test_config2_full_size_ntt_and_rotate_bit_exact():
+    """The rest of the config-2 run at full size (N=2^16, 24 Q + 2 P primes): the crate's cyclic
+    forward NTT of all 26 limbs (NttPlan::forward per limb) and its inverse, and a rotation by one
+    slot with a seeded uniform key, limb by limb against the oracle."""
+    params = PR.config_per_op_gpu()
+    n, L = params.poly_degree, 24
+    ctx = G.Context(params)
+    qb = O.Basis(n, params.primes, params.roots)
+    eb = O.Basis(n, params.primes + params.aux_primes, params.roots + params.aux_roots)
+    rs = np.random.default_rng(24)
+    x = rand_poly(eb.moduli, n, rs)
+    v = dev(x)
+    ctx.ntt_forward(v, L, 2)
+    got = v.numpy()
+    for l in range(L + 2):
+        assert np.array_equal(got[l], O.ntt_forward(eb.plan(l), x[l])), f"limb {l}"
+    ctx.ntt_inverse(v, L, 2)
+    assert np.array_equal(v.numpy(), x)
+    D = ctx.num_decomp_digits(L)
+    key = np.stack([rand_poly(eb.moduli, n, rs, 2) for _ in range(D)])
+    gel = O.galois_element_for_rotation(n, 1)
+    drk = ctx.ksk_create(key, L - 1, 2, gel)
+    ct = rand_poly(qb.moduli, n, rs, 2)
+    out = empty((2, L, n))
+    ctx.rotate(dev(ct), L, 40, drk, out)
+    want = O.rotate(qb, eb, ct, gel, key)
+    got = out.numpy()
+    for comp in range(2):
+        for l in range(L):
+            assert np.array_equal(got[comp, l], want[comp, l]), f"c{comp} limb {l}"
+
+
 @pytest.mark.parametrize("n,L,K", [(4096, 5, 1), (4096, 4, 3), (1 << 14, 5, 4), (1 << 13, 6, 1)])
 def test_hybrid_aux_prime_counts_match_oracle(n, L, K):
     """Hybrid relinearize, rotate and mult_relinearized with K = 1, 3, 4 aux primes (the mod-down's
