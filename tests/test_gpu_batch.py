@@ -182,3 +182,49 @@ def test_headline_workload_batched_matches_per_op_and_oracle():
         got = out.numpy()
         for b in range(B):
             assert np.array_equal(got[b], want[b]), (pf, b)
+
+
+def test_groups_larger_than_32_match_per_op():
+    """Groups of 40 mult+relin, 36 rotations (one Galois element) and 34 mult+relin+rescale at one
+    level, scattered operands: the mod-down epilogue takes ciphertexts in chunks of 32, so every
+    group spans two chunks."""
+    n, limbs = 1 << 12, 3
+    p = PR.config_batched(n=1 << 15, limbs=8, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    rs = np.random.default_rng(23)
+    moduli = p.primes[:limbs] + p.aux_primes
+    D = ctx.num_decomp_digits(limbs)
+    gel = G.galois_element_for_rotation(n, -5)
+    rkey = np.stack([uniform(moduli, n, rs, 2) for _ in range(D)])
+    kr = ctx.ksk_create(np.stack([uniform(moduli, n, rs, 2) for _ in range(D)]), limbs - 1, 2)
+    kg = ctx.ksk_create(rkey, limbs - 1, 2, gel)
+    bt = G.Batch(ctx)
+    ops = []
+    for kind, count in (("mr", 40), ("rot", 36), ("mrr", 34)):
+        for _ in range(count):
+            a = dev(uniform(p.primes[:limbs], n, rs, 2))
+            b = dev(uniform(p.primes[:limbs], n, rs, 2)) if kind != "rot" else None
+            out = G.DeviceBuffer((2, limbs - 1 if kind == "mrr" else limbs, n))
+            op = {"mr": G.OP_MULT_RELIN, "rot": G.OP_ROTATE, "mrr": G.OP_MULT_RELIN_RESCALE}[kind]
+            bt.add(op, limbs, a, b, kg if kind == "rot" else kr, out)
+            ops.append((kind, a, b, out))
+    bt.compile(tune=False)
+    rep = bt.report()
+    assert sorted(g["ops"] for g in rep["groups"]) == [34, 36, 40]
+    got = [o[3].numpy() for o in ops]
+    quad = G.DeviceBuffer((3, limbs, n))
+    for i, (kind, a, b, out) in enumerate(ops):
+        ref = G.DeviceBuffer(out.shape)
+        if kind == "rot":
+            ctx.rotate(a, limbs, 40, kg, ref)
+        elif kind == "mr":
+            ctx.mult(a, b, limbs, 40, 40, quad)
+            ctx.relinearize(quad, limbs, 80, kr, ref)
+        else:
+            ctx.mult_relinearized(a, b, limbs, 40, 40, kr, ref)
+        assert np.array_equal(ref.numpy(), got[i]), f"op {i} ({kind})"
+    # the last rotation of the second chunk against the oracle
+    qb = O.Basis(n, p.primes[:limbs], p.roots[:limbs])
+    eb = O.Basis(n, moduli, p.roots[:limbs] + p.aux_roots)
+    kind, a, _, out = ops[40 + 35]
+    assert np.array_equal(got[40 + 35], O.rotate(qb, eb, a.numpy(), gel, rkey))
