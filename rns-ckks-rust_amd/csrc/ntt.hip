@@ -20,7 +20,8 @@ __device__ __forceinline__ double load_in(const void* p, long long idx, double q
 // ---------------------------------------------------------------------------------------
 template <int LOGR, int IN>
 __global__ __launch_bounds__(256) void k_fwd_cols(Batch<const void> in, int in_elem, Batch<double> out, int logn,
-                                                  PrimeMap pm, const DevPrime* __restrict__ primes) {
+                                                  PrimeMap pm, const DevPrime* __restrict__ primes,
+                                                  const void* in2, int zsplit) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[R * S];
     const int N2 = 1 << (logn - LOGR);
@@ -29,8 +30,10 @@ __global__ __launch_bounds__(256) void k_fwd_cols(Batch<const void> in, int in_e
     const int y = blockIdx.y, z = blockIdx.z;
     const DevPrime P = primes[pm(y)];
     const double q = P.q, qinv = P.qinv;
-    const char* src = static_cast<const char*>(in.base) +
-                      ((long long)z * in.ps + (long long)y * in.ls) * in_elem;
+    // polys z >= zsplit come from in2 (same strides): both operands of a product in one launch
+    const bool second = in2 != nullptr && z >= zsplit;
+    const char* src = static_cast<const char*>(second ? in2 : in.base) +
+                      ((long long)(second ? z - zsplit : z) * in.ps + (long long)y * in.ls) * in_elem;
     double v[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = load_in<IN>(src, r + (long long)N2 * (tau + TPS * e), q);
@@ -316,17 +319,17 @@ static void check_dims(const Context& c, int limbs, int polys) {
 
 template <int LOGR>
 static void fwd_cols_t(const Context& c, int in_kind, Batch<const void> in, Batch<double> out, int limbs, int polys,
-                       PrimeMap pm, hipStream_t s) {
+                       PrimeMap pm, hipStream_t s, const void* in2, int zsplit) {
     constexpr int S = 256 / ((1 << LOGR) / 16);
     const int n2 = 1 << (c.logn - LOGR);
     dim3 grid(n2 / S, limbs, polys);
     ProfScope ps(s, "ntt_fwd_cols");
     if (in_kind == kInU64)
-        k_fwd_cols<LOGR, kInU64><<<grid, 256, 0, s>>>(in, 8, out, c.logn, pm, c.d_primes);
+        k_fwd_cols<LOGR, kInU64><<<grid, 256, 0, s>>>(in, 8, out, c.logn, pm, c.d_primes, in2, zsplit);
     else if (in_kind == kInI32)
-        k_fwd_cols<LOGR, kInI32><<<grid, 256, 0, s>>>(in, 4, out, c.logn, pm, c.d_primes);
+        k_fwd_cols<LOGR, kInI32><<<grid, 256, 0, s>>>(in, 4, out, c.logn, pm, c.d_primes, in2, zsplit);
     else
-        k_fwd_cols<LOGR, kInF64><<<grid, 256, 0, s>>>(in, 8, out, c.logn, pm, c.d_primes);
+        k_fwd_cols<LOGR, kInF64><<<grid, 256, 0, s>>>(in, 8, out, c.logn, pm, c.d_primes, in2, zsplit);
     RCK_LAUNCH_CHECK();
 }
 
@@ -407,17 +410,19 @@ void launch_mult_rows(const Context& c, const double* xl, const double* xr, doub
 }
 
 void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
-                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s,
+                     const void* in2, int zsplit) {
     check_dims(c, limbs, polys);
+    RCK_REQUIRE(in2 == nullptr || (zsplit > 0 && zsplit < polys), "bad second operand split");
     RCK_REQUIRE(!small_ring(c), "two-pass NTT needs N >= 2^12");
     Batch<const void> bi{in, in_ls, in_ps};
     Batch<double> bo{out, out_ls, out_ps};
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
     switch (n1) {
-        case 6: fwd_cols_t<6>(c, in_kind, bi, bo, limbs, polys, pm, s); break;
-        case 7: fwd_cols_t<7>(c, in_kind, bi, bo, limbs, polys, pm, s); break;
-        case 8: fwd_cols_t<8>(c, in_kind, bi, bo, limbs, polys, pm, s); break;
+        case 6: fwd_cols_t<6>(c, in_kind, bi, bo, limbs, polys, pm, s, in2, zsplit); break;
+        case 7: fwd_cols_t<7>(c, in_kind, bi, bo, limbs, polys, pm, s, in2, zsplit); break;
+        case 8: fwd_cols_t<8>(c, in_kind, bi, bo, limbs, polys, pm, s, in2, zsplit); break;
         default: RCK_REQUIRE(false, "unsupported ring degree");
     }
 }

@@ -223,9 +223,11 @@ __host__ __device__ inline unsigned ntt_phys(unsigned i, int logn) {
 // forward transform of a batch (coefficient form -> NTT form, doubles, reduced)
 void launch_fwd(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
                 long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
-// forward row pass only / column pass only (used by fused pipelines)
+// forward row pass only / column pass only (used by fused pipelines); in2 (nullable): polys
+// zsplit .. polys-1 are read from in2 (poly zsplit + k = in2's poly k, same strides as in)
 void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
-                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+                     long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s,
+                     const void* in2 = nullptr, int zsplit = 0);
 void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
                      hipStream_t s);
 // inverse transform (NTT form doubles -> coefficient form u64 canonical, or reduced doubles)

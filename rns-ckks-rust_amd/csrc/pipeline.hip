@@ -51,8 +51,8 @@ double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long lo
     double* xl = x;
     double* xr = x + (long long)B * 2 * P;
     if (B == 1 || in_ps == 2 * P) {
-        launch_fwd_cols(c, kInU64, lhs, n, P, xl, n, P, limbs, 2 * B, pm, s);
-        launch_fwd_cols(c, kInU64, rhs, n, P, xr, n, P, limbs, 2 * B, pm, s);
+        // one launch for both operands: xr follows xl, so output poly z is x + z * P
+        launch_fwd_cols(c, kInU64, lhs, n, P, xl, n, P, limbs, 4 * B, pm, s, rhs, 2 * B);
     } else {
         for (int b = 0; b < B; b++) {
             launch_fwd_cols(c, kInU64, lhs + b * in_ps, n, P, xl + b * 2 * P, n, P, limbs, 2, pm, s);
