@@ -132,3 +132,15 @@ def test_bsgs_rotation_steps_kats():
     steps = capi.bsgs_rotation_steps(4096)
     assert len(steps) == 126 and steps == OB.rotation_steps(4096)
     assert steps[:63] == list(range(1, 64)) and steps[63:] == [64 * g for g in range(1, 64)]
+
+
+@pytest.mark.parametrize("gpus", [2, 8])
+def test_bench_refuses_more_than_one_gpu(gpus):
+    """bench.py measures one GPU only: --gpus N > 1 exits with status 2 and says so, before
+    anything touches a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", str(gpus)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
