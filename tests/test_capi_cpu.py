@@ -144,3 +144,10 @@ def test_bench_refuses_more_than_one_gpu(gpus):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_rust_binding_covers_every_declared_symbol():
+    """INTEGRATION.md's Rust extern block has one `pub fn` per entry point of include/*.h."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    bound = set(re.findall(r"pub fn (rck_\w+)\(", text))
+    assert declared_symbols() == bound
