@@ -4,6 +4,8 @@ the C-ABI result against the CPU oracle bit for bit.  The shapes cover levels be
 full chain (the key and ciphertext at a prefix of the primes, context/ops.rs:97-122,
 context/rotation.rs:104-135), single-limb ciphertexts and ring degrees drawn from 2^11 (one-pass transform) .. 2^16.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -14,6 +16,7 @@ from rnsckks import params as PR
 from test_gpu_parity import chain, dev, empty, rand_poly
 
 OPS = ("relinearize", "keyswitch", "rotate", "conjugate", "mult_relinearized")
+SEEDS = int(os.environ.get("RCK_SWEEP_SEEDS", "18"))  # a longer sweep: RCK_SWEEP_SEEDS=100
 
 
 def _case(seed):
@@ -26,7 +29,7 @@ def _case(seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("seed", range(18))
+@pytest.mark.parametrize("seed", range(SEEDS))
 def test_seeded_shape_sweep_matches_oracle(seed):
     hybrid, n, L, op, rs = _case(seed)
     if op == "mult_relinearized" and L < 2:
