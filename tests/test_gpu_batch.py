@@ -1,5 +1,7 @@
 """Batched executor: a captured hipGraph over many ciphertexts at uneven levels with mixed
 operations gives exactly the per-op results, on every replay (with eager work in between)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -228,3 +230,67 @@ def test_groups_larger_than_32_match_per_op():
     eb = O.Basis(n, moduli, p.roots[:limbs] + p.aux_roots)
     kind, a, _, out = ops[40 + 35]
     assert np.array_equal(got[40 + 35], O.rotate(qb, eb, a.numpy(), gel, rkey))
+
+
+BATCH_SEEDS = int(os.environ.get("RCK_BATCH_SWEEP_SEEDS", "8"))  # longer: RCK_BATCH_SWEEP_SEEDS=200
+
+
+@pytest.mark.parametrize("seed", range(BATCH_SEEDS))
+def test_seeded_batch_mixes_match_per_op(seed):
+    """Drawn batches: ring degree 2^12 or 2^13, 8-30 ops of all four kinds at 2-8 limbs, rotation
+    steps from +-1 .. +-N/4, tuned or untuned launch shapes; every output equals the per-op call,
+    on the eager run and on two graph replays."""
+    rs = np.random.default_rng(500 + seed)
+    n = 1 << int(rs.integers(12, 14))
+    p = PR.config_batched(n=1 << 15, limbs=8, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    keys = {}
+
+    def key_for(limbs, galois):
+        if (limbs, galois) not in keys:
+            D = ctx.num_decomp_digits(limbs)
+            kd = np.stack([uniform(p.primes[:limbs] + p.aux_primes, n, rs, 2) for _ in range(D)])
+            keys[(limbs, galois)] = ctx.ksk_create(kd, limbs - 1, 2, galois)
+        return keys[(limbs, galois)]
+
+    bt = G.Batch(ctx)
+    ops = []
+    for _ in range(int(rs.integers(8, 31))):
+        kind = int(rs.integers(1, 5))
+        limbs = int(rs.integers(2, 9))
+        a = dev(uniform(p.primes[:limbs], n, rs, 2))
+        b, k = None, None
+        if kind in (G.OP_MULT_RELIN, G.OP_MULT_RELIN_RESCALE):
+            b, k = dev(uniform(p.primes[:limbs], n, rs, 2)), key_for(limbs, 0)
+        elif kind == G.OP_ROTATE:
+            step = int(rs.integers(1, n // 4 + 1)) * (1 if rs.integers(0, 2) else -1)
+            k = key_for(limbs, G.galois_element_for_rotation(n, step))
+        else:
+            b = dev(uniform(p.primes[:limbs], n, rs, 1)[0])
+        out = G.DeviceBuffer((2, limbs - 1 if kind == G.OP_MULT_RELIN_RESCALE else limbs, n))
+        bt.add(kind, limbs, a, b, k, out)
+        ops.append((kind, limbs, a, b, k, out))
+    print(f"[batch sweep] seed={seed} n={n} ops={len(ops)}", flush=True)
+    bt.compile(tune=bool(seed % 2))
+    first = [o[5].numpy() for o in ops]
+    for _ in range(2):
+        for o in ops:
+            o[5].upload(np.zeros(o[5].shape, dtype=np.uint64))
+        bt.run()
+        bt.synchronize()
+        for i, o in enumerate(ops):
+            assert np.array_equal(o[5].numpy(), first[i]), f"replay, op {i}"
+    quad = {}
+    for i, (kind, limbs, a, b, k, out) in enumerate(ops):
+        ref = G.DeviceBuffer(out.shape)
+        if kind == G.OP_MULT_RELIN:
+            q3 = quad.setdefault(limbs, G.DeviceBuffer((3, limbs, n)))
+            ctx.mult(a, b, limbs, 40, 40, q3)
+            ctx.relinearize(q3, limbs, 80, k, ref)
+        elif kind == G.OP_MULT_RELIN_RESCALE:
+            ctx.mult_relinearized(a, b, limbs, 40, 40, k, ref)
+        elif kind == G.OP_ROTATE:
+            ctx.rotate(a, limbs, 40, k, ref)
+        else:
+            ctx.mult_plain(a, b, limbs, 40, ref)
+        assert np.array_equal(ref.numpy(), first[i]), f"op {i} (kind {kind}, {limbs} limbs)"
