@@ -306,7 +306,8 @@ int rck_batch_create(rck_context* ctx, rck_batch** out);
 int rck_batch_add(rck_batch* batch, int op, uint32_t limbs, const uint64_t* in0, const uint64_t* in1,
                   const rck_ksk* key, uint64_t* out);
 /* Plans the groups, optionally autotunes each group's digit-MAC launch shape (tune != 0) by timing
- * candidates on the group's operands, runs the batch once eagerly and captures it in a hipGraph. */
+ * candidates on the group's operands, runs the batch once eagerly and captures it in a hipGraph.
+ * A batch with no operations is rejected (RCK_ERR_INVALID). */
 int rck_batch_compile(rck_batch* batch, int tune);
 /* Launches the captured graph on `stream` (NULL: the batch's own stream). */
 int rck_batch_run(rck_batch* batch, void* stream);

@@ -353,6 +353,7 @@ int rck_batch_add(rck_batch* bt, int op, uint32_t limbs, const uint64_t* in0, co
 int rck_batch_compile(rck_batch* bt, int tune) {
     return bguard([&] {
         RCK_REQUIRE(bt && !bt->compiled, "batch missing or already compiled");
+        RCK_REQUIRE(!bt->ops.empty(), "batch has no operations");
         Context& c = *bt->sc;
         std::map<std::tuple<int, uint32_t, const rck_ksk*>, int> index;
         for (size_t i = 0; i < bt->ops.size(); i++) {

@@ -132,6 +132,8 @@ def test_batch_rejects_bad_ops():
     p = PR.config_batched(n=1 << 15, limbs=4, aux=2)
     ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
     bt = G.Batch(ctx)
+    with pytest.raises(G.RckError, match="batch has no operations"):
+        bt.compile()
     a = G.DeviceBuffer((2, 4, n))
     with pytest.raises(G.RckError, match="unknown batch op kind"):
         bt.add(9, 4, a, a, None, a)
