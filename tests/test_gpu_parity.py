@@ -467,3 +467,34 @@ def test_context_rejects_invalid_chains():
     q, g = chain("above_2^50", 1 << 16, 1)
     ctx = G.Context(poly_degree=8, primes=q, roots=g)  # primes just above 2^50 are accepted
     assert ctx is not None
+
+
+def test_limb_counts_out_of_range_rejected():
+    """Empty (0 limbs) and oversized (more limbs than the chain, or a partial aux basis) operands are
+    rejected with an error before any launch, for the polynomial ops, the scheme ops and the
+    keyswitching entry points."""
+    n = 4096
+    params = PR.realistic_8_level()
+    ctx = G.Context(poly_degree=n, primes=params.primes[:3], roots=params.roots[:3],
+                    aux_primes=params.aux_primes, aux_roots=params.aux_roots, scale_bits=45)
+    buf = empty((3, 6, n))
+    for nq in (0, 4):
+        with pytest.raises(G.RckError, match="limb count must be in 1..=3"):
+            ctx.ntt_forward(buf, nq)
+        with pytest.raises(G.RckError, match="limb count must be in 1..=3"):
+            ctx.poly_add(buf, buf, buf, nq)
+    with pytest.raises(G.RckError, match="aux limb count must be 0 or the aux basis size"):
+        ctx.poly_multiply_ntt(buf, buf, buf, 3, 1)
+    with pytest.raises(G.RckError, match="limb count must be in 1..=3, got 0"):
+        ctx.mult(buf, buf, 0, 45, 45, buf)
+    with pytest.raises(G.RckError, match="limb count must be in 1..=3, got 4"):
+        ctx.rescale(buf, 4, 90, buf)
+    with pytest.raises(G.RckError, match="rescale requires at least two levels"):
+        ctx.rescale(buf, 1, 90, empty((2, 1, n)))
+    # the device is still usable afterwards
+    rs = np.random.default_rng(3)
+    a, b = rand_poly(params.primes[:3], n, rs), rand_poly(params.primes[:3], n, rs)
+    out = empty((3, n))
+    ctx.poly_add(dev(a), dev(b), out, 3)
+    qb = O.Basis(n, params.primes[:3], params.roots[:3])
+    assert np.array_equal(out.numpy(), O.rns_add(qb, a, b))
