@@ -234,7 +234,7 @@ def test_groups_larger_than_32_match_per_op():
     assert np.array_equal(got[40 + 35], O.rotate(qb, eb, a.numpy(), gel, rkey))
 
 
-BATCH_SEEDS = int(os.environ.get("RCK_BATCH_SWEEP_SEEDS", "8"))  # longer: RCK_BATCH_SWEEP_SEEDS=200
+BATCH_SEEDS = int(os.environ.get("RCK_BATCH_SWEEP_SEEDS", "8"))  # longer: RCK_BATCH_SWEEP_SEEDS=600
 
 
 @pytest.mark.parametrize("seed", range(BATCH_SEEDS))

@@ -16,7 +16,7 @@ from rnsckks import params as PR
 from test_gpu_parity import chain, dev, empty, rand_poly
 
 OPS = ("relinearize", "keyswitch", "rotate", "conjugate", "mult_relinearized")
-SEEDS = int(os.environ.get("RCK_SWEEP_SEEDS", "18"))  # a longer sweep: RCK_SWEEP_SEEDS=100
+SEEDS = int(os.environ.get("RCK_SWEEP_SEEDS", "18"))  # a longer sweep: RCK_SWEEP_SEEDS=1500
 
 
 def _case(seed):
