@@ -672,9 +672,11 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
                 a1[8 * g8 + e] += mulmod(v[8 * g8 + e], x1[e], q, qinv);
             }
         }
-        // each product is <= 0.94q (inputs <= 3.26q, mac_phase_a), so six of them on a reduced
-        // accumulator stay below 6.2q < 2^53 / q: exact integers
-        if (++pend == 6) {
+        // mac_phase_b leaves |v| <= 2.17q (four stages), 2.906q (three) or 2.124q (two), so a product
+        // is <= 0.79q, 0.89q or 0.79q; eight (seven with three stages) of them on a reduced
+        // accumulator stay below 6.81q (6.71q) < 2^53 / q = 7.52q for q < 2^50 + 2^46: exact integers
+        constexpr int kPend = LOGR == 7 ? 7 : 8;
+        if (++pend == kPend) {
             pend = 0;
             reduce_all(a0, q, qinv);
             reduce_all(a1, q, qinv);

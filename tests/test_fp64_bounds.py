@@ -81,3 +81,15 @@ def test_bound_rule_stage_chain():
     assert all(x <= 3.75 for x in chain[:-1])
     assert chain[-1] <= 3.2589 + 1e-3
     assert all(any(abs(x - A) < 2e-3 for A in AS) for x in chain)  # the lemma is checked at each
+
+
+def test_mac_accumulation_schedule():
+    """The streamed MAC reduces its accumulators every 8 key products after a four- or two-stage row
+    phase and every 7 after a three-stage one: from a reduced accumulator (0.5q) the sum stays an
+    exact integer (below 2^53) for every allowed q, and one more product could exceed it."""
+    qmax = (1 << 50) + (1 << 46)
+    limit = 2 ** 53 / qmax
+    for phase_out, k in ((2.17, 8), (2.906, 7), (2.124, 8)):
+        product = 0.5 + 0.133 * phase_out
+        assert 0.5 + k * product < limit
+        assert 0.5 + (k + 1) * product >= limit - 0.1  # the schedule is as long as the bound allows
