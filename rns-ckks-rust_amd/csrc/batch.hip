@@ -355,6 +355,9 @@ int rck_batch_compile(rck_batch* bt, int tune) {
         RCK_REQUIRE(bt && !bt->compiled, "batch missing or already compiled");
         RCK_REQUIRE(!bt->ops.empty(), "batch has no operations");
         Context& c = *bt->sc;
+        // a compile retried after a failure starts from no groups (every op joins exactly one group)
+        bt->groups.clear();
+        release_lanes(bt);
         std::map<std::tuple<int, uint32_t, const rck_ksk*>, int> index;
         for (size_t i = 0; i < bt->ops.size(); i++) {
             const BatchOp& o = bt->ops[i];
@@ -373,7 +376,12 @@ int rck_batch_compile(rck_batch* bt, int tune) {
                 bt->groups[it->second].ops.push_back((int)i);
             }
         }
-        for (auto& g : bt->groups) check_group(c, g);
+        try {
+            for (auto& g : bt->groups) check_group(c, g);
+        } catch (...) {
+            bt->groups.clear();
+            throw;
+        }
         const bool prof = prof_enabled();
         prof_set(false);
         try {
@@ -407,6 +415,9 @@ int rck_batch_compile(rck_batch* bt, int tune) {
         } catch (...) {
             for (Context* lc : bt->lane_sc) lc->frozen = false;
             prof_set(prof);
+            (void)hipStreamSynchronize(bt->stream);
+            release_lanes(bt);
+            bt->groups.clear();
             throw;
         }
         prof_set(prof);

@@ -673,8 +673,9 @@ __global__ __launch_bounds__(256) void k_digit_mac_dma(const double* __restrict_
             }
         }
         // mac_phase_b leaves |v| <= 2.17q (four stages), 2.906q (three) or 2.124q (two), so a product
-        // is <= 0.79q, 0.89q or 0.79q; eight (seven with three stages) of them on a reduced
-        // accumulator stay below 6.81q (6.71q) < 2^53 / q = 7.52q for q < 2^50 + 2^46: exact integers
+        // is <= 0.79q, 0.89q or 0.79q; eight of them (seven with the three-stage row pass, LOGR = 7, of
+        // N = 2^13 and 2^14) on a reduced accumulator stay below 6.81q (6.71q) < 2^53 / q = 7.52q for
+        // q < 2^50 + 2^46: exact integers
         constexpr int kPend = LOGR == 7 ? 7 : 8;
         if (++pend == kPend) {
             pend = 0;

@@ -16,6 +16,7 @@
 //         56   payload  u64 residues, [count][(2 for keys)][Q limbs + aux limbs][N], canonical
 // Keys are written in the crate's coefficient form (what rck_ksk_create takes).
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/rnsckks.h"
@@ -106,6 +107,24 @@ Header read_header(const Context& c, const uint8_t* b, size_t len, uint32_t kind
     return h;
 }
 
+// Every residue of the payload ([groups][Q limbs + aux limbs][N]) must be canonical, below its limb's
+// modulus: the kernels' exact arithmetic assumes reduced inputs, so a damaged payload is rejected here
+// instead of producing wrong results later.
+void check_canonical(const Context& c, const uint8_t* payload, uint64_t groups, uint32_t nq, uint32_t np) {
+    const uint32_t limbs = nq + np;
+    for (uint64_t g = 0; g < groups; g++) {
+        for (uint32_t l = 0; l < limbs; l++) {
+            const uint64_t q = l < nq ? c.hp[l].q : c.hp[c.L + (l - nq)].q;
+            const uint8_t* row = payload + ((g * limbs + l) * (uint64_t)c.n) * 8;
+            for (uint32_t i = 0; i < c.n; i++) {
+                const uint64_t v = get64(row + (uint64_t)i * 8);
+                RCK_REQUIRE(v < q, "serialized residue " + std::to_string(v) + " at limb " + std::to_string(l) +
+                                       " is not below its modulus");
+            }
+        }
+    }
+}
+
 }  // namespace
 }  // namespace rck
 
@@ -141,6 +160,7 @@ int rck_deserialize_polys(rck_context* ctx, const uint8_t* buf, size_t len, uint
         if (scale_bits) *scale_bits = h.scale;
         if (!polys) return;
         RCK_REQUIRE(capacity >= h.payload, "destination buffer too small");
+        check_canonical(*ctx->c, buf + kHeader, h.count, h.nq, h.np);
         RCK_CHECK_HIP(hipMemcpy(polys, buf + kHeader, h.payload, hipMemcpyHostToDevice));
     });
 }
@@ -166,6 +186,7 @@ int rck_ksk_deserialize(rck_context* ctx, const uint8_t* buf, size_t len, rck_ks
         RCK_REQUIRE(ctx && ctx->c && out, "null argument");
         const Header h = read_header(*ctx->c, buf, len, 2);
         RCK_REQUIRE(h.scale == h.nq - 1, "serialized key level does not match its limb count");
+        check_canonical(*ctx->c, buf + kHeader, (uint64_t)h.count * 2, h.nq, h.np);
         // the payload is the crate's coefficient form: upload through rck_ksk_create (host memory;
         // the payload is 8-byte aligned when buf is)
         std::vector<uint64_t> tmp;

@@ -87,6 +87,7 @@ struct rck_server {
 
     void run();
     bool round();
+    void fail_all(const std::string& msg);
 };
 
 namespace {
@@ -234,6 +235,31 @@ bool rck_server::round() {
     return true;
 }
 
+// A device or host failure in the worker: every request in flight or waiting fails (the device state
+// of the active ones is unknown), their working sets are released, and later submissions are refused.
+void rck_server::fail_all(const std::string& msg) {
+    for (hipStream_t st : lanes) (void)hipStreamSynchronize(st);
+    std::lock_guard<std::mutex> lk(mu);
+    fatal = msg;
+    for (Request* r : active) {
+        if (r->pool) (void)hipFree(r->pool);
+        r->pool = nullptr;
+        r->regs.clear();
+        r->status = -1;
+        r->error = msg;
+        failed++;
+    }
+    for (Request* r : waiting) {
+        r->status = -1;
+        r->error = msg;
+        failed++;
+    }
+    used = 0;
+    active.clear();
+    waiting.clear();
+    cv_done.notify_all();
+}
+
 void rck_server::run() {
     while (true) {
         {
@@ -241,25 +267,18 @@ void rck_server::run() {
             cv_work.wait(lk, [&] { return stop || !waiting.empty() || !active.empty(); });
             if (stop) return;
         }
+        std::string err;
         try {
             round();
         } catch (const Error& e) {
-            std::lock_guard<std::mutex> lk(mu);
-            fatal = e.msg;
-            // fail everything in flight: the device state of these requests is unknown
-            for (Request* r : active) {
-                r->status = -1;
-                r->error = e.msg;
-                failed++;
-            }
-            for (Request* r : waiting) {
-                r->status = -1;
-                r->error = e.msg;
-                failed++;
-            }
-            active.clear();
-            waiting.clear();
-            cv_done.notify_all();
+            err = e.msg;
+        } catch (const std::exception& e) {
+            err = std::string("server worker: ") + e.what();
+        } catch (...) {
+            err = "server worker: unknown exception";
+        }
+        if (!err.empty()) {
+            fail_all(err);
             return;
         }
     }

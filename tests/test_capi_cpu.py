@@ -151,3 +151,30 @@ def test_rust_binding_covers_every_declared_symbol():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     bound = set(re.findall(r"pub fn (rck_\w+)\(", text))
     assert declared_symbols() == bound
+
+
+def test_wire_fixture_is_self_consistent():
+    """The golden RCK1 images (tests/golden/wire_rck1.json) carry the header fields their JSON states,
+    payload sizes that match, and the stated residues limb-major as little-endian u64, all canonical."""
+    import json
+    import struct
+    with open(os.path.join(os.path.dirname(__file__), "golden", "wire_rck1.json")) as f:
+        fx = json.load(f)
+    n = fx["N"]
+    for kind, rec, nq, np_, count, groups in (
+            (1, fx["ciphertext"], fx["ciphertext"]["limbs"], 0, 2, 2),
+            (2, fx["key"], fx["key"]["level"] + 1, fx["key"]["aux"], fx["key"]["digits"], 2 * fx["key"]["digits"])):
+        raw = bytes.fromhex(rec["bytes_hex"])
+        assert raw[:4] == b"RCK1"
+        ver, k, logn, cnt, hq, hp, sc = struct.unpack("<7I", raw[4:32])
+        gal, _fp, payload = struct.unpack("<3Q", raw[32:56])
+        assert (ver, k, 1 << logn, cnt, hq, hp) == (1, kind, n, count, nq, np_)
+        assert sc == (rec["scale_bits"] if kind == 1 else rec["level"])
+        assert gal == (0 if kind == 1 else rec["galois"])
+        assert payload == len(raw) - 56 == groups * (nq + np_) * n * 8
+        vals = struct.unpack(f"<{payload // 8}Q", raw[56:])
+        assert list(vals) == [v for row in rec["residues"] for v in row]
+        moduli = fx["primes"][:nq] + fx["aux_primes"][:np_]
+        for g in range(groups):
+            for l, q in enumerate(moduli):
+                assert all(v < q for v in rec["residues"][g * len(moduli) + l])

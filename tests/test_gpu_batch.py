@@ -143,6 +143,34 @@ def test_batch_rejects_bad_ops():
     bt.add(G.OP_ROTATE, 4, a, None, k, G.DeviceBuffer((2, 4, n)))
     with pytest.raises(G.RckError, match="rotation key level must match the ciphertext level"):
         bt.compile()
+    # a failed compile leaves no groups behind, and a retry fails the same way with no duplicates
+    assert bt.report()["groups"] == []
+    with pytest.raises(G.RckError, match="rotation key level must match the ciphertext level"):
+        bt.compile()
+    assert bt.report()["groups"] == []
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_compile_failing_in_the_run_leaves_no_groups(tune):
+    """A valid group and a mult+relin whose key has another level: the compile fails inside the eager
+    run (or the tuner), and every retry starts from no groups (each op in exactly one group)."""
+    n = 1 << 12
+    p = PR.config_batched(n=1 << 15, limbs=4, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    rs = np.random.default_rng(3)
+    D = ctx.num_decomp_digits(3)
+    k3 = ctx.ksk_create(np.stack([uniform(p.primes[:3] + p.aux_primes, n, rs, 2) for _ in range(D)]), 2, 2)
+    bt = G.Batch(ctx)
+    a = dev(uniform(p.primes[:3], n, rs, 2))
+    b = dev(uniform(p.primes[:3], n, rs, 2))
+    bt.add(G.OP_MULT_RELIN, 3, a, b, k3, G.DeviceBuffer((2, 3, n)))
+    a4 = dev(uniform(p.primes[:4], n, rs, 2))
+    bt.add(G.OP_MULT_RELIN, 4, a4, a4, k3, G.DeviceBuffer((2, 4, n)))
+    for _ in range(2):
+        with pytest.raises(G.RckError, match="keyswitch key level must match"):
+            bt.compile(tune=tune)
+        rep = bt.report()
+        assert rep["ops"] == 2 and rep["groups"] == []
 
 
 def test_headline_workload_batched_matches_per_op_and_oracle():

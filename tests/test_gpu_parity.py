@@ -456,6 +456,56 @@ def test_largest_allowed_primes_match_oracle():
     assert np.array_equal(rot.numpy(), O.rotate(qb, eb, ct, gel, key))
 
 
+@pytest.mark.parametrize("logn", [13, 15])
+def test_largest_allowed_primes_long_digit_runs(logn):
+    """Primes just below 2^50 + 2^46 with 10 limbs (D = 17 digits): the streamed MAC's accumulators
+    take 7 (N = 2^13, three-stage row pass) or 8 (N = 2^15) products between reductions at the largest
+    values the bound allows.  A batch of 4 (one ciphertext per wave, every digit on one wave) and the
+    per-op calls (four digit ranges per workgroup) for mult+relin and rotate; op 0 against the oracle,
+    the rest batch == per-op."""
+    n, L, K, B = 1 << logn, 10, 2, 4
+    step, top = 2 * n, (1 << 50) + (1 << 46)
+    q, cand = [], (top - 1) // step * step + 1
+    while len(q) < L + K:
+        if PR.is_prime(cand):
+            q.append(cand)
+        cand -= step
+    g = [PR.smallest_primitive_root(x) for x in q]
+    ctx = G.Context(poly_degree=n, primes=q[:L], roots=g[:L], aux_primes=q[L:], aux_roots=g[L:])
+    qb = O.Basis(n, q[:L], g[:L])
+    eb = O.Basis(n, q, g)
+    D = ctx.num_decomp_digits(L)
+    assert D >= 15
+    rs = np.random.default_rng(4600 + logn)
+    key = np.stack([rand_poly(eb.moduli, n, rs, 2) for _ in range(D)])
+    gel = O.galois_element_for_rotation(n, -3)
+    dk = ctx.ksk_create(key, L - 1, K)
+    drk = ctx.ksk_create(key, L - 1, K, gel)
+    P = L * n
+    lhs = rand_poly(qb.moduli, n, rs, 2 * B).reshape(B, 2, L, n)
+    rhs = rand_poly(qb.moduli, n, rs, 2 * B).reshape(B, 2, L, n)
+    dl, dr = dev(lhs), dev(rhs)
+    out_m, out_r = empty((B, 2, L, n)), empty((B, 2, L, n))
+    bt = G.Batch(ctx)
+    for b in range(B):
+        bt.add(G.OP_MULT_RELIN, L, dl.offset(b * 2 * P), dr.offset(b * 2 * P), dk, out_m.offset(b * 2 * P))
+        bt.add(G.OP_ROTATE, L, dl.offset(b * 2 * P), None, drk, out_r.offset(b * 2 * P))
+    bt.compile(tune=False)
+    got_m, got_r = out_m.numpy(), out_r.numpy()
+    O.set_threads(16)
+    want_m = O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs[0], rhs[0]), key)
+    want_r = O.rotate(qb, eb, lhs[0], gel, key)
+    assert np.array_equal(got_m[0], want_m)
+    assert np.array_equal(got_r[0], want_r)
+    quad, one = empty((3, L, n)), empty((2, L, n))
+    for b in range(B):
+        ctx.mult(dl.offset(b * 2 * P), dr.offset(b * 2 * P), L, 40, 40, quad)
+        ctx.relinearize(quad, L, 80, dk, one)
+        assert np.array_equal(one.numpy(), got_m[b]), f"mult+relin {b}"
+        ctx.rotate(dl.offset(b * 2 * P), L, 40, drk, one)
+        assert np.array_equal(one.numpy(), got_r[b]), f"rotate {b}"
+
+
 def test_context_rejects_invalid_chains():
     """RnsContext / NttPlan::new validation (math/ntt.rs:17-60) with the crate's error texts, and the
     modulus bound of the FP64-exact arithmetic (2^50 + 2^46)."""

@@ -53,3 +53,55 @@ def test_round_trips_and_rejections():
     small = G.Context(poly_degree=n // 2, primes=p.primes, roots=p.roots)
     with pytest.raises(G.RckError, match="another ring degree"):
         small.deserialize_polys(data)
+
+
+def _golden():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "wire_rck1.json")) as f:
+        return json.load(f)
+
+
+def test_serialized_bytes_match_the_golden_layout():
+    """The library's bytes equal the golden RCK1 images written from the format description alone
+    (tests/golden/make_wire_fixture.py), and the golden images load back to the same residues."""
+    fx = _golden()
+    n = fx["N"]
+    ctx = G.Context(poly_degree=n, primes=fx["primes"], roots=fx["roots"], aux_primes=fx["aux_primes"],
+                    aux_roots=fx["aux_roots"])
+    c = fx["ciphertext"]
+    ct = np.array(c["residues"], dtype=np.uint64).reshape(2, c["limbs"], n)
+    got = ctx.serialize_polys(G.DeviceBuffer.from_numpy(ct), 2, c["limbs"], 0, c["scale_bits"])
+    assert got.hex() == c["bytes_hex"]
+    back, meta = ctx.deserialize_polys(bytes.fromhex(c["bytes_hex"]))
+    assert meta == {"count": 2, "nq": c["limbs"], "np": 0, "scale_bits": c["scale_bits"]}
+    assert np.array_equal(back.numpy().reshape(ct.shape), ct)
+    k = fx["key"]
+    limbs = k["level"] + 1 + k["aux"]
+    key = np.array(k["residues"], dtype=np.uint64).reshape(k["digits"], 2, limbs, n)
+    dk = ctx.ksk_create(key, k["level"], k["aux"], k["galois"])
+    assert ctx.ksk_serialize(dk).hex() == k["bytes_hex"]
+    dk2 = ctx.ksk_deserialize(bytes.fromhex(k["bytes_hex"]))
+    assert (dk2.digits, dk2.level, dk2.aux, dk2.galois) == (k["digits"], k["level"], k["aux"], k["galois"])
+    assert np.array_equal(ctx.ksk_export(dk2), key)
+
+
+def test_non_canonical_residues_are_rejected():
+    """A payload whose header and fingerprint are intact but holds a residue at or above its modulus is
+    refused before upload, for polynomial sets and keys."""
+    fx = _golden()
+    n = fx["N"]
+    ctx = G.Context(poly_degree=n, primes=fx["primes"], roots=fx["roots"], aux_primes=fx["aux_primes"],
+                    aux_roots=fx["aux_roots"])
+    import struct
+    raw = bytearray(bytes.fromhex(fx["ciphertext"]["bytes_hex"]))
+    q1 = fx["primes"][1]
+    raw[56 + (1 * n + 5) * 8: 56 + (1 * n + 6) * 8] = struct.pack("<Q", q1)  # limb 1, coefficient 5 := q_1
+    with pytest.raises(G.RckError, match="not below its modulus"):
+        ctx.deserialize_polys(bytes(raw))
+    raw = bytearray(bytes.fromhex(fx["key"]["bytes_hex"]))
+    pa = fx["aux_primes"][0]
+    off = 56 + ((1 * 2 + 1) * 4 + 3) * n * 8  # digit 1, component 1, the aux limb, coefficient 0
+    raw[off: off + 8] = struct.pack("<Q", pa + 7)
+    with pytest.raises(G.RckError, match="not below its modulus"):
+        ctx.ksk_deserialize(bytes(raw))
