@@ -1,15 +1,23 @@
 // Request server: Paterson-Stockmeyer evaluations submitted at any time, admitted by device
 // working-set size, and advanced together with continuous batching.
 //
-// A request is a compiled PS program (algorithms.hpp) over its own device registers.  A worker
-// thread runs scheduling rounds: it admits waiting requests in arrival order while their working
-// sets fit the byte budget, advances every admitted request through its cheap instructions (constant
-// products and adds, truncations), then launches each set of ct x ct multiplications that share a
-// ring, a level and a relinearization key as ONE batched mult + relinearize + rescale
-// (run_op_group: the key is streamed once for all of them), and finishes requests whose program
-// ended.  Requests that arrive while others run join at the next round.  The arithmetic of every
-// request is exactly rck_ps_eval's (same instructions, same kernels), so a result does not depend
-// on what it was batched with.
+// A request is a compiled PS program (algorithms.hpp) over its own device registers, carved out of one
+// device pool the server allocates once (its byte budget) by a first-fit allocator: admission in arrival
+// order (no overtaking) as soon as a block of the request's working-set size is free.
+//
+// A worker thread issues rounds without waiting for the GPU.  A round advances every admitted request
+// by one ct x ct multiplication: the requests whose next multiplication shares a ring, a level and a
+// relinearization key form ONE batched mult + relinearize + rescale (run_op_group: the key is streamed
+// once for all of them), preceded and followed by each request's cheap instructions (constant products
+// and adds, truncations) on the same stream lane.  Requests whose program has been issued to its end get
+// their output copy on that lane.  Each group records an event on its lane; a request whose next group
+// lands on another lane makes that lane wait for its last event, so every request's instructions stay
+// in order while independent groups run concurrently on up to 8 lanes.  At most kMaxInflight rounds are
+// queued on the GPU: a new arrival joins the next round issued, which is queued while the previous one
+// runs, so the device does not idle between rounds.  Rounds retire in order through their events
+// (polled, then waited for when the queue is full); a request retires with the round that copied its
+// output, and its pool block is released.  The arithmetic of every request is exactly rck_ps_eval's
+// (same instructions, same kernels), so a result does not depend on what it was batched with.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -29,11 +37,15 @@ using namespace rck;
 namespace {
 
 using Clock = std::chrono::steady_clock;
+constexpr int kMaxInflight = 2;      // rounds queued on the device at once
+constexpr size_t kPoolAlign = 256;   // sub-allocation granule of the request pool
 
 struct Ring {
     std::vector<Context*> lane;       // one scratch clone per stream lane: the server's own workspaces
     std::vector<const KeyDev*> keys;  // relinearization keys by level
 };
+
+struct Round;
 
 struct Request {
     uint64_t id = 0;
@@ -42,12 +54,62 @@ struct Request {
     const uint64_t* in = nullptr;
     uint64_t* out = nullptr;
     size_t footprint = 0;
-    uint64_t* pool = nullptr;
+    size_t pool_off = 0;
+    bool has_block = false;
     std::vector<uint64_t*> regs;
-    size_t pc = 0;
+    size_t pc = 0;               // next instruction to issue
+    bool issued_all = false;     // every instruction and the output copy are queued
+    int last_lane = -1;          // lane of the request's last queued work
+    hipEvent_t last_event = nullptr;  // recorded after that work (null: nothing pending)
+    Round* last_round = nullptr;      // round that owns last_event
     Clock::time_point t_submit, t_admit, t_done;
     int status = 1;  // 1 waiting, 2 running, 0 done, <0 failed
     std::string error;
+};
+
+struct Round {
+    std::vector<hipEvent_t> events;     // one per group launch and per output copy, in issue order
+    std::vector<Request*> finishing;    // requests whose output copy is in this round
+};
+
+// First-fit allocator over [0, size) with coalescing free blocks (host bookkeeping only).
+struct PoolAlloc {
+    size_t size = 0;
+    std::map<size_t, size_t> free;  // offset -> length
+    void init(size_t n) {
+        size = n;
+        free.clear();
+        if (n) free[0] = n;
+    }
+    bool alloc(size_t n, size_t* off) {
+        n = (n + kPoolAlign - 1) / kPoolAlign * kPoolAlign;
+        for (auto it = free.begin(); it != free.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const size_t rest = it->second - n;
+            const size_t at = it->first + n;
+            free.erase(it);
+            if (rest) free[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = (n + kPoolAlign - 1) / kPoolAlign * kPoolAlign;
+        auto next = free.lower_bound(off);
+        if (next != free.end() && off + n == next->first) {
+            n += next->second;
+            next = free.erase(next);
+        }
+        if (next != free.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += n;
+                return;
+            }
+        }
+        free[off] = n;
+    }
 };
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
@@ -76,7 +138,11 @@ struct rck_server {
     uint32_t max_active = 0;
     uint64_t next_id = 1;
     bool stop = false;
+    uint8_t* pool = nullptr;         // the requests' working sets (budget bytes, allocated once)
+    PoolAlloc alloc;
     std::vector<hipStream_t> lanes;  // independent multiplication groups run concurrently, one per lane
+    std::vector<hipEvent_t> event_pool;
+    std::deque<Round*> inflight;     // issued rounds, oldest first
     std::thread worker;
     // statistics
     uint64_t rounds = 0, mult_launches = 0, mults = 0, completed = 0, failed = 0;
@@ -86,8 +152,11 @@ struct rck_server {
     std::string fatal;
 
     void run();
-    bool round();
+    void admit();
+    void issue_round();
+    bool retire_ready(bool wait_oldest);
     void fail_all(const std::string& msg);
+    hipEvent_t take_event();
 };
 
 namespace {
@@ -111,34 +180,44 @@ int sguard(F&& f) {
 
 }  // namespace
 
-// One scheduling round.  Returns false when there was nothing to do.
-bool rck_server::round() {
-    std::vector<Request*> run;
+hipEvent_t rck_server::take_event() {
+    if (event_pool.empty()) {
+        hipEvent_t e;
+        RCK_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    }
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+}
+
+// Admission in arrival order while the pool has a free block of the next request's working-set size.
+void rck_server::admit() {
+    std::vector<Request*> admitted;
     {
         std::lock_guard<std::mutex> lk(mu);
-        // admission in arrival order (no overtaking: a large request is not starved)
         while (!waiting.empty() && (max_active == 0 || active.size() < max_active)) {
             Request* r = waiting.front();
-            if (used + r->footprint > budget) break;
+            size_t off = 0;
+            if (r->footprint && !alloc.alloc(r->footprint, &off)) break;
             waiting.pop_front();
+            r->pool_off = off;
+            r->has_block = r->footprint != 0;
             used += r->footprint;
             peak = std::max(peak, used);
             r->status = 2;
             r->t_admit = Clock::now();
             active.push_back(r);
+            admitted.push_back(r);
         }
-        run = active;
     }
-    if (run.empty()) return false;
-    // allocate the working sets of newly admitted requests (outside the lock)
-    for (Request* r : run) {
-        if (r->pool || r->footprint == 0) continue;
-        RCK_CHECK_HIP(hipMalloc(&r->pool, r->footprint));
+    for (Request* r : admitted) {
         const uint32_t n = rings.at(r->ctx).lane[0]->n;
+        uint64_t* base = reinterpret_cast<uint64_t*>(pool + r->pool_off);
         std::vector<uint64_t*> slot_ptr;
         size_t off = 0;
         for (int sl : r->prog.slot_limbs) {
-            slot_ptr.push_back(r->pool + off);
+            slot_ptr.push_back(base + off);
             off += (size_t)sl * 2 * n;
         }
         r->regs.assign(r->prog.reg_limbs.size(), nullptr);
@@ -146,10 +225,48 @@ bool rck_server::round() {
         for (size_t k = 1; k < r->regs.size(); k++)
             if (r->prog.slot_of[k] >= 0) r->regs[k] = slot_ptr[r->prog.slot_of[k]];
     }
-    // Group the requests by the (ring, level) of their next multiplication; each group is one
-    // batched launch.  Groups go to the stream lanes (largest first, to the least loaded lane); a
-    // request's cheap instructions before and after its multiplication run on its group's lane, so
-    // every request's instructions stay in order on one stream within the round.
+}
+
+// One round: every admitted request with instructions left advances by one multiplication (grouped by
+// ring and level into batched launches spread over the lanes), or to its end.
+void rck_server::issue_round() {
+    std::vector<Request*> run;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Request* r : active)
+            if (!r->issued_all) run.push_back(r);
+    }
+    Round* rd = new Round();
+    const int nl = (int)lanes.size();
+    // make lane li wait for r's last work when that was queued on another lane
+    auto order_on = [&](Request* r, int li) {
+        if (r->last_event && r->last_lane != li) RCK_CHECK_HIP(hipStreamWaitEvent(lanes[li], r->last_event, 0));
+    };
+    auto mark = [&](const std::vector<Request*>& rs, int li) {
+        hipEvent_t e = take_event();
+        RCK_CHECK_HIP(hipEventRecord(e, lanes[li]));
+        rd->events.push_back(e);
+        for (Request* r : rs) {
+            r->last_lane = li;
+            r->last_event = e;
+            r->last_round = rd;
+        }
+    };
+    auto cheap = [&](Request* r, int li) {
+        Ring& g = rings.at(r->ctx);
+        while (r->pc < r->prog.code.size() && r->prog.code[r->pc].kind != kPsMultRelinRescale) {
+            ps_exec(*g.lane[li], r->prog, r->prog.code[r->pc], r->regs, g.keys, lanes[li]);
+            r->pc++;
+        }
+    };
+    // the output copy of a request issued to its end (on the lane of its last work)
+    auto finish = [&](Request* r, int li) {
+        const uint32_t n = rings.at(r->ctx).lane[0]->n;
+        const size_t rb = (size_t)2 * r->prog.reg_limbs[r->prog.result] * n * 8;
+        RCK_CHECK_HIP(hipMemcpyAsync(r->out, r->regs[r->prog.result], rb, hipMemcpyDeviceToDevice, lanes[li]));
+        r->issued_all = true;
+        rd->finishing.push_back(r);
+    };
     std::map<std::pair<rck_context*, int>, std::vector<Request*>> groups;
     std::vector<Request*> tail;  // no multiplication left: only cheap instructions
     for (Request* r : run) {
@@ -165,15 +282,7 @@ bool rck_server::round() {
         order.push_back({(double)kv.second.size() * (kv.first.second + 1) * rings.at(kv.first.first).lane[0]->n,
                          kv.first});
     std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-    const int nl = (int)lanes.size();
     std::vector<double> load(nl, 0.0);
-    auto cheap = [&](Request* r, int li) {
-        Ring& g = rings.at(r->ctx);
-        while (r->pc < r->prog.code.size() && r->prog.code[r->pc].kind != kPsMultRelinRescale) {
-            ps_exec(*g.lane[li], r->prog, r->prog.code[r->pc], r->regs, g.keys, lanes[li]);
-            r->pc++;
-        }
-    };
     for (auto& o : order) {
         const int li = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         load[li] += o.first;
@@ -183,6 +292,7 @@ bool rck_server::round() {
         std::vector<const uint64_t*> as, bs;
         std::vector<uint64_t*> outs;
         for (Request* r : grp) {
+            order_on(r, li);
             cheap(r, li);
             const PsInstr& ins = r->prog.code[r->pc];
             as.push_back(r->regs[ins.a]);
@@ -193,57 +303,87 @@ bool rck_server::round() {
         for (Request* r : grp) {
             r->pc++;
             cheap(r, li);  // up to the next multiplication (next round) or the end
+            if (r->pc >= r->prog.code.size()) finish(r, li);
         }
+        mark(grp, li);
         mult_launches++;
         mults += grp.size();
         batch_hist[(int)grp.size()]++;
     }
-    for (Request* r : tail) cheap(r, 0);
-    std::vector<Request*> finished;
-    for (Request* r : run) {
-        if (r->pc < r->prog.code.size()) continue;
-        // a finished request's last instruction ran on some lane: copy on every lane's order by
-        // synchronizing first (the copy is small)
-        finished.push_back(r);
+    for (Request* r : tail) {
+        const int li = r->last_lane >= 0 ? r->last_lane : 0;
+        order_on(r, li);
+        cheap(r, li);
+        finish(r, li);
+        mark({r}, li);
     }
-    for (hipStream_t st : lanes) RCK_CHECK_HIP(hipStreamSynchronize(st));
-    for (Request* r : finished) {
-        const uint32_t n = rings.at(r->ctx).lane[0]->n;
-        const size_t rb = (size_t)2 * r->prog.reg_limbs[r->prog.result] * n * 8;
-        RCK_CHECK_HIP(hipMemcpyAsync(r->out, r->regs[r->prog.result], rb, hipMemcpyDeviceToDevice, lanes[0]));
-    }
-    if (!finished.empty()) RCK_CHECK_HIP(hipStreamSynchronize(lanes[0]));
     rounds++;
-    const Clock::time_point now = Clock::now();
-    for (Request* r : finished) {
-        if (r->pool) RCK_CHECK_HIP(hipFree(r->pool));
-        r->pool = nullptr;
-        r->regs.clear();
-    }
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        for (Request* r : finished) {
-            r->t_done = now;
-            r->status = 0;
-            used -= r->footprint;
-            completed++;
-            last_done = now;
-            active.erase(std::find(active.begin(), active.end(), r));
+    inflight.push_back(rd);
+}
+
+// Retires the rounds whose events have all completed, oldest first (waits for the oldest when asked).
+// Returns true when a round retired.
+bool rck_server::retire_ready(bool wait_oldest) {
+    bool any = false;
+    while (!inflight.empty()) {
+        Round* rd = inflight.front();
+        bool done = true;
+        for (hipEvent_t e : rd->events) {
+            if (wait_oldest && !any) {
+                RCK_CHECK_HIP(hipEventSynchronize(e));
+                continue;
+            }
+            const hipError_t q = hipEventQuery(e);
+            if (q == hipErrorNotReady) {
+                done = false;
+                break;
+            }
+            RCK_CHECK_HIP(q);
         }
+        if (!done) break;
+        inflight.pop_front();
+        any = true;
+        const Clock::time_point now = Clock::now();
+        // a retired round's events are no longer pending work for anyone
+        for (Request* r : active)
+            if (r->last_round == rd) {
+                r->last_event = nullptr;
+                r->last_round = nullptr;
+            }
+        for (hipEvent_t e : rd->events) event_pool.push_back(e);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (Request* r : rd->finishing) {
+                if (r->has_block) alloc.release(r->pool_off, r->footprint);
+                r->has_block = false;
+                r->regs.clear();
+                r->t_done = now;
+                r->status = 0;
+                used -= r->footprint;
+                completed++;
+                last_done = now;
+                active.erase(std::find(active.begin(), active.end(), r));
+            }
+        }
+        if (!rd->finishing.empty()) cv_done.notify_all();
+        delete rd;
     }
-    if (!finished.empty()) cv_done.notify_all();
-    return true;
+    return any;
 }
 
 // A device or host failure in the worker: every request in flight or waiting fails (the device state
 // of the active ones is unknown), their working sets are released, and later submissions are refused.
 void rck_server::fail_all(const std::string& msg) {
     for (hipStream_t st : lanes) (void)hipStreamSynchronize(st);
+    for (Round* rd : inflight) {
+        for (hipEvent_t e : rd->events) event_pool.push_back(e);
+        delete rd;
+    }
+    inflight.clear();
     std::lock_guard<std::mutex> lk(mu);
     fatal = msg;
     for (Request* r : active) {
-        if (r->pool) (void)hipFree(r->pool);
-        r->pool = nullptr;
+        r->has_block = false;
         r->regs.clear();
         r->status = -1;
         r->error = msg;
@@ -255,6 +395,7 @@ void rck_server::fail_all(const std::string& msg) {
         failed++;
     }
     used = 0;
+    alloc.init(budget);
     active.clear();
     waiting.clear();
     cv_done.notify_all();
@@ -262,14 +403,29 @@ void rck_server::fail_all(const std::string& msg) {
 
 void rck_server::run() {
     while (true) {
-        {
-            std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !waiting.empty() || !active.empty(); });
-            if (stop) return;
-        }
         std::string err;
         try {
-            round();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (stop) break;
+            }
+            admit();
+            retire_ready(false);
+            bool issuable = false;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (Request* r : active) issuable |= !r->issued_all;
+            }
+            if (issuable && (int)inflight.size() < kMaxInflight) {
+                issue_round();
+                continue;
+            }
+            if (!inflight.empty()) {
+                retire_ready(true);  // the queue is full or nothing new to issue: wait for the oldest round
+                continue;
+            }
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !waiting.empty(); });
         } catch (const Error& e) {
             err = e.msg;
         } catch (const std::exception& e) {
@@ -294,16 +450,22 @@ int rck_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t l
         rck_server* s = new rck_server();
         s->budget = hbm_budget_bytes;
         s->max_active = max_active;
-        for (uint32_t i = 0; i < lanes; i++) {
+        // the requests' working sets: one device pool of the budget (rounded to the sub-allocation
+        // granule), allocated once; requests get blocks of it at admission
+        const size_t pool_bytes = (hbm_budget_bytes + kPoolAlign - 1) / kPoolAlign * kPoolAlign;
+        hipError_t e = hipMalloc(&s->pool, pool_bytes);
+        for (uint32_t i = 0; e == hipSuccess && i < lanes; i++) {
             hipStream_t st = nullptr;
-            hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-            if (e != hipSuccess) {
-                for (hipStream_t x : s->lanes) (void)hipStreamDestroy(x);
-                delete s;
-                RCK_CHECK_HIP(e);
-            }
-            s->lanes.push_back(st);
+            e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+            if (e == hipSuccess) s->lanes.push_back(st);
         }
+        if (e != hipSuccess) {
+            for (hipStream_t x : s->lanes) (void)hipStreamDestroy(x);
+            if (s->pool) (void)hipFree(s->pool);
+            delete s;
+            RCK_CHECK_HIP(e);
+        }
+        s->alloc.init(pool_bytes);
         s->worker = std::thread([s] { s->run(); });
         *out = s;
     });
@@ -445,13 +607,16 @@ int rck_server_destroy(rck_server* s) {
         s->cv_work.notify_all();
         if (s->worker.joinable()) s->worker.join();
         for (hipStream_t st : s->lanes) (void)hipStreamSynchronize(st);
-        for (auto& kv : s->requests) {
-            if (kv.second->pool) (void)hipFree(kv.second->pool);
-            delete kv.second;
+        for (Round* rd : s->inflight) {
+            for (hipEvent_t e : rd->events) s->event_pool.push_back(e);
+            delete rd;
         }
+        for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+        for (auto& kv : s->requests) delete kv.second;
         for (auto& kv : s->rings)
             for (Context* c : kv.second.lane) delete c;
         for (hipStream_t st : s->lanes) (void)hipStreamDestroy(st);
+        if (s->pool) (void)hipFree(s->pool);
         delete s;
     });
 }

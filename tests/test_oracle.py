@@ -362,6 +362,62 @@ def test_noiseless_keyswitch_recovers_source_product(hybrid):
         assert np.array_equal(O.decrypt(q, ks, s), O.rns_multiply_ntt(q, inp, src))
 
 
+def galois_py(a, g, q):
+    """X -> X^g on one limb by the definition (coefficient i moves to i*g mod 2N, negated past N)."""
+    n = len(a)
+    out = [0] * n
+    for i, v in enumerate(a):
+        k = (i * g) % (2 * n)
+        if k < n:
+            out[k] = (out[k] + int(v)) % q
+        else:
+            out[k - n] = (out[k - n] - int(v)) % q
+    return out
+
+
+@pytest.mark.parametrize("hybrid,steps", [(False, 1), (False, -3), (True, 2), (True, "conj")])
+def test_noiseless_rotation_decrypts_to_the_rotated_plaintext(hybrid, steps):
+    """rotate (context/rotation.rs:104-127) with a noiseless key for s(X^g): decrypt(rotate(ct)) equals
+    sigma_g(decrypt(ct)) exactly, sigma_g applied by its definition in Python integers."""
+    if hybrid:
+        p = __import__("rnsckks.params", fromlist=["x"]).realistic_8_level()
+        n = 128
+        q = O.Basis(n, p.primes[:3], p.roots[:3])
+        e = O.Basis(n, p.primes[:3] + p.aux_primes, p.roots[:3] + p.aux_roots)
+    else:
+        ps, gs = chain("below_2^50", 1 << 12, 3)
+        n = 64
+        q = O.Basis(n, ps, gs)
+        e = None
+    rng, s, pk, _ = _keys(q, 33, hamming=8)
+    g = 2 * n - 1 if steps == "conj" else O.galois_element_for_rotation(n, steps)
+    src = O.rns_galois(q, s, g)
+    ksk = O.keygen_hybrid(q, e, rng, src, s, noiseless=True) if hybrid else \
+        O.keygen_qonly(q, rng, src, pk, noiseless=True)
+    ct = np.stack([O.sample_uniform(q, rng), O.sample_uniform(q, rng)])
+    rot = O.rotate(q, e, ct, g, ksk)
+    dec = O.decrypt(q, ct, s)
+    got = O.decrypt(q, rot, s)
+    for l, m in enumerate(q.moduli):
+        assert [int(v) for v in got[l]] == galois_py(dec[l], g, m), f"limb {l}"
+
+
+def test_multiply_plain_matches_schoolbook():
+    """multiply_plain (rns/ops.rs:54-66): each component times the plaintext, by a Python-int schoolbook
+    negacyclic product per limb."""
+    ps, gs = chain("below_2^50", 1 << 12, 3)
+    n = 32
+    b = O.Basis(n, ps, gs)
+    rng = O.Rng(12)
+    ct = np.stack([O.sample_uniform(b, rng), O.sample_uniform(b, rng)])
+    pt = O.sample_uniform(b, rng)
+    out = O.multiply_plain(b, ct, pt)
+    assert out.shape == ct.shape
+    for c in range(2):
+        for l, m in enumerate(ps):
+            assert [int(v) for v in out[c, l]] == negacyclic_py(ct[c, l], pt[l], m), (c, l)
+
+
 def test_keyswitch_rejects_short_key():
     ps, gs = chain("below_2^50", 1 << 12, 3)
     q = O.Basis(16, ps, gs)
