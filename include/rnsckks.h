@@ -89,6 +89,14 @@ int rck_poly_sub(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_
 /* RnsPolynomial::multiply_ntt (polynomial.rs:121-136 -> poly/mod.rs:134-161): negacyclic product */
 int rck_poly_multiply_ntt(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out, uint32_t nq,
                           uint32_t np, void* stream);
+/* No counterpart in the crate: the same negacyclic product as rck_poly_multiply_ntt, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  It calls no
+ * device allocator: the operands' NTT forms go to a fixed 64 MiB scratch array that every GPU the
+ * library is loaded on holds until the process ends, so 2 x (nq + np) x N x 8 bytes must be at most
+ * 64 MiB (e.g. up to 64 limbs at N = 2^16), else RCK_ERR_INVALID.  All calls on one GPU share that
+ * scratch: calls that overlap on one GPU (other host threads, unordered streams) race on it. */
+int rck_poly_multiply_ntt_noalloc(rck_context* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out,
+                                  uint32_t nq, uint32_t np, void* stream);
 /* RnsPolynomial::apply_galois_automorphism (polynomial.rs:138-145 -> poly/mod.rs:110-130) */
 int rck_poly_galois(rck_context* ctx, const uint64_t* a, uint64_t* out, uint32_t nq, uint32_t np, uint64_t galois,
                     void* stream);

@@ -10,6 +10,7 @@
 namespace rck {
 Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32_t L, const uint64_t* p,
                         const uint64_t* gp, uint32_t K, uint32_t scale_bits);
+void noalloc_setup_current_device();
 }
 
 using namespace rck;
@@ -120,6 +121,12 @@ int rck_context_create(uint32_t poly_degree, const uint64_t* primes, const uint6
         RCK_REQUIRE(n_primes == 0 || (primes && roots), "null prime list");
         RCK_REQUIRE(n_aux == 0 || (aux_primes && aux_roots), "null aux prime list");
         Context* c = context_create(poly_degree, primes, roots, n_primes, aux_primes, aux_roots, n_aux, scale_bits);
+        try {
+            noalloc_setup_current_device();  // this device's allocation-free scratch address
+        } catch (...) {
+            delete c;
+            throw;
+        }
         *out = new rck_context{c};
     });
 }

@@ -14,12 +14,18 @@ namespace rck {
 
 enum { kSlotLhs = 10, kSlotRhs = 11, kSlotQuad = 12, kSlotTmp = 13 };
 
-// negacyclic product of `polys` polynomial pairs a[z] * b[z or 0] -> out[z] (NTT resident)
+// negacyclic product of `polys` polynomial pairs a[z] * b[z or 0] -> out[z] (NTT resident).  The NTT
+// forms live in the context's workspace, or in `scratch` when given (poly_mul_scratch_bytes bytes).
+size_t poly_mul_scratch_bytes(const Context& c, long long b_ps, int limbs, int polys) {
+    return (size_t)(polys + (b_ps ? polys : 1)) * limbs * c.n * 8;
+}
+
 void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
-                     long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+              long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, double* scratch) {
     const long long n = c.n, P = (long long)limbs * n;
     const int bpolys = b_ps ? polys : 1;
-    double* t = static_cast<double*>(c.workspace(0, (size_t)(polys + bpolys) * P * 8));
+    double* t = scratch ? scratch
+                        : static_cast<double*>(c.workspace(0, poly_mul_scratch_bytes(c, b_ps, limbs, polys)));
     double* tb = t + (size_t)polys * P;
     launch_fwd(c, kInU64, a, n, a_ps, t, n, P, limbs, polys, pm, s);
     launch_fwd(c, kInU64, b, n, b_ps, tb, n, P, limbs, bpolys, pm, s);

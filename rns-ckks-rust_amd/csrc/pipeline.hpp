@@ -25,9 +25,11 @@ inline void check_ptr(const void* p, const char* what) {
     RCK_REQUIRE(p != nullptr, std::string("null ") + what);
 }
 
-// negacyclic products a[z] * b[z] (b_ps == 0: the same b for every z) of `polys` polynomials
+// negacyclic products a[z] * b[z] (b_ps == 0: the same b for every z) of `polys` polynomials; the NTT
+// forms go to the context's workspace, or to `scratch` (poly_mul_scratch_bytes bytes) when given
+size_t poly_mul_scratch_bytes(const Context& c, long long b_ps, int limbs, int polys);
 void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
-              long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
+              long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, double* scratch = nullptr);
 // B products lhs[b] x rhs[b] (ciphertext stride in_ps) -> quadratic out[b] (stride out_ps).  fold_ntt:
 // out[b] gets d2 only and d0, d1 stay in NTT form in the returned buffer ([B][3][limbs][N] doubles)
 double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,

@@ -43,6 +43,7 @@ EXPORTS = {
     "rck_poly_add": (ctypes.c_int, [vp, vp, vp, vp, u32, u32, vp]),
     "rck_poly_sub": (ctypes.c_int, [vp, vp, vp, vp, u32, u32, vp]),
     "rck_poly_multiply_ntt": (ctypes.c_int, [vp, vp, vp, vp, u32, u32, vp]),
+    "rck_poly_multiply_ntt_noalloc": (ctypes.c_int, [vp, vp, vp, vp, u32, u32, vp]),
     "rck_poly_galois": (ctypes.c_int, [vp, vp, vp, u32, u32, u64, vp]),
     "rck_poly_mul_add": (ctypes.c_int, [vp, vp, vp, vp, vp, u32, u32, vp]),
     "rck_ct_add": (ctypes.c_int, [vp, vp, vp, u32, u32, u32, vp, u32p, vp]),
@@ -418,6 +419,10 @@ class Context:
 
     def poly_multiply_ntt(self, a, b, out, nq, np_=0, stream=None):
         _check(lib().rck_poly_multiply_ntt(self.ptr, _p(a), _p(b), _p(out), nq, np_, stream))
+
+    def poly_multiply_ntt_noalloc(self, a, b, out, nq, np_=0, stream=None):
+        """The same product through the library's fixed device scratch (no device allocation)."""
+        _check(lib().rck_poly_multiply_ntt_noalloc(self.ptr, _p(a), _p(b), _p(out), nq, np_, stream))
 
     def poly_galois(self, a, out, nq, galois, np_=0, stream=None):
         _check(lib().rck_poly_galois(self.ptr, _p(a), _p(out), nq, np_, galois, stream))
