@@ -80,15 +80,19 @@ def modmul_per_op(n, L, K, D):
             "mod_down": 2 * n * L * (K + 2), "tensor": 4 * L * n}
 
 
+# the sources the measured mult+relinearize step is compiled from (kernels, their launchers and the
+# batched executor): counter records taken on other versions of these files are not reused
+MEASURED_SOURCES = ("modarith.hpp", "context.hpp", "ntt.hpp", "ntt.hip", "ops.hpp", "ops.hip", "pipeline.hpp",
+                    "pipeline.hip", "moddown.hpp", "keyswitch.hip", "ks_exact.hip", "batch.hip")
+
+
 def source_sha():
-    """Hash of the library sources: counter records taken on other sources are not reused."""
+    """Hash of the measured path's sources."""
     h = hashlib.sha256()
-    for d in (os.path.join(ROOT, "rns-ckks-rust_amd", "csrc"), os.path.join(ROOT, "include")):
-        for f in sorted(os.listdir(d)):
-            if f.endswith((".hip", ".hpp", ".h")):
-                h.update(f.encode())
-                with open(os.path.join(d, f), "rb") as fh:
-                    h.update(fh.read())
+    for f in MEASURED_SOURCES:
+        h.update(f.encode())
+        with open(os.path.join(ROOT, "rns-ckks-rust_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
     return h.hexdigest()[:16]
 
 

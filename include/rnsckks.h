@@ -71,6 +71,18 @@ int rck_context_create(uint32_t poly_degree, const uint64_t* primes, const uint6
 int rck_context_destroy(rck_context* ctx);
 /* num_decomp_digits(total_modulus_bits(Q[0..limbs)), 30) (src/rns/keyswitching.rs:238-240) */
 int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digits);
+/* Keyswitch method of the keys created (or generated) from now on in this context; no counterpart in
+ * the crate, whose single method both produce bit for bit (keyswitching.rs:159-194).
+ * RCK_KS_EXACT_CONVOLUTION (default): hybrid keys at N >= 2^12 are kept as the NTT forms, under two
+ * auxiliary primes, of their centred limbs; a digit times a key limb is then an exact integer
+ * convolution modulo the two primes' product (every digit transformed twice instead of once per eval
+ * prime; the key takes twice the memory).  RCK_KS_DIGIT_NTT: keys kept in NTT form mod every eval
+ * prime, every digit transformed under every eval prime.  Q-only keys and N <= 2^11 always use the
+ * latter.  rck_ksk_method reports a key's method. */
+#define RCK_KS_EXACT_CONVOLUTION 0
+#define RCK_KS_DIGIT_NTT 1
+int rck_context_set_keyswitch_method(rck_context* ctx, int method);
+int rck_ksk_method(const rck_ksk* key, int* method);
 /* galois_element_for_rotation (src/rns/rotation.rs:14-26): 5^steps mod 2N, (5^-1)^|steps| if < 0 */
 uint64_t rck_galois_element_for_rotation(uint32_t poly_degree, int64_t steps);
 

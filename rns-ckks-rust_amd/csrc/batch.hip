@@ -192,13 +192,14 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
     const int E = (int)g.limbs + (int)g.key->aux;
     const int D = g.key->aux ? (int)((c.qbits[g.limbs] + 29) / 30) : (int)g.key->digits;
     const int B = (int)g.ops.size();
-    if (ks_tuned_find(c, (int)g.limbs, E, D, B, &g.shape, &g.tuned_ms)) return;
+    const bool exact = g.key->sdata != nullptr;
+    if (ks_tuned_find(c, (int)g.limbs, E, D, B, &g.shape, &g.tuned_ms, exact)) return;
     hipEvent_t e0, e1;
     RCK_CHECK_HIP(hipEventCreate(&e0));
     RCK_CHECK_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     KsShape best_shape{};
-    for (const KsShape& sh : ks_candidates(c, E, D, B)) {
+    for (const KsShape& sh : ks_candidates(c, E, D, B, exact)) {
         run_group(bt, g, bt->stream, sh);  // warm-up (sizes the scratch)
         RCK_CHECK_HIP(hipEventRecord(e0, bt->stream));
         const int reps = 2;
@@ -218,7 +219,7 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
     (void)hipEventDestroy(e1);
     g.shape = best_shape;
     g.tuned_ms = best;
-    ks_tuned_store(c, (int)g.limbs, E, D, B, best_shape, best);
+    ks_tuned_store(c, (int)g.limbs, E, D, B, best_shape, best, exact);
 }
 
 // Per-op tuning: time every candidate shape of keyswitch_batch for B ciphertexts at `limbs` limbs
@@ -243,7 +244,8 @@ static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsSh
         for (int b = 0; b < B; b++) outs.push_back(out + b * 2 * P);
         float best = 1e30f;
         KsShape bs{};
-        for (const KsShape& sh : ks_candidates(c, E, D, B)) {
+        const bool exact = key->sdata != nullptr;
+        for (const KsShape& sh : ks_candidates(c, E, D, B, exact)) {
             keyswitch_batch(c, in, P, B, limbs, key, nullptr, nullptr, 0, outs, s, sh);
             RCK_CHECK_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < 2; r++) keyswitch_batch(c, in, P, B, limbs, key, nullptr, nullptr, 0, outs, s, sh);
@@ -257,7 +259,7 @@ static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsSh
                 bs = sh;
             }
         }
-        ks_tuned_store(c, limbs, E, D, B, bs, best);
+        ks_tuned_store(c, limbs, E, D, B, bs, best, exact);
         *best_out = bs;
         *best_ms = best;
     } catch (...) {

@@ -139,6 +139,21 @@ int rck_context_destroy(rck_context* ctx) {
     });
 }
 
+int rck_context_set_keyswitch_method(rck_context* ctx, int method) {
+    return guard([&] {
+        Context& c = C(ctx);
+        RCK_REQUIRE(method == RCK_KS_EXACT_CONVOLUTION || method == RCK_KS_DIGIT_NTT, "unknown keyswitch method");
+        c.ks_method = method;
+    });
+}
+
+int rck_ksk_method(const rck_ksk* key, int* method) {
+    return guard([&] {
+        RCK_REQUIRE(key != nullptr && method != nullptr, "null argument");
+        *method = key->sdata ? RCK_KS_EXACT_CONVOLUTION : RCK_KS_DIGIT_NTT;
+    });
+}
+
 int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digits) {
     return guard([&] {
         const Context& c = C(const_cast<rck_context*>(ctx));
@@ -356,8 +371,10 @@ static void ksk_create(rck_context* ctx, const uint64_t* key, bool on_device, ui
         }
         RCK_CHECK_HIP(hipDeviceSynchronize());
         if (stage) RCK_CHECK_HIP(hipFree(stage));
+        if (c.ks_method == RCK_KS_EXACT_CONVOLUTION && exact_keyswitch_available(c, aux)) key_make_exact(c, k);
     } catch (...) {
         if (k->data) (void)hipFree(k->data);
+        if (k->sdata) (void)hipFree(k->sdata);
         delete k;
         throw;
     }
@@ -379,6 +396,7 @@ int rck_ksk_destroy(rck_ksk* key) {
         if (!key) return;
         RCK_CHECK_HIP(hipDeviceSynchronize());
         if (key->data) RCK_CHECK_HIP(hipFree(key->data));
+        if (key->sdata) RCK_CHECK_HIP(hipFree(key->sdata));
         delete key;
     });
 }

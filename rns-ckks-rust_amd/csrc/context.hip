@@ -1,5 +1,6 @@
 // Context construction: validation (src/rns/context/params.rs:48-73, src/rns/basis.rs:32-59,
 // src/math/ntt.rs:17-60), twiddle tables and basis-conversion constants uploaded to HBM.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -203,6 +204,31 @@ static uint32_t word_bits(const std::vector<uint32_t>& w) {
     return (uint32_t)(n - 1) * 32 + b;
 }
 
+// deterministic Miller-Rabin for 64-bit n (these bases suffice below 3.3 * 10^24)
+static bool h_is_prime(uint64_t n) {
+    if (n < 2) return false;
+    static const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (uint64_t p : bases)
+        if (n % p == 0) return n == p;
+    uint64_t d = n - 1;
+    int r = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        r++;
+    }
+    for (uint64_t a : bases) {
+        uint64_t x = h_powmod(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int i = 1; i < r && comp; i++) {
+            x = h_mulmod(x, x, n);
+            if (x == n - 1) comp = false;
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+
 template <typename T>
 static T* upload(const std::vector<T>& v) {
     T* d = nullptr;
@@ -233,6 +259,25 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
     for (size_t i = 0; i < all.size(); i++)
         for (size_t j = 0; j < i; j++)
             RCK_REQUIRE(all[i] != all[j], "duplicate RNS modulus " + std::to_string(all[i]));
+    // Two auxiliary NTT primes for the exact-convolution keyswitch (hybrid keys, N >= 2^12): the largest
+    // primes below 2^50 + 2^46 with s = 1 mod 2N outside Q || P, each with its smallest quadratic
+    // non-residue as generator (a non-residue g makes g^((s-1)/2N) a primitive 2N-th root of unity).
+    uint32_t S = 0;
+    if (K > 0 && n >= (1u << 12)) {
+        const uint64_t step = 2ull * n;
+        uint64_t cand = (((1ull << 50) + (1ull << 46) - 2) / step) * step + 1;
+        while (S < 2 && cand > step) {
+            if (h_is_prime(cand) && std::find(all.begin(), all.end(), cand) == all.end()) {
+                uint64_t g2 = 2;
+                while (h_powmod(g2, (cand - 1) / 2, cand) == 1) g2++;
+                all.push_back(cand);
+                roots.push_back(g2);
+                S++;
+            }
+            cand -= step;
+        }
+        RCK_REQUIRE(S == 2, "no auxiliary NTT primes for the exact-convolution keyswitch");
+    }
     Context* c = new Context();
     try {
         c->n = n;
@@ -240,10 +285,11 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
         while ((1u << c->logn) < n) c->logn++;
         c->L = L;
         c->K = K;
+        c->S = S;
         c->scale_bits = scale_bits;
-        const uint32_t T = L + K;
-        c->hp.resize(T);
-        for (uint32_t i = 0; i < T; i++) {
+        const uint32_t T = L + K, TS = T + S;
+        c->hp.resize(TS);
+        for (uint32_t i = 0; i < TS; i++) {
             // the FP64-exact products need |a*w/q| < 2^51 with lazily reduced inputs (modarith.hpp)
             RCK_REQUIRE(all[i] < (1ull << 50) + (1ull << 46),
                         "moduli must be below 2^50 + 2^46 (FP64-exact arithmetic)");
@@ -251,8 +297,8 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
             RCK_REQUIRE(err.empty(), "invalid RNS prime " + std::to_string(all[i]) + ": " + err);
         }
         // twiddle tables: per prime fw, iv (bit-reversed) and tw, twi (natural order)
-        std::vector<double> tables((size_t)T * 4 * n);
-        for (uint32_t i = 0; i < T; i++) {
+        std::vector<double> tables((size_t)TS * 4 * n);
+        for (uint32_t i = 0; i < TS; i++) {
             const HostPrime& h = c->hp[i];
             double* fw = &tables[(size_t)i * 4 * n];
             double *iv = fw + n, *tw = iv + n, *twi = tw + n;
@@ -273,14 +319,29 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
             }
         }
         c->d_tables = upload(tables);
-        std::vector<DevPrime> dp(T);
-        for (uint32_t i = 0; i < T; i++) {
+        std::vector<DevPrime> dp(TS);
+        for (uint32_t i = 0; i < TS; i++) {
             const HostPrime& h = c->hp[i];
             double* base = c->d_tables + (size_t)i * 4 * n;
             dp[i] = DevPrime{(double)h.q, 1.0 / (double)h.q, h_centred(h.n_inv, h.q), h.q, base, base + n,
                              base + 2 * n, base + 3 * n};
         }
         c->d_primes = upload(dp);
+        if (S == 2) {
+            // exact-convolution constants: s1^{-1} mod s2, and s1 mod p_j, s1 s2 mod p_j per eval prime
+            const uint64_t s1 = all[T], s2 = all[T + 1];
+            c->sp[0] = s1;
+            c->sp[1] = s2;
+            uint64_t inv;
+            RCK_REQUIRE(h_invmod(s1 % s2, s2, &inv), "auxiliary NTT primes must be coprime");
+            c->sp_inv12 = h_centred(inv, s2);
+            std::vector<double> spm((size_t)T * 2);
+            for (uint32_t j = 0; j < T; j++) {
+                spm[2 * j] = h_centred(s1 % all[j], all[j]);
+                spm[2 * j + 1] = h_centred(h_mulmod(s1 % all[j], s2 % all[j], all[j]), all[j]);
+            }
+            c->d_spmod = upload(spm);
+        }
         // inverse table q_j^{-1} mod q_i
         std::vector<double> invq((size_t)T * T, 0.0);
         for (uint32_t i = 0; i < T; i++)
@@ -398,6 +459,12 @@ Context* Context::scratch_clone() const {
     c->d_pmodq = d_pmodq;
     c->d_pinvq = d_pinvq;
     c->d_pmod_aux = d_pmod_aux;
+    c->S = S;
+    c->sp[0] = sp[0];
+    c->sp[1] = sp[1];
+    c->sp_inv12 = sp_inv12;
+    c->d_spmod = d_spmod;
+    c->ks_method = ks_method;
     c->tune_group = tune_group;
     c->tune_dsplit = tune_dsplit;
     c->owns_tables = false;
@@ -411,7 +478,7 @@ Context::~Context() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (aux) (void)hipStreamDestroy(aux);
     if (!owns_tables) return;
-    void* ptrs[] = {d_primes, d_tables, d_invq, d_qwords, d_pwords, d_pmodq, d_pinvq, d_pmod_aux};
+    void* ptrs[] = {d_primes, d_tables, d_invq, d_qwords, d_pwords, d_pmodq, d_pinvq, d_pmod_aux, d_spmod};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }

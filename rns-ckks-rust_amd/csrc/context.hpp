@@ -56,6 +56,14 @@ struct Context {
     double* d_pmodq = nullptr;     // [L] P mod q_i (centred)
     double* d_pinvq = nullptr;     // [L] P^{-1} mod q_i (centred)
     double* d_pmod_aux = nullptr;  // [K][L] p_k mod q_i (centred)
+    // Exact-convolution keyswitch: two auxiliary NTT primes s1, s2 (prime indices L+K, L+K+1 of
+    // d_primes / the tables, not part of any operand's basis).  A digit times a key limb is computed as
+    // an exact integer negacyclic convolution modulo s1 s2 and then reduced mod each eval prime.
+    uint32_t S = 0;                   // auxiliary primes present (0 or 2)
+    uint64_t sp[2] = {0, 0};
+    double sp_inv12 = 0;              // s1^{-1} mod s2 (centred)
+    double* d_spmod = nullptr;        // [L+K][2]: s1 mod p_j, s1 s2 mod p_j (centred), per eval prime
+    int ks_method = 0;                // keys created from now on: 0 exact convolution (hybrid, N >= 2^12), 1 digit NTT
     bool owns_tables = true;  // false for a scratch clone sharing another context's tables
     bool frozen = false;      // workspace may not grow (graph capture in progress)
     // keyswitch launch-shape tuning (0 = automatic): eval limbs per digit-MAC group, digit splits

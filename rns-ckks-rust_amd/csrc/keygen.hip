@@ -282,8 +282,10 @@ rck_ksk* keygen_switch(rck_context* ctx, rck_rng* r, double sigma, const uint64_
                 RCK_CHECK_HIP(hipDeviceSynchronize());
             }
         }
+        if (c.ks_method == RCK_KS_EXACT_CONVOLUTION && exact_keyswitch_available(c, (uint32_t)K)) key_make_exact(c, k);
     } catch (...) {
         if (k->data) (void)hipFree(k->data);
+        if (k->sdata) (void)hipFree(k->sdata);
         delete k;
         throw;
     }
@@ -469,11 +471,18 @@ int rck_keygen_rotation(rck_context* ctx, rck_rng* rng, double sigma, const uint
 int rck_ksk_export(rck_context* ctx, const rck_ksk* key, uint64_t* out) {
     return kguard([&] {
         Context& c = ctx_of(ctx);
-        RCK_REQUIRE(key != nullptr && key->data != nullptr, "null key");
+        RCK_REQUIRE(key != nullptr && (key->data != nullptr || key->sdata != nullptr), "null key");
         check_ptr(out, "out");
         const long long n = c.n, P = (long long)key->limbs * n;
         const PrimeMap pm{(int)key->level + 1, (int)c.L, 0};
         DevBuf t(2 * P * 8), u(2 * P * 8);
+        if (key->sdata) {
+            for (uint32_t d = 0; d < key->digits; d++) {
+                key_export_exact_digit(c, key, d, u.as<uint64_t>());
+                RCK_CHECK_HIP(hipMemcpy(out + (size_t)d * 2 * P, u.p, 2 * P * 8, hipMemcpyDeviceToHost));
+            }
+            return;
+        }
         for (uint32_t d = 0; d < key->digits; d++) {
             RCK_CHECK_HIP(hipMemcpy(t.p, key->data + (size_t)d * 2 * P, 2 * P * 8, hipMemcpyDeviceToDevice));
             launch_inv(c, t.as<double>(), n, P, kOutU64, u.p, n, P, (int)key->limbs, 2, pm, nullptr);

@@ -1,0 +1,414 @@
+// Keyswitch by exact integer convolution through two auxiliary NTT primes.
+//
+// keyswitch_rns_hybrid (src/rns/keyswitching.rs:159-194) needs, for every eval prime p_j of Q_l || P,
+//   acc_j = sum_i d_i * k_ij  mod (X^N + 1, p_j),
+// where the digits d_i (|d| <= 2^29) are the same integer polynomials for every p_j and k_ij is key
+// digit i on limb j.  Take k_ij as its centred integer polynomial (|k| <= (p_j - 1)/2): then every
+// coefficient of the INTEGER negacyclic sum V_j = sum_{i in a group} d_i * k_ij is bounded by
+//   |V_j| <= G * N * 2^29 * max_j (p_j - 1)/2,
+// and with G (digits per group) small enough that bound stays below (s1 s2 - 1)/2 for two auxiliary
+// NTT primes s1, s2 < 2^50 + 2^46.  V_j is then recovered exactly from its residues mod s1 and s2
+// (centred CRT), and V_j mod p_j summed over the groups is acc_j itself -- the same residues the
+// crate's per-prime products give, bit for bit.
+//
+// So each digit is transformed under s1 and s2 only (2 limb NTTs instead of one per eval prime), the
+// key is stored as the s1/s2 NTT forms of its centred limbs ([digit][comp][limb][s][N]), the products
+// are a per-coefficient multiply-accumulate over the digits of a group (every digit value reused for
+// all 2 (l + K) key columns, every key value for all ciphertexts of a batch), and the accumulators are
+// transformed back under s1, s2, recombined, reduced mod p_j and taken through mod-down by P.
+#include <algorithm>
+
+#include "moddown.hpp"
+#include "pipeline.hpp"
+
+namespace rck {
+
+// Constants of the recombination: the auxiliary primes, s1^{-1} mod s2, and the centring thresholds.
+struct SpecConst {
+    double s1, s1inv, s2, s2inv, inv12;  // inv12 = s1^{-1} mod s2 (centred)
+    double half_t, half_a;               // (s2 - 1)/2, (s1 - 1)/2
+};
+
+static SpecConst spec_const(const Context& c) {
+    SpecConst k;
+    k.s1 = (double)c.sp[0];
+    k.s1inv = 1.0 / k.s1;
+    k.s2 = (double)c.sp[1];
+    k.s2inv = 1.0 / k.s2;
+    k.inv12 = c.sp_inv12;
+    k.half_t = (double)((c.sp[1] - 1) / 2);
+    k.half_a = (double)((c.sp[0] - 1) / 2);
+    return k;
+}
+
+// V mod q for the integer V in (-(s1 s2)/2, (s1 s2)/2] with residues a1 = V mod s1, a2 = V mod s2
+// (canonical).  Garner: V = a1 + s1 t, t = (a2 - a1) s1^{-1} mod s2 in [0, s2), minus s1 s2 when
+// a1 + s1 t > (s1 s2 - 1)/2, i.e. t > (s2 - 1)/2, or t == (s2 - 1)/2 and a1 > (s1 - 1)/2.
+// p1m = s1 mod q, p12m = s1 s2 mod q (centred).  Returns a representative with |r| <= q/2 + 1.
+__device__ __forceinline__ double crt_mod_q(double a1, double a2, const SpecConst& K, double q, double qinv,
+                                            double p1m, double p12m) {
+    const double d = reduce(a2 - a1, K.s2, K.s2inv);                          // |a2 - a1| < 2^51
+    const double t = canon(mulmod(d, K.inv12, K.s2, K.s2inv), K.s2, K.s2inv);  // [0, s2)
+    const bool neg = t > K.half_t || (t == K.half_t && a1 > K.half_a);
+    double r = reduce(a1, q, qinv) + mulmod(reduce(t, q, qinv), p1m, q, qinv);  // <= q/2 + 1 + 0.97q
+    if (neg) r -= p12m;
+    return reduce(r, q, qinv);
+}
+
+// Centred key limbs reduced mod s1, s2: in [2][E][N] canonical mod p_j -> out [2][E][2][N] (reduced
+// doubles, the input of the forward NTT under s1, s2).
+__global__ void k_key_to_special(const uint64_t* __restrict__ in, double* __restrict__ out, int n, int E, int nq,
+                                 int L, int sbase, const DevPrime* __restrict__ primes) {
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int j = blockIdx.y, comp = blockIdx.z;
+    const DevPrime Pj = primes[prime_of(j, nq, L)];
+    const double v = load_centred(in[((long long)comp * E + j) * n + x], Pj.q);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const DevPrime Ps = primes[sbase + s];
+        out[(((long long)comp * E + j) * 2 + s) * n + x] = reduce(v, Ps.q, Ps.qinv);
+    }
+}
+
+// Inverse of k_key_to_special after the inverse NTT: in [2][E][2][N] canonical mod s -> out [2][E][N]
+// canonical mod p_j (the key's coefficient form).
+__global__ void k_key_from_special(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, int n, int E, int nq,
+                                   int L, SpecConst K, const double* __restrict__ spmod,
+                                   const DevPrime* __restrict__ primes) {
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int j = blockIdx.y, comp = blockIdx.z;
+    const int pj = prime_of(j, nq, L);
+    const DevPrime Pj = primes[pj];
+    const long long base = (((long long)comp * E + j) * 2) * n + x;
+    const double v = crt_mod_q((double)in[base], (double)in[base + n], K, Pj.q, Pj.qinv, spmod[2 * pj],
+                               spmod[2 * pj + 1]);
+    out[((long long)comp * E + j) * n + x] = f64_to_u64(canon(v, Pj.q, Pj.qinv));
+}
+
+// ---------------------------------------------------------------------------------------
+// Multiply-accumulate: for coefficient x, auxiliary prime s and digit group g,
+//   acc[g][b][jc][s][x] = sum_{i in g} dn[b][i][s][x] * key[i][jc][s][x]  mod s,  jc = comp * E + j.
+// A workgroup is four waves over 64 consecutive coefficients; wave w takes BT ciphertexts of its
+// b-tile and JT key columns of its j-tile (WB waves along the ciphertexts, 4 / WB along the
+// columns), so the waves of a workgroup read the same key values (served by the CU's caches) or the
+// same digit values.  Work items are numbered key-column group fastest, then coefficient block,
+// ciphertext group, prime and digit group, and each XCD takes a contiguous range of them (workgroup
+// w runs on XCD w % 8), so consecutive items on an XCD share their digit tiles in its L2.
+// ---------------------------------------------------------------------------------------
+template <int BT, int JT>
+__global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__ dn, const double* __restrict__ key,
+                                                      double* __restrict__ acc, int n, int B, int D, int gsize,
+                                                      int cols, int WB, int njg, int nxb, int nbg, int items,
+                                                      int sbase, const DevPrime* __restrict__ primes) {
+    const int w = blockIdx.x;
+    const int per = gridDim.x >> 3;
+    const int item = (w & 7) * per + (w >> 3);
+    if (item >= items) return;
+    int r = item;
+    const int jg = r % njg;
+    r /= njg;
+    const int xb = r % nxb;
+    r /= nxb;
+    const int bg = r % nbg;
+    r /= nbg;
+    const int s = r & 1, g = r >> 1;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int WJ = 4 / WB;
+    const int wb = wave % WB, wj = wave / WB;
+    const int b0 = (bg * WB + wb) * BT;
+    const int c0 = (jg * WJ + wj) * JT;
+    if (b0 >= B || c0 >= cols) return;
+    const long long x = (long long)xb * 64 + lane;
+    const DevPrime P = primes[sbase + s];
+    const double q = P.q, qinv = P.qinv;
+    const int i0 = g * gsize, i1 = min(D, i0 + gsize);
+    double a[BT][JT];
+#pragma unroll
+    for (int bt = 0; bt < BT; bt++)
+#pragma unroll
+        for (int jt = 0; jt < JT; jt++) a[bt][jt] = 0.0;
+    const long long n2 = 2LL * n;
+    int pend = 0;
+    for (int i = i0; i < i1; i++) {
+        double d[BT], k[JT];
+#pragma unroll
+        for (int bt = 0; bt < BT; bt++)
+            d[bt] = b0 + bt < B ? dn[(((long long)(b0 + bt) * D + i) * 2 + s) * n + x] : 0.0;
+#pragma unroll
+        for (int jt = 0; jt < JT; jt++)
+            k[jt] = c0 + jt < cols ? key[((long long)i * cols + c0 + jt) * n2 + (long long)s * n + x] : 0.0;
+#pragma unroll
+        for (int bt = 0; bt < BT; bt++)
+#pragma unroll
+            for (int jt = 0; jt < JT; jt++) a[bt][jt] += mulmod(d[bt], k[jt], q, qinv);
+        // reduced operands: each product is <= 0.57 s, so eight of them on a reduced accumulator stay
+        // below 5.1 s < 2^53 / s
+        if (++pend == 8) {
+            pend = 0;
+#pragma unroll
+            for (int bt = 0; bt < BT; bt++)
+#pragma unroll
+                for (int jt = 0; jt < JT; jt++) a[bt][jt] = reduce(a[bt][jt], q, qinv);
+        }
+    }
+#pragma unroll
+    for (int bt = 0; bt < BT; bt++) {
+        if (b0 + bt >= B) continue;
+#pragma unroll
+        for (int jt = 0; jt < JT; jt++) {
+            if (c0 + jt >= cols) continue;
+            acc[((((long long)g * B + b0 + bt) * cols + c0 + jt) * 2 + s) * n + x] = reduce(a[bt][jt], q, qinv);
+        }
+    }
+}
+
+template <int BT, int JT>
+static void mac_exact_t(const Context& c, const double* dn, const double* key, double* acc, int B, int D, int gsize,
+                        int G, int E, hipStream_t s) {
+    const int n = (int)c.n;
+    const int nbt = (B + BT - 1) / BT;  // b-tiles; WB of them per workgroup (1, 2 or 4 waves along them)
+    const int WB = nbt >= 3 ? 4 : nbt;
+    const int WJ = 4 / WB;
+    const int cols = 2 * E;
+    const int njt = (cols + JT - 1) / JT;
+    const int njg = (njt + WJ - 1) / WJ;
+    const int nxb = n / 64;
+    const int nbg = (B + WB * BT - 1) / (WB * BT);
+    const long long items = (long long)njg * nxb * nbg * 2 * G;
+    RCK_REQUIRE(items < (1LL << 30), "exact keyswitch launch too large");
+    const int per = (int)((items + 7) / 8);
+    ProfScope ps(s, "ks_mac");
+    k_ks_mac_exact<BT, JT><<<dim3(per * 8), 256, 0, s>>>(dn, key, acc, n, B, D, gsize, cols, WB, njg, nxb, nbg,
+                                                         (int)items, (int)(c.L + c.K), c.d_primes);
+    RCK_CHECK_HIP(hipGetLastError());
+}
+
+template <int BT>
+static void mac_exact_j(const Context& c, int jt, const double* dn, const double* key, double* acc, int B, int D,
+                        int gsize, int G, int E, hipStream_t s) {
+    if (jt == 4)
+        mac_exact_t<BT, 4>(c, dn, key, acc, B, D, gsize, G, E, s);
+    else if (jt == 16)
+        mac_exact_t<BT, 16>(c, dn, key, acc, B, D, gsize, G, E, s);
+    else
+        mac_exact_t<BT, 8>(c, dn, key, acc, B, D, gsize, G, E, s);
+}
+
+// bt ciphertexts by jt key columns per thread (bt in {1, 2, 4}, jt in {4, 8, 16})
+static void launch_mac_exact(const Context& c, int bt, int jt, const double* dn, const double* key, double* acc, int B,
+                             int D, int gsize, int G, int E, hipStream_t s) {
+    RCK_REQUIRE((bt == 1 || bt == 2 || bt == 4) && (jt == 4 || jt == 8 || jt == 16), "bad exact keyswitch tile");
+    if (bt == 4)
+        mac_exact_j<4>(c, jt, dn, key, acc, B, D, gsize, G, E, s);
+    else if (bt == 2)
+        mac_exact_j<2>(c, jt, dn, key, acc, B, D, gsize, G, E, s);
+    else
+        mac_exact_j<1>(c, jt, dn, key, acc, B, D, gsize, G, E, s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Recombination and mod-down: one thread per (coefficient x, component, ciphertext).
+//   x_j = sum_g crt(acc[g][b][comp][j][s1], acc[..][s2]) mod p_j for every eval limb j;
+//   the aux part through Garner (p_garner), then for each Q limb i
+//   out_i = (x_i - centred(x_P) mod q_i) P^{-1} mod q_i (+ addend)          (keyswitching.rs:278-322)
+// addends: add0 for component 0 (read through X -> X^g when ginv != 0, the rotated c0 of a rotation),
+// add1 for component 1, each at element offsets relative to ciphertext b (stride add_ps).
+// ---------------------------------------------------------------------------------------
+struct ExactOut {
+    uint64_t* p[32];
+};
+
+template <int KT>
+__global__ __launch_bounds__(256) void k_crt_moddown(const uint64_t* __restrict__ accc, int n, int nq, int E, int B,
+                                                     int G, int L, int T, SpecConst SK,
+                                                     const double* __restrict__ spmod,
+                                                     const uint64_t* __restrict__ add0,
+                                                     const uint64_t* __restrict__ add1, long long add_ps,
+                                                     unsigned long long ginv, ExactOut outs, int b0,
+                                                     const DevPrime* __restrict__ primes,
+                                                     const double* __restrict__ invq,
+                                                     const uint32_t* __restrict__ pwords,
+                                                     const double* __restrict__ pmodq,
+                                                     const double* __restrict__ pinvq,
+                                                     const double* __restrict__ pmod_aux) {
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int z = blockIdx.y, bl = z >> 1, comp = z & 1, b = b0 + bl;
+    const int cols = 2 * E;
+    // x_j for eval limb j (prime index pj), summed over the digit groups
+    auto xj = [&](int j, int pj) -> double {
+        const DevPrime Pj = primes[pj];
+        const double q = Pj.q, qinv = Pj.qinv, p1m = spmod[2 * pj], p12m = spmod[2 * pj + 1];
+        double v = 0.0;
+        for (int g = 0; g < G; g++) {
+            const long long base = ((((long long)g * B + b) * cols + comp * E + j) * 2) * n + x;
+            v += crt_mod_q((double)accc[base], (double)accc[base + n], SK, q, qinv, p1m, p12m);
+        }
+        return v;  // |v| <= G (q/2 + 1)
+    };
+    double r[KT], m[KT];
+#pragma unroll
+    for (int k = 0; k < KT; k++) {
+        const DevPrime Pk = primes[L + k];
+        r[k] = canon(xj(nq + k, L + k), Pk.q, Pk.qinv);
+    }
+    const bool gt = p_garner(r, KT, L, T, m, primes, invq, pwords);
+    const uint64_t* add = comp == 0 ? add0 : add1;
+    uint64_t* o = outs.p[bl] + (long long)comp * nq * n;
+    for (int i = 0; i < nq; i++) {
+        const DevPrime Pi = primes[i];
+        const double q = Pi.q, qinv = Pi.qinv;
+        const double xi = reduce(xj(i, i), q, qinv);
+        double cv = reduce(m[KT - 1], q, qinv);
+#pragma unroll
+        for (int kk = KT - 2; kk >= 0; kk--)
+            cv = reduce(mulmod(cv, pmod_aux[(long long)kk * L + i], q, qinv) + m[kk], q, qinv);
+        if (gt) cv -= pmodq[i];
+        double res = mulmod(reduce(xi - cv, q, qinv), pinvq[i], q, qinv);
+        if (add) {
+            const uint64_t* ad = add + (long long)b * add_ps + (long long)i * n;
+            if (comp == 0 && ginv) {
+                const unsigned long long k = ((unsigned long long)x * ginv) & (2ull * n - 1);
+                const uint64_t v = ad[k >= (unsigned long long)n ? (long long)k - n : (long long)k];
+                res += k >= (unsigned long long)n && v != 0 ? q - (double)v : (double)v;
+            } else {
+                res += (double)ad[x];
+            }
+        }
+        o[(long long)i * n + x] = f64_to_u64(canon(res, q, qinv));
+    }
+}
+
+static void launch_crt_moddown(const Context& c, const uint64_t* accc, int nq, int B, int G, const uint64_t* add0,
+                               const uint64_t* add1, long long add_ps, unsigned long long ginv,
+                               const std::vector<uint64_t*>& outs, hipStream_t s) {
+    const int E = nq + (int)c.K;
+    const SpecConst SK = spec_const(c);
+    for (int b0 = 0; b0 < B; b0 += 32) {
+        const int bc = std::min(32, B - b0);
+        ExactOut op{};
+        for (int b = 0; b < bc; b++) op.p[b] = outs[b0 + b];
+        dim3 grid((c.n + 255) / 256, 2 * bc);
+        ProfScope ps(s, "crt_moddown");
+#define RCK_CM(KK)                                                                                             \
+    k_crt_moddown<KK><<<grid, 256, 0, s>>>(accc, (int)c.n, nq, E, B, G, (int)c.L, (int)(c.L + c.K), SK, c.d_spmod, \
+                                           add0, add1, add_ps, ginv, op, b0, c.d_primes, c.d_invq, c.d_pwords,  \
+                                           c.d_pmodq, c.d_pinvq, c.d_pmod_aux)
+        switch (c.K) {
+            case 1: RCK_CM(1); break;
+            case 2: RCK_CM(2); break;
+            case 3: RCK_CM(3); break;
+            default: RCK_CM(4); break;
+        }
+#undef RCK_CM
+        RCK_CHECK_HIP(hipGetLastError());
+    }
+}
+
+// Digits per group: the largest G with G N 2^29 max_j (p_j - 1)/2 <= (s1 s2 - 1)/2 over the eval
+// primes of `nq` Q limbs and the aux basis.
+int exact_group_size(const Context& c, int nq) {
+    using u128 = unsigned __int128;
+    uint64_t maxhalf = 0;
+    for (int j = 0; j < nq; j++) maxhalf = std::max(maxhalf, (c.hp[j].q - 1) / 2);
+    for (uint32_t k = 0; k < c.K; k++) maxhalf = std::max(maxhalf, (c.hp[c.L + k].q - 1) / 2);
+    const u128 lim = ((u128)c.sp[0] * c.sp[1] - 1) / 2;
+    const u128 per = (u128)c.n * ((u128)1 << 29) * maxhalf;
+    const u128 g = lim / per;
+    RCK_REQUIRE(g >= 1, "auxiliary primes too small for one digit");
+    return (int)std::min<u128>(g, 1 << 20);
+}
+
+bool exact_keyswitch_available(const Context& c, uint32_t aux) { return c.S == 2 && aux > 0 && c.logn >= 12; }
+
+// Converts a key from its NTT form mod the eval primes (data) to the exact-convolution form (sdata);
+// data is released.
+void key_make_exact(Context& c, KeyDev* k) {
+    RCK_REQUIRE(k && k->data, "null key");
+    RCK_REQUIRE(exact_keyswitch_available(c, k->aux), "exact-convolution keys need hybrid keys at N >= 2^12");
+    const long long n = c.n;
+    const int E = (int)k->limbs, nq = (int)k->level + 1;
+    const long long P = (long long)E * n;
+    double* sd = nullptr;
+    uint64_t* coeff = nullptr;
+    double* stage = nullptr;  // 2 P doubles (inverse NTT input), then 4 P (the centred limbs mod s1, s2)
+    try {
+        RCK_CHECK_HIP(hipMalloc(&sd, (size_t)k->digits * 2 * P * 2 * 8));
+        RCK_CHECK_HIP(hipMalloc(&coeff, (size_t)2 * P * 8));
+        RCK_CHECK_HIP(hipMalloc(&stage, (size_t)4 * P * 8));
+        const PrimeMap pm{nq, (int)c.L, 0};
+        const PrimeMap ps{0, (int)(c.L + c.K), 0};
+        for (uint32_t d = 0; d < k->digits; d++) {
+            RCK_CHECK_HIP(hipMemcpy(stage, k->data + (size_t)d * 2 * P, 2 * P * 8, hipMemcpyDeviceToDevice));
+            launch_inv(c, stage, n, P, kOutU64, coeff, n, P, E, 2, pm, nullptr);
+            k_key_to_special<<<dim3((c.n + 255) / 256, E, 2), 256>>>(coeff, stage, (int)c.n, E, nq, (int)c.L,
+                                                                    (int)(c.L + c.K), c.d_primes);
+            RCK_CHECK_HIP(hipGetLastError());
+            launch_fwd(c, kInF64, stage, n, 2 * n, sd + (size_t)d * 2 * P * 2, n, 2 * n, 2, 2 * E, ps, nullptr);
+        }
+        RCK_CHECK_HIP(hipDeviceSynchronize());
+    } catch (...) {
+        if (sd) (void)hipFree(sd);
+        if (coeff) (void)hipFree(coeff);
+        if (stage) (void)hipFree(stage);
+        throw;
+    }
+    (void)hipFree(coeff);
+    (void)hipFree(stage);
+    RCK_CHECK_HIP(hipFree(k->data));
+    k->data = nullptr;
+    k->sdata = sd;
+}
+
+// The crate's coefficient form of digit d of an exact-convolution key: out [2][E][N] (device).
+void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out) {
+    const long long n = c.n;
+    const int E = (int)k->limbs, nq = (int)k->level + 1;
+    const long long P = (long long)E * n;
+    double* stage = nullptr;
+    uint64_t* coeff = nullptr;
+    try {
+        RCK_CHECK_HIP(hipMalloc(&stage, (size_t)2 * P * 2 * 8));
+        RCK_CHECK_HIP(hipMalloc(&coeff, (size_t)2 * P * 2 * 8));
+        RCK_CHECK_HIP(hipMemcpy(stage, k->sdata + (size_t)d * 2 * P * 2, 2 * P * 2 * 8, hipMemcpyDeviceToDevice));
+        launch_inv(c, stage, n, 2 * n, kOutU64, coeff, n, 2 * n, 2, 2 * E, PrimeMap{0, (int)(c.L + c.K), 0}, nullptr);
+        k_key_from_special<<<dim3((c.n + 255) / 256, E, 2), 256>>>(coeff, out, (int)c.n, E, nq, (int)c.L,
+                                                                  spec_const(c), c.d_spmod, c.d_primes);
+        RCK_CHECK_HIP(hipGetLastError());
+        RCK_CHECK_HIP(hipDeviceSynchronize());
+    } catch (...) {
+        if (stage) (void)hipFree(stage);
+        if (coeff) (void)hipFree(coeff);
+        throw;
+    }
+    (void)hipFree(stage);
+    (void)hipFree(coeff);
+}
+
+// Keyswitch of B polynomials (stride in_ps) at `limbs` Q limbs with an exact-convolution key, `levels`
+// digits; adds add0/add1 and writes outs[b] = [c0][c1] (keyswitch_batch's contract).
+void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
+                     const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
+                     hipStream_t s, unsigned long long ginv, int bt, int jt) {
+    const long long n = c.n;
+    const int E = limbs + (int)key->aux;
+    const int gsize = exact_group_size(c, limbs);
+    const int G = (levels + gsize - 1) / gsize;
+    const int gs = (levels + G - 1) / G;  // balanced groups
+    int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
+    // workspace slots 30-32 (slots 20-26 hold the BSGS buffers of a transform whose rotations come here)
+    double* dn = static_cast<double*>(c.workspace(30, (size_t)B * levels * 2 * n * 8));
+    double* acc = static_cast<double*>(c.workspace(31, (size_t)G * B * 2 * E * 2 * n * 8));
+    uint64_t* accc = static_cast<uint64_t*>(c.workspace(32, (size_t)G * B * 2 * E * 2 * n * 8));
+    launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps, ginv);
+    const PrimeMap ps{0, (int)(c.L + c.K), 0};
+    // every digit under s1 and s2: limb stride 0 on the input (the same digit row for both)
+    launch_fwd(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
+    launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, gs, G, E, s);
+    launch_inv(c, acc, n, 2 * n, kOutU64, accc, n, 2 * n, 2, G * B * 2 * E, ps, s);
+    launch_crt_moddown(c, accc, limbs, B, G, add0, add1, add_ps, ginv, outs, s);
+}
+
+}  // namespace rck

@@ -121,7 +121,9 @@ void rotate_batch(Context& c, const std::vector<const uint64_t*>& as, int limbs,
     keyswitch_batch(c, rot + P, 2 * P, B, limbs, key, rot, nullptr, 2 * P, outs, s, shape);
 }
 
-bool mult_relin_folds(const Context& c, const KeyDev* key) { return c.logn >= 12 && key && key->aux > 0; }
+bool mult_relin_folds(const Context& c, const KeyDev* key) {
+    return c.logn >= 12 && key && key->aux > 0 && key->data != nullptr;
+}
 
 void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
                       const KeyDev* key, const std::vector<uint64_t*>& outs, hipStream_t s, KsShape shape) {
@@ -195,17 +197,18 @@ void run_op_group(Context& c, int kind, int l, const KeyDev* key, const std::vec
 // otherwise a heuristic that keeps >= ~2048 MAC workgroups in flight.
 namespace {
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int, int>, std::pair<KsShape, float>> g_tuned;
+// key: (logn, limbs, eval limbs, digits, batch, method: 0 digit NTT, 1 exact convolution)
+std::map<std::tuple<int, int, int, int, int, int>, std::pair<KsShape, float>> g_tuned;
 }  // namespace
 
-void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape sh, float ms) {
+void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape sh, float ms, bool exact) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[std::make_tuple((int)c.logn, limbs, E, D, B)] = {sh, ms};
+    g_tuned[std::make_tuple((int)c.logn, limbs, E, D, B, exact ? 1 : 0)] = {sh, ms};
 }
 
-bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* sh, float* ms) {
+bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* sh, float* ms, bool exact) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    auto it = g_tuned.find(std::make_tuple((int)c.logn, limbs, E, D, B));
+    auto it = g_tuned.find(std::make_tuple((int)c.logn, limbs, E, D, B, exact ? 1 : 0));
     if (it == g_tuned.end()) return false;
     *sh = it->second.first;
     if (ms) *ms = it->second.second;
@@ -228,7 +231,7 @@ std::string ks_tuned_export() {
 
 int ks_tuned_import(const std::string& text) {
     // parse everything first: a malformed table changes nothing
-    std::vector<std::pair<std::tuple<int, int, int, int, int>, std::pair<KsShape, float>>> rows;
+    std::vector<std::pair<std::tuple<int, int, int, int, int, int>, std::pair<KsShape, float>>> rows;
     size_t pos = 0;
     while (pos < text.size()) {
         size_t end = text.find('\n', pos);
@@ -245,19 +248,31 @@ int ks_tuned_import(const std::string& text) {
         RCK_REQUIRE(v[1] >= 1 && v[2] >= v[1] && v[3] >= 1 && v[4] >= 1, "tuning table entry out of range: " + line);
         const bool w16 = v[9] == 16 && (v[7] == 4 || v[7] == 16) && v[8] >= 0 && (v[8] <= 4 || ((v[8] == 5 || v[8] == 6) && v[7] == 4));
         const bool w8 = v[9] == 8 && (v[7] == 8 || v[7] == 16) && v[8] == 0;
-        RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (w16 || w8),
+        // exact-convolution entries: load schedule kExactShape, rows = ciphertexts per thread, width = key
+        // columns per thread
+        const bool ex = v[8] == kExactShape && v[5] == v[2] && v[6] == 1 && (v[7] == 1 || v[7] == 2 || v[7] == 4) &&
+                        (v[9] == 4 || v[9] == 8 || v[9] == 16);
+        RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (w16 || w8 || ex),
                     "tuning table launch shape out of range: " + line);
-        rows.push_back(
-            {std::make_tuple(v[0], v[1], v[2], v[3], v[4]), {KsShape{v[5], v[6], v[7], v[8], v[9]}, (float)ms}});
+        rows.push_back({std::make_tuple(v[0], v[1], v[2], v[3], v[4], ex ? 1 : 0),
+                        {KsShape{v[5], v[6], v[7], v[8], v[9]}, (float)ms}});
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const auto& r : rows) g_tuned[r.first] = r.second;
     return (int)rows.size();
 }
 
-KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B) {
+KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B, bool exact) {
     KsShape r;
-    if (ks_tuned_find(c, limbs, E, D, B, &r, nullptr)) return r;
+    if (ks_tuned_find(c, limbs, E, D, B, &r, nullptr, exact)) return r;
+    if (exact) {  // four ciphertexts by eight key columns per thread from three ciphertexts up
+        r.group = E;
+        r.dsplit = 1;
+        r.rows = B >= 3 ? 4 : B;
+        r.prefetch = kExactShape;
+        r.width = 8;
+        return r;
+    }
     r.group = c.tune_group > 0 ? c.tune_group : E;
     r.prefetch = 0;
     if (c.logn <= 11) {
@@ -273,8 +288,15 @@ KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B) {
     return r;
 }
 
-std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B) {
+std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B, bool exact) {
     std::vector<KsShape> out;
+    if (exact) {
+        for (int bt : {1, 2, 4}) {
+            if (bt > 1 && bt >= 2 * B) continue;
+            for (int jt : {4, 8, 16}) out.push_back({E, 1, bt, kExactShape, jt});
+        }
+        return out;
+    }
     if (c.logn <= 11) {
         out.push_back({E, 1, 4, 0});
         return out;
@@ -319,6 +341,12 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
         levels = active;  // keyswitching.rs:170
     }
     const int naux = key->aux;
+    if (key->sdata) {
+        RCK_REQUIRE(addn == nullptr, "folded addends need a digit-NTT key");
+        const KsShape sh = shape.prefetch == kExactShape ? shape : ks_auto_shape(c, limbs, limbs + naux, levels, B, true);
+        keyswitch_exact(c, poly, in_ps, B, limbs, key, levels, add0, add1, add_ps, outs, s, ginv, sh.rows, sh.width);
+        return;
+    }
     RCK_REQUIRE(addn == nullptr || (naux > 0 && c.logn >= 12), "folded addends need hybrid keys and the two-pass NTT");
     RCK_REQUIRE(ginv == 0 || (naux > 0 && c.logn >= 12), "the fused automorphism needs hybrid keys and the two-pass NTT");
     const int E = limbs + naux;

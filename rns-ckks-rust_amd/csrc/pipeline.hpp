@@ -8,9 +8,21 @@ namespace rck {
 // over level+1 Q-limbs followed by aux limbs (hybrid) or none (Q-only).
 struct KeyDev {
     double* data = nullptr;
+    // exact-convolution form instead (hybrid keys, N >= 2^12): sdata[digit][comp][limb][s][N], the NTT
+    // forms under the two auxiliary primes s of the key's centred limbs (data is then null)
+    double* sdata = nullptr;
     uint32_t digits = 0, level = 0, aux = 0, limbs = 0;
     uint64_t galois = 0;
 };
+
+// exact-convolution keyswitch (ks_exact.hip)
+bool exact_keyswitch_available(const Context& c, uint32_t aux);
+int exact_group_size(const Context& c, int nq);
+void key_make_exact(Context& c, KeyDev* k);
+void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out);
+void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
+                     const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
+                     hipStream_t s, unsigned long long ginv, int bt, int jt);
 
 // Launch shape of the digit MAC: eval limbs per group and digit splits (0 = automatic).
 struct KsShape {
@@ -20,6 +32,9 @@ struct KsShape {
     int prefetch = -1; // load schedule: 0 late keys, 1 early keys, 2 next digit's rows prefetched, 3 keys in groups
     int width = 16;    // elements per thread in the row pass (16, or 8 with rows 8 / 16)
 };
+// load schedule value of an exact-convolution shape: rows = ciphertexts and width = key columns per
+// thread of its multiply-accumulate
+constexpr int kExactShape = 7;
 
 inline void check_ptr(const void* p, const char* what) {
     RCK_REQUIRE(p != nullptr, std::string("null ") + what);
@@ -60,15 +75,15 @@ void run_op_group(Context& c, int kind, int limbs, const KeyDev* key, const std:
                   KsShape shape = {});
 // the automatic digit-MAC shape for B ciphertexts with E eval limbs and D digits: a tuned entry
 // when one exists for (N, limbs, E, D, B), otherwise a heuristic
-KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B);
-// tuned-shape table shared by the per-op calls and every batch of the process
-void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape s, float ms);
-bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* s, float* ms);
+KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B, bool exact = false);
+// tuned-shape table shared by the per-op calls and every batch of the process (one entry per method)
+void ks_tuned_store(const Context& c, int limbs, int E, int D, int B, KsShape s, float ms, bool exact = false);
+bool ks_tuned_find(const Context& c, int limbs, int E, int D, int B, KsShape* s, float* ms, bool exact = false);
 // The table as text, one entry per line: "logn limbs E D B group dsplit rows prefetch width ms"; import
 // validates every line before merging (returns the number of entries merged).
 std::string ks_tuned_export();
 int ks_tuned_import(const std::string& text);
 // candidates the autotuner times for a shape
-std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B);
+std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B, bool exact = false);
 
 }  // namespace rck

@@ -17,6 +17,9 @@ vp = ctypes.c_void_p
 u32, u64, i64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64
 u32p, u64p = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)
 
+KS_EXACT_CONVOLUTION = 0  # keyswitch methods (rck_context_set_keyswitch_method)
+KS_DIGIT_NTT = 1
+
 EXPORTS = {
     "rck_last_error": (ctypes.c_char_p, []),
     "rck_version": (ctypes.c_int, []),
@@ -37,6 +40,8 @@ EXPORTS = {
     "rck_context_create": (ctypes.c_int, [u32, u64p, u64p, u32, u64p, u64p, u32, u32, ctypes.POINTER(vp)]),
     "rck_context_destroy": (ctypes.c_int, [vp]),
     "rck_num_decomp_digits": (ctypes.c_int, [vp, u32, u32p]),
+    "rck_context_set_keyswitch_method": (ctypes.c_int, [vp, ctypes.c_int]),
+    "rck_ksk_method": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
     "rck_galois_element_for_rotation": (u64, [u32, i64]),
     "rck_ntt_forward": (ctypes.c_int, [vp, vp, u32, u32, vp]),
     "rck_ntt_inverse": (ctypes.c_int, [vp, vp, u32, u32, vp]),
@@ -450,6 +455,16 @@ class Context:
 
     def rescale_quadratic(self, quad, limbs, scale, out):
         return self._scaled(lib().rck_rescale_quadratic, self.ptr, _p(quad), limbs, scale, _p(out))
+
+    def set_keyswitch_method(self, method):
+        """Keyswitch method of the keys created from now on (KS_EXACT_CONVOLUTION or KS_DIGIT_NTT)."""
+        _check(lib().rck_context_set_keyswitch_method(self.ptr, method))
+
+    @staticmethod
+    def ksk_method(ksk):
+        m = ctypes.c_int()
+        _check(lib().rck_ksk_method(ksk.ptr, ctypes.byref(m)))
+        return m.value
 
     def ksk_create(self, key, level, aux, galois=0):
         """key: numpy uint64 [digits][2][level+1+aux][N] (host) or a DeviceBuffer of that shape."""

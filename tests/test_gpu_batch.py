@@ -185,7 +185,9 @@ def test_headline_workload_batched_matches_per_op_and_oracle():
     D = ctx.num_decomp_digits(L)
     assert D == 40
     key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
-    k = ctx.ksk_create(key, L - 1, K)
+    k = ctx.ksk_create(key, L - 1, K)  # exact-convolution key (the default)
+    ctx.set_keyswitch_method(G.KS_DIGIT_NTT)
+    kc = ctx.ksk_create(key, L - 1, K)
     P = L * n
     lhs_h = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
     rhs_h = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
@@ -201,12 +203,13 @@ def test_headline_workload_batched_matches_per_op_and_oracle():
     eb = O.Basis(n, p.primes + p.aux_primes, p.roots + p.aux_roots)
     O.set_threads(16)
     assert np.array_equal(want[0], O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs_h[0], rhs_h[0]), key))
-    for pf in (5, 6):
-        G.tune_table_import(f"16 {L} {L + K} {D} {B} {L + K} 1 4 {pf} 16 1.0\n")
+    for pf in (5, 6, 7):  # the streamed digit-NTT MACs, then the exact-convolution method
+        key_used = k if pf == 7 else kc
+        G.tune_table_import(f"16 {L} {L + K} {D} {B} {L + K} 1 4 {pf} {8 if pf == 7 else 16} 1.0\n")
         out = G.DeviceBuffer((B, 2, L, n))
         bt = G.Batch(ctx)
         for b in range(B):
-            bt.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), k, out.offset(b * 2 * P))
+            bt.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), key_used, out.offset(b * 2 * P))
         bt.compile(tune=True)
         assert bt.report()["groups"][0]["load_schedule"] == pf
         bt.run()

@@ -1,7 +1,8 @@
 """Every digit-MAC launch shape the autotuner may pick gives the same keyswitch bit for bit:
 row-pass width 16 (rows 4 / 16, every load schedule) and width 8 (rows 8 / 16), eval-limb groups
-and digit splits, for every row length (ring degrees 2^12 .. 2^16).  The default shape is pinned
-to the oracle by the parity tests; here each shape is forced through the tuning table."""
+and digit splits, for every row length (ring degrees 2^12 .. 2^16), with digit-NTT keys; and every
+multiply-accumulate tile of the exact-convolution method.  The default shapes are pinned to the
+oracle by the parity tests; here each shape is forced through the tuning table."""
 import numpy as np
 import pytest
 
@@ -35,7 +36,9 @@ def test_mac_shapes_bit_identical(logn, limbs):
     D = ctx.num_decomp_digits(limbs)
     E = limbs + 2
     key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    ctx.set_keyswitch_method(G.KS_DIGIT_NTT)
     k = ctx.ksk_create(key, limbs - 1, 2)
+    assert ctx.ksk_method(k) == G.KS_DIGIT_NTT
     quad = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 3))
     out = G.DeviceBuffer((2, limbs, n))
     ctx.relinearize(quad, limbs, 80, k, out)
@@ -61,6 +64,7 @@ def test_mac_streamed_batch_bit_identical(logn, limbs):
     D = ctx.num_decomp_digits(limbs)
     E = limbs + 2
     key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    ctx.set_keyswitch_method(G.KS_DIGIT_NTT)
     k = ctx.ksk_create(key, limbs - 1, 2)
     P = limbs * n
     lhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B).reshape(B, 2, limbs, n))
@@ -83,3 +87,45 @@ def test_mac_streamed_batch_bit_identical(logn, limbs):
         got = out.numpy()
         for b in range(B):
             assert np.array_equal(got[b], want[b]), (ds, pf, b)
+
+
+@pytest.mark.parametrize("logn,limbs,B", [(12, 4, 1), (13, 6, 5), (16, 5, 3)])
+def test_exact_mac_tiles_bit_identical(logn, limbs, B):
+    """Every multiply-accumulate tile of the exact-convolution keyswitch (ciphertexts 1 / 2 / 4 by key
+    columns 4 / 8 / 16 per thread), forced through the tuning table, gives the digit-NTT result."""
+    n = 1 << logn
+    p = PR.config_batched(n=1 << 16, limbs=limbs, aux=2)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    rs = np.random.default_rng(300 + logn)
+    D = ctx.num_decomp_digits(limbs)
+    E = limbs + 2
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    ctx.set_keyswitch_method(G.KS_DIGIT_NTT)
+    kc = ctx.ksk_create(key, limbs - 1, 2)
+    ctx.set_keyswitch_method(G.KS_EXACT_CONVOLUTION)
+    ke = ctx.ksk_create(key, limbs - 1, 2)
+    assert ctx.ksk_method(ke) == G.KS_EXACT_CONVOLUTION
+    P = limbs * n
+    lhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B).reshape(B, 2, limbs, n))
+    rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B).reshape(B, 2, limbs, n))
+    quad = G.DeviceBuffer((3, limbs, n))
+    one = G.DeviceBuffer((2, limbs, n))
+    want = []
+    for b in range(B):
+        ctx.mult(lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), limbs, 40, 40, quad)
+        ctx.relinearize(quad, limbs, 80, kc, one)
+        want.append(one.numpy())
+    for bt in (1, 2, 4):
+        for jt in (4, 8, 16):
+            G.tune_table_import(f"{logn} {limbs} {E} {D} {B} {E} 1 {bt} 7 {jt} 1.0\n")
+            out = G.DeviceBuffer((B, 2, limbs, n))
+            batch = G.Batch(ctx)
+            for b in range(B):
+                batch.add(G.OP_MULT_RELIN, limbs, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ke,
+                          out.offset(b * 2 * P))
+            batch.compile(tune=True)
+            grp = batch.report()["groups"][0]
+            assert (grp["mac_rows"], grp["load_schedule"]) == (bt, 7)
+            got = out.numpy()
+            for b in range(B):
+                assert np.array_equal(got[b], want[b]), (bt, jt, b)
