@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=32.0, help="serve: working-set budget (GB)")
     ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent stream lanes")
     ap.add_argument("--batch", type=int, default=16, help="ciphertext pairs per batch")
+    ap.add_argument("--method", default="exact", choices=["exact", "digit-ntt"],
+                    help="keyswitch method of the measured keys (the other is timed beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None,
                     help="oracle threads (default: the host cores this process may use)")
@@ -268,15 +270,24 @@ def main():
     rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B))
     log(f"[bench] relin key: {D} digits x 2 x {L + K} limbs")
     key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    # the measured keys use --method; the other method's key is timed beside it (not in counter runs)
+    methods = {"exact": G.KS_EXACT_CONVOLUTION, "digit-ntt": G.KS_DIGIT_NTT}
+    other = "digit-ntt" if args.method == "exact" else "exact"
+    ctx.set_keyswitch_method(methods[args.method])
     ksk = ctx.ksk_create(key, L - 1, K)
+    ksk_alt = None
+    if not args.counter_run:
+        ctx.set_keyswitch_method(methods[other])
+        ksk_alt = ctx.ksk_create(key, L - 1, K)
+        ctx.set_keyswitch_method(methods[args.method])
     del key
     quad = G.DeviceBuffer((3, L, n))
     out = G.DeviceBuffer((B, 2, L, n))
     one = G.DeviceBuffer((2, L, n))
 
-    def per_op_pair(b, dst):
+    def per_op_pair(b, dst, k=None):
         ctx.mult(lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), L, 40, 40, quad)
-        ctx.relinearize(quad, L, 80, ksk, dst)
+        ctx.relinearize(quad, L, 80, k if k is not None else ksk, dst)
 
     def per_op_step():
         per_op_pair(0, one)
@@ -294,6 +305,8 @@ def main():
         log(f"[bench] per-op keyswitch tuned: { {k: v for k, v in tune1.items() if k != 'candidates'} }")
         for cand in sorted(tune1["candidates"], key=lambda c: c["ms"])[:8]:
             log(f"[bench]     candidate {cand}")
+        if ksk_alt is not None:
+            ctx.tune_keyswitch(L, ksk_alt, 1)
 
     bt = None
     if args.mode == "batched":
@@ -326,11 +339,19 @@ def main():
         per_op_step()
     per_op_steps = args.steps if args.mode == "per-op" else max(4, args.steps // 2)
     per_op_ms = time_it(G, per_op_step, per_op_steps) / per_op_steps
-    per_op_value = 1000.0 / per_op_ms
-    log(f"[bench] per-op: {per_op_ms:.3f} ms/op = {per_op_value:.1f} ops/s")
+    per_op_by = {args.method: round(1000.0 / per_op_ms, 2)}
+    if ksk_alt is not None:
+        for _ in range(args.warmup):
+            per_op_pair(0, one, ksk_alt)
+        ms_alt = time_it(G, lambda: per_op_pair(0, one, ksk_alt), per_op_steps) / per_op_steps
+        per_op_by[other] = round(1000.0 / ms_alt, 2)
+    # the plain per-op path: the faster of the two methods
+    per_op_value = max(per_op_by.values())
+    log(f"[bench] per-op: {per_op_by} ops/s")
 
     prof_steps = 2
     report = None
+    batched_by = None
     value_rescale = None
     bit_exact = None
     sclk = None
@@ -378,6 +399,19 @@ def main():
         value_rescale = B * 1000.0 / ms_r
         log(f"[bench] batched with rescale: {ms_r:.3f} ms per batch of {B} = {value_rescale:.1f} ops/s")
         del btr, out_r
+        # the same batch with the other method's key (same outputs)
+        out_a = G.DeviceBuffer((B, 2, L, n))
+        bta = G.Batch(ctx)
+        for b in range(B):
+            bta.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ksk_alt, out_a.offset(b * 2 * P))
+        bta.compile(tune=not args.no_tune)
+        for _ in range(args.warmup):
+            bta.run(stream)
+        ms_a = time_it(G, lambda: bta.run(stream), args.steps, stream) / args.steps
+        batched_by = {args.method: round(value, 2), other: round(B * 1000.0 / ms_a, 2)}
+        bit_exact = bit_exact and np.array_equal(out_a.numpy(), got)
+        log(f"[bench] batched by method: {batched_by} ops/s; outputs equal: {bit_exact}")
+        del bta, out_a
     else:
         ms_per_step, value = per_op_ms, per_op_value
         prof = profile_kernels(G, per_op_step, prof_steps)
@@ -422,13 +456,19 @@ def main():
     rec = counter_record(args.mode, n, L, K, D, B)
     roofline = None
     alg_op = alg_bytes_per_op(n, L, K, D, B)
-    if name == "digit_mac":
-        G_ = -(-(L + K) // launches)  # eval limbs per launch
-        ds = report["groups"][0]["digit_splits"] if report else tune1["digit_splits"]
-        if ds is None:  # automatic shape without tuning: the library's heuristic for one ciphertext
-            ds = 1
-        folded = report is not None and report["groups"][0]["load_schedule"] in (5, 6) and ds == 1
-        io = int(mac_bytes(n, D, G_, B, ds, L / launches if folded else 0))
+    if name in ("digit_mac", "ks_mac"):
+        if name == "ks_mac":
+            # exact-convolution MAC: key read once (2 D E limbs under two auxiliary primes), digit NTTs read
+            # (B D under two primes), accumulators written (G groups x B x 2 E under two primes)
+            groups = ctx.exact_group_size(L)[1]
+            io = int((2 * D * (L + K) * 2 + B * D * 2 + groups * B * 2 * (L + K) * 2) * n * 8 / launches)
+        else:
+            G_ = -(-(L + K) // launches)  # eval limbs per launch
+            ds = report["groups"][0]["digit_splits"] if report else tune1["digit_splits"]
+            if ds is None:  # automatic shape without tuning: the library's heuristic for one ciphertext
+                ds = 1
+            folded = report is not None and report["groups"][0]["load_schedule"] in (5, 6) and ds == 1
+            io = int(mac_bytes(n, D, G_, B, ds, L / launches if folded else 0))
         alg = int(alg_op * ops_per_step / launches)
         achieved = alg / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -482,7 +522,9 @@ def main():
         "dtype": "u64 residues (FP64-exact modular arithmetic)", "data": "synthetic (seeded uniform residues)",
         "config": {"workload": f"{args.mode} mult+relinearize, N=2^16, L=24 primes < 2^50, K=2 aux, D={D} digits"
                                + (f", {B} ciphertext pairs per step sharing one relin key" if B > 1 else ""),
-                   "mode": args.mode, "batch": B, "per_op_ops_s": round(per_op_value, 2),
+                   "mode": args.mode, "batch": B, "keyswitch_method": args.method,
+                   "per_op_ops_s": round(per_op_value, 2), "per_op_ops_s_by_method": per_op_by,
+                   "batched_ops_s_by_method": batched_by,
                    "batched_vs_per_op": round(value / per_op_value, 3),
                    "bit_exact_vs_per_op": bit_exact,
                    "batched_with_rescale_ops_s": round(value_rescale, 2) if value_rescale else None},

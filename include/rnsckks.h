@@ -83,6 +83,9 @@ int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digi
 #define RCK_KS_DIGIT_NTT 1
 int rck_context_set_keyswitch_method(rck_context* ctx, int method);
 int rck_ksk_method(const rck_ksk* key, int* method);
+/* Exact-convolution keyswitch at `limbs` Q limbs: the largest digit group whose exact sums stay below
+ * half the auxiliary primes' product, and the number of groups the active digits take. */
+int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digits_per_group, uint32_t* groups);
 /* galois_element_for_rotation (src/rns/rotation.rs:14-26): 5^steps mod 2N, (5^-1)^|steps| if < 0 */
 uint64_t rck_galois_element_for_rotation(uint32_t poly_degree, int64_t steps);
 

@@ -147,6 +147,17 @@ int rck_context_set_keyswitch_method(rck_context* ctx, int method) {
     });
 }
 
+int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digits_per_group, uint32_t* groups) {
+    return guard([&] {
+        Context& c = C(ctx);
+        RCK_REQUIRE(limbs >= 1 && limbs <= c.L, "bad limb count");
+        RCK_REQUIRE(exact_keyswitch_available(c, c.K), "the context has no exact-convolution keyswitch");
+        const uint32_t D = (c.qbits[limbs] + 29) / 30, g = (uint32_t)exact_group_size(c, (int)limbs);
+        if (digits_per_group) *digits_per_group = g;
+        if (groups) *groups = (D + g - 1) / g;
+    });
+}
+
 int rck_ksk_method(const rck_ksk* key, int* method) {
     return guard([&] {
         RCK_REQUIRE(key != nullptr && method != nullptr, "null argument");

@@ -114,52 +114,81 @@ __global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__
     const int bg = r % nbg;
     r /= nbg;
     const int s = r & 1, g = r >> 1;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // wave-uniform quantities live in scalar registers: every load below is a scalar base plus the
+    // lane's offset, with no per-lane address arithmetic
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const int WJ = 4 / WB;
     const int wb = wave % WB, wj = wave / WB;
     const int b0 = (bg * WB + wb) * BT;
     const int c0 = (jg * WJ + wj) * JT;
     if (b0 >= B || c0 >= cols) return;
-    const long long x = (long long)xb * 64 + lane;
     const DevPrime P = primes[sbase + s];
     const double q = P.q, qinv = P.qinv;
     const int i0 = g * gsize, i1 = min(D, i0 + gsize);
+    const long long n2 = 2LL * n, x0 = (long long)xb * 64;
+    // rows past the batch or past the key columns read the last valid one (their products are dropped);
+    // every load is one of these wave-uniform offsets from a kernel argument plus the lane
+    long long dbase[BT], kbase[JT];
+#pragma unroll
+    for (int bt = 0; bt < BT; bt++) dbase[bt] = ((long long)min(b0 + bt, B - 1) * D * 2 + s) * n + x0;
+#pragma unroll
+    for (int jt = 0; jt < JT; jt++) kbase[jt] = (long long)min(c0 + jt, cols - 1) * n2 + s * n + x0;
+    const long long kstep = (long long)cols * n2;  // key digit i -> i + 1
     double a[BT][JT];
 #pragma unroll
     for (int bt = 0; bt < BT; bt++)
 #pragma unroll
         for (int jt = 0; jt < JT; jt++) a[bt][jt] = 0.0;
-    const long long n2 = 2LL * n;
-    int pend = 0;
-    for (int i = i0; i < i1; i++) {
-        double d[BT], k[JT];
+    auto load = [&](int t, double (&d)[BT], double (&k)[JT]) {
 #pragma unroll
-        for (int bt = 0; bt < BT; bt++)
-            d[bt] = b0 + bt < B ? dn[(((long long)(b0 + bt) * D + i) * 2 + s) * n + x] : 0.0;
+        for (int bt = 0; bt < BT; bt++) d[bt] = (dn + dbase[bt] + t * n2)[lane];
 #pragma unroll
-        for (int jt = 0; jt < JT; jt++)
-            k[jt] = c0 + jt < cols ? key[((long long)i * cols + c0 + jt) * n2 + (long long)s * n + x] : 0.0;
+        for (int jt = 0; jt < JT; jt++) k[jt] = (key + kbase[jt] + t * kstep)[lane];
+        // keeps the loads ahead of the multiplies that follow (the compiler would otherwise sink them to
+        // their first use); neither barrier reads a register, so neither adds a wait
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    auto mac = [&](const double (&d)[BT], const double (&k)[JT]) {
 #pragma unroll
         for (int bt = 0; bt < BT; bt++)
 #pragma unroll
             for (int jt = 0; jt < JT; jt++) a[bt][jt] += mulmod(d[bt], k[jt], q, qinv);
-        // reduced operands: each product is <= 0.57 s, so eight of them on a reduced accumulator stay
-        // below 5.1 s < 2^53 / s
-        if (++pend == 8) {
-            pend = 0;
+    };
+    // reduced operands: each product is <= 0.57 s, so eight of them on a reduced accumulator stay
+    // below 5.1 s < 2^53 / s
+    auto fold = [&]() {
 #pragma unroll
-            for (int bt = 0; bt < BT; bt++)
+        for (int bt = 0; bt < BT; bt++)
 #pragma unroll
-                for (int jt = 0; jt < JT; jt++) a[bt][jt] = reduce(a[bt][jt], q, qinv);
-        }
+            for (int jt = 0; jt < JT; jt++) a[bt][jt] = reduce(a[bt][jt], q, qinv);
+    };
+    // two register buffers, unrolled by two: the digit after the one being multiplied is always in
+    // flight (its index clamped to the group, so every pass issues the same loads and the only wait
+    // is on loads issued one digit earlier)
+    double dA[BT], kA[JT], dB[BT], kB[JT];
+    load(i0, dA, kA);
+    for (int t = i0; t < i1; t += 2) {
+        load(min(t + 1, i1 - 1), dB, kB);
+        mac(dA, kA);
+        load(min(t + 2, i1 - 1), dA, kA);
+        // an odd group's last pass has no second digit: its key operands become 0, and every product
+        // with 0 is exactly 0
+        const bool none = t + 1 == i1;
+#pragma unroll
+        for (int jt = 0; jt < JT; jt++) kB[jt] = none ? 0.0 : kB[jt];
+        mac(dB, kB);
+        if (((t - i0) & 7) == 6) fold();
     }
+    fold();
 #pragma unroll
     for (int bt = 0; bt < BT; bt++) {
         if (b0 + bt >= B) continue;
 #pragma unroll
         for (int jt = 0; jt < JT; jt++) {
             if (c0 + jt >= cols) continue;
-            acc[((((long long)g * B + b0 + bt) * cols + c0 + jt) * 2 + s) * n + x] = reduce(a[bt][jt], q, qinv);
+            acc[((((long long)g * B + b0 + bt) * cols + c0 + jt) * 2 + s) * n + x0 + lane] = a[bt][jt];
         }
     }
 }

@@ -42,6 +42,7 @@ EXPORTS = {
     "rck_num_decomp_digits": (ctypes.c_int, [vp, u32, u32p]),
     "rck_context_set_keyswitch_method": (ctypes.c_int, [vp, ctypes.c_int]),
     "rck_ksk_method": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
+    "rck_keyswitch_exact_groups": (ctypes.c_int, [vp, u32, u32p, u32p]),
     "rck_galois_element_for_rotation": (u64, [u32, i64]),
     "rck_ntt_forward": (ctypes.c_int, [vp, vp, u32, u32, vp]),
     "rck_ntt_inverse": (ctypes.c_int, [vp, vp, u32, u32, vp]),
@@ -459,6 +460,12 @@ class Context:
     def set_keyswitch_method(self, method):
         """Keyswitch method of the keys created from now on (KS_EXACT_CONVOLUTION or KS_DIGIT_NTT)."""
         _check(lib().rck_context_set_keyswitch_method(self.ptr, method))
+
+    def exact_group_size(self, limbs):
+        """(digits per group, groups) of the exact-convolution keyswitch at `limbs` Q limbs."""
+        g, k = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().rck_keyswitch_exact_groups(self.ptr, limbs, ctypes.byref(g), ctypes.byref(k)))
+        return g.value, k.value
 
     @staticmethod
     def ksk_method(ksk):
