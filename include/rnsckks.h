@@ -62,9 +62,9 @@ int rck_event_elapsed_ms(void* start, void* stop, float* ms);
  * Replaces RnsCkksContext::new(RnsCkksParams) (src/rns/context/construction.rs:9-30) with its
  * validation (src/rns/context/params.rs:48-73, src/rns/basis.rs:32-59, src/math/ntt.rs:17-60;
  * same error messages).  primes/roots: the Q chain (RnsPrime{modulus, primitive_root}),
- * aux_primes/aux_roots: the P chain (may be empty: Q-only keyswitching).  Beyond the crate's
- * checks, every modulus must lie below 2^50 + 2^46 (the device's FP64-exact modular products;
- * RCK_ERR_INVALID "moduli must be below 2^50 + 2^46 (FP64-exact arithmetic)"). */
+ * aux_primes/aux_roots: the P chain (may be empty: Q-only keyswitching).  Every odd prime below 2^64
+ * is accepted, as in the crate.  Moduli below 2^50 + 2^46 run on the FP64-exact kernels; a larger
+ * modulus selects the 64-bit integer arithmetic for the whole context (see rck_context_set_arithmetic). */
 int rck_context_create(uint32_t poly_degree, const uint64_t* primes, const uint64_t* roots, uint32_t n_primes,
                        const uint64_t* aux_primes, const uint64_t* aux_roots, uint32_t n_aux, uint32_t scale_bits,
                        rck_context** out);
@@ -86,6 +86,18 @@ int rck_ksk_method(const rck_ksk* key, int* method);
 /* Exact-convolution keyswitch at `limbs` Q limbs: the largest digit group whose exact sums stay below
  * half the auxiliary primes' product, and the number of groups the active digits take. */
 int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digits_per_group, uint32_t* groups);
+/* Arithmetic of the context's kernels; no counterpart in the crate, whose u128 products both reproduce
+ * bit for bit (src/math/ntt.rs:132-150).  RCK_ARITH_FP64: FP64-exact modular products (every modulus
+ * below 2^50 + 2^46; the default then).  RCK_ARITH_INT64: Montgomery products on 64-bit integers, any
+ * modulus below 2^64 (the default when a modulus is at or above 2^50 + 2^46; requesting it on any
+ * context is allowed).  Keys belong to the arithmetic they were created under.  The 64-bit integer
+ * arithmetic serves the NTT, polynomial, ciphertext, key and keyswitch entry points (rck_ntt_* through
+ * rck_rotate, rck_ksk_*, sampling, rck_keygen_public, rck_encrypt / rck_decrypt); the other entry
+ * points return RCK_ERR_INVALID on such a context. */
+#define RCK_ARITH_FP64 0
+#define RCK_ARITH_INT64 1
+int rck_context_set_arithmetic(rck_context* ctx, int arithmetic);
+int rck_context_arithmetic(const rck_context* ctx, int* arithmetic);
 /* galois_element_for_rotation (src/rns/rotation.rs:14-26): 5^steps mod 2N, (5^-1)^|steps| if < 0 */
 uint64_t rck_galois_element_for_rotation(uint32_t poly_degree, int64_t steps);
 

@@ -413,6 +413,7 @@ int rck_ps_plan(rck_context* ctx, uint32_t limbs, uint32_t scale_bits, const dou
     return aguard([&] {
         RCK_REQUIRE(ctx && ctx->c, "null context");
         const Context& c = *ctx->c;
+        require_fp64(c);
         const PsProgram p = ps_compile(c, (int)limbs, scale_bits, coeffs, (int)count, std::vector<bool>(c.L, true));
         if (out_limbs) *out_limbs = (uint32_t)p.reg_limbs[p.result];
         if (out_scale) *out_scale = p.reg_scale[p.result];
@@ -427,6 +428,7 @@ int rck_ps_eval(rck_context* ctx, const uint64_t* ct, uint32_t limbs, uint32_t s
     return aguard([&] {
         RCK_REQUIRE(ctx && ctx->c, "null context");
         Context& c = *ctx->c;
+        require_fp64(c);
         check_ptr(ct, "ciphertext");
         check_ptr(out, "out");
         std::vector<const KeyDev*> keys;

@@ -283,6 +283,7 @@ extern "C" {
 int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uint32_t batch, char* buf, size_t len) {
     return bguard([&] {
         RCK_REQUIRE(ctx && ctx->c && key, "null argument");
+        require_fp64(*ctx->c);
         RCK_REQUIRE(limbs >= 1 && limbs <= ctx->c->L && (int)key->level == (int)limbs - 1,
                     "keyswitch key level must match the limb count");
         RCK_REQUIRE(batch >= 1, "batch must be positive");
@@ -332,6 +333,7 @@ int rck_tune_table_import(const char* text, uint32_t* entries) {
 int rck_batch_create(rck_context* ctx, rck_batch** out) {
     return bguard([&] {
         RCK_REQUIRE(ctx && ctx->c && out, "null argument");
+        require_fp64(*ctx->c);
         rck_batch* b = new rck_batch();
         b->ctx = ctx;
         b->sc = ctx->c->scratch_clone();

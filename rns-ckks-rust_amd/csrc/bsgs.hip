@@ -171,6 +171,7 @@ int rck_diag_pack(rck_context* ctx, const double* matrix, uint32_t dim, uint32_t
     return bguard2([&] {
         RCK_REQUIRE(ctx && ctx->c && matrix && out, "null argument");
         Context& c = *ctx->c;
+        require_fp64(c);
         const uint32_t slots = c.n / 2;
         RCK_REQUIRE(dim > 0, "matrix cannot be empty");
         RCK_REQUIRE((dim & (dim - 1)) == 0, "matrix dimension must be a power of two");
@@ -284,6 +285,7 @@ int rck_bsgs_matvec(rck_context* ctx, const uint64_t* cts, uint32_t batch, uint3
     return bguard2([&] {
         RCK_REQUIRE(ctx && ctx->c && cts && d && out, "null argument");
         Context& c = *ctx->c;
+        require_fp64(c);
         RCK_REQUIRE(batch >= 1, "batch must be positive");
         RCK_REQUIRE(d->ctx == ctx, "packed diagonals belong to another context");
         RCK_REQUIRE(d->limbs == limbs, "packed diagonals must match the ciphertext level");

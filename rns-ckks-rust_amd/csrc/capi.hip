@@ -6,6 +6,7 @@
 
 #include "../../include/rnsckks.h"
 #include "capi_types.hpp"
+#include "wide.hpp"
 
 namespace rck {
 Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32_t L, const uint64_t* p,
@@ -147,6 +148,26 @@ int rck_context_set_keyswitch_method(rck_context* ctx, int method) {
     });
 }
 
+int rck_context_set_arithmetic(rck_context* ctx, int arithmetic) {
+    return guard([&] {
+        Context& c = C(ctx);
+        RCK_REQUIRE(arithmetic == RCK_ARITH_FP64 || arithmetic == RCK_ARITH_INT64, "unknown arithmetic");
+        if (arithmetic == RCK_ARITH_INT64) return context_make_wide(c);
+        for (uint32_t i = 0; i < c.L + c.K; i++)
+            RCK_REQUIRE(c.hp[i].q < (1ull << 50) + (1ull << 46),
+                        "moduli at or above 2^50 + 2^46 need the 64-bit integer arithmetic");
+        c.wide = false;
+    });
+}
+
+int rck_context_arithmetic(const rck_context* ctx, int* arithmetic) {
+    return guard([&] {
+        const Context& c = C(const_cast<rck_context*>(ctx));
+        check_ptr(arithmetic, "arithmetic");
+        *arithmetic = c.wide ? RCK_ARITH_INT64 : RCK_ARITH_FP64;
+    });
+}
+
 int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digits_per_group, uint32_t* groups) {
     return guard([&] {
         Context& c = C(ctx);
@@ -161,7 +182,7 @@ int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digit
 int rck_ksk_method(const rck_ksk* key, int* method) {
     return guard([&] {
         RCK_REQUIRE(key != nullptr && method != nullptr, "null argument");
-        *method = key->sdata ? RCK_KS_EXACT_CONVOLUTION : RCK_KS_DIGIT_NTT;
+        *method = key->sdata ? RCK_KS_EXACT_CONVOLUTION : RCK_KS_DIGIT_NTT;  // 64-bit integer keys: digit NTT
     });
 }
 
@@ -199,6 +220,7 @@ int rck_ntt_forward(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, 
         const int limbs = nq + np;
         const long long n = c.n;
         PrimeMap pm{(int)nq, (int)c.L, 0};
+        if (c.wide) return wide_ntt_cyclic(c, false, data, limbs, pm, S(stream));
         double* t = static_cast<double*>(c.workspace(8, (size_t)limbs * n * 8));
         launch_twist_in(c, {data, n, 0}, {t, n, 0}, limbs, pm, S(stream));
         launch_fwd(c, kInF64, t, n, 0, t, n, 0, limbs, 1, pm, S(stream));
@@ -214,6 +236,7 @@ int rck_ntt_inverse(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, 
         const int limbs = nq + np;
         const long long n = c.n;
         PrimeMap pm{(int)nq, (int)c.L, 0};
+        if (c.wide) return wide_ntt_cyclic(c, true, data, limbs, pm, S(stream));
         double* t = static_cast<double*>(c.workspace(8, (size_t)2 * limbs * n * 8));
         double* t2 = t + (size_t)limbs * n;
         launch_brv_in(c, {data, n, 0}, {t, n, 0}, limbs, pm, S(stream));
@@ -368,6 +391,22 @@ static void ksk_create(rck_context* ctx, const uint64_t* key, bool on_device, ui
     k->limbs = limbs;
     k->galois = galois;
     try {
+        if (c.wide) {
+            uint64_t* dev = nullptr;
+            if (!on_device) {
+                RCK_CHECK_HIP(hipMalloc(&dev, (size_t)digits * 2 * P * 8));
+                RCK_CHECK_HIP(hipMemcpy(dev, key, (size_t)digits * 2 * P * 8, hipMemcpyHostToDevice));
+            }
+            try {
+                wide_key_create(c, k, on_device ? key : dev);
+            } catch (...) {
+                if (dev) (void)hipFree(dev);
+                throw;
+            }
+            if (dev) RCK_CHECK_HIP(hipFree(dev));
+            *out = k;
+            return;
+        }
         RCK_CHECK_HIP(hipMalloc(&k->data, (size_t)digits * 2 * P * 8));
         uint64_t* stage = nullptr;
         if (!on_device) RCK_CHECK_HIP(hipMalloc(&stage, (size_t)2 * P * 8));
@@ -386,6 +425,7 @@ static void ksk_create(rck_context* ctx, const uint64_t* key, bool on_device, ui
     } catch (...) {
         if (k->data) (void)hipFree(k->data);
         if (k->sdata) (void)hipFree(k->sdata);
+        if (k->wdata) (void)hipFree(k->wdata);
         delete k;
         throw;
     }
@@ -408,6 +448,7 @@ int rck_ksk_destroy(rck_ksk* key) {
         RCK_CHECK_HIP(hipDeviceSynchronize());
         if (key->data) RCK_CHECK_HIP(hipFree(key->data));
         if (key->sdata) RCK_CHECK_HIP(hipFree(key->sdata));
+        if (key->wdata) RCK_CHECK_HIP(hipFree(key->wdata));
         delete key;
     });
 }

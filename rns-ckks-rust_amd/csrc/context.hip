@@ -7,6 +7,7 @@
 #include <mutex>
 
 #include "context.hpp"
+#include "wide.hpp"
 
 namespace rck {
 
@@ -262,8 +263,12 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
     // Two auxiliary NTT primes for the exact-convolution keyswitch (hybrid keys, N >= 2^12): the largest
     // primes below 2^50 + 2^46 with s = 1 mod 2N outside Q || P, each with its smallest quadratic
     // non-residue as generator (a non-residue g makes g^((s-1)/2N) a primitive 2N-th root of unity).
+    // the FP64-exact products need |a*w/q| < 2^51 with lazily reduced inputs (modarith.hpp); a larger
+    // modulus selects the 64-bit integer path for the whole context
+    bool wide = false;
+    for (uint64_t x : all) wide |= x >= (1ull << 50) + (1ull << 46);
     uint32_t S = 0;
-    if (K > 0 && n >= (1u << 12)) {
+    if (K > 0 && n >= (1u << 12) && !wide) {
         const uint64_t step = 2ull * n;
         uint64_t cand = (((1ull << 50) + (1ull << 46) - 2) / step) * step + 1;
         while (S < 2 && cand > step) {
@@ -290,9 +295,6 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
         const uint32_t T = L + K, TS = T + S;
         c->hp.resize(TS);
         for (uint32_t i = 0; i < TS; i++) {
-            // the FP64-exact products need |a*w/q| < 2^51 with lazily reduced inputs (modarith.hpp)
-            RCK_REQUIRE(all[i] < (1ull << 50) + (1ull << 46),
-                        "moduli must be below 2^50 + 2^46 (FP64-exact arithmetic)");
             std::string err = plan_check(n, all[i], roots[i], &c->hp[i]);
             RCK_REQUIRE(err.empty(), "invalid RNS prime " + std::to_string(all[i]) + ": " + err);
         }
@@ -375,7 +377,7 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
         if (K > 0) {
             auto pw = product_words(all.data() + L, K);
             std::vector<uint32_t> pwv(16, 0);  // [0..8) = P, [8..16) = floor(P / 2)
-            RCK_REQUIRE(pw.size() <= 7, "aux modulus too large");
+            RCK_REQUIRE(pw.size() <= 7 || wide, "aux modulus too large");
             std::memcpy(pwv.data(), pw.data(), pw.size() * 4);
             auto ph = half_words(pw);
             std::memcpy(pwv.data() + 8, ph.data(), ph.size() * 4);
@@ -396,11 +398,21 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
             c->d_pinvq = upload(pinvq);
             c->d_pmod_aux = upload(pmodaux);
         }
+        if (wide) context_make_wide(*c);
     } catch (...) {
         delete c;
         throw;
     }
     return c;
+}
+
+void context_make_wide(Context& c) {
+    if (!c.d_wp) {
+        std::vector<uint64_t> all;
+        for (uint32_t i = 0; i < c.L + c.K; i++) all.push_back(c.hp[i].q);
+        wide_tables_create(c, all, c.L + c.K);
+    }
+    c.wide = true;
 }
 
 void* Context::workspace(int slot, size_t bytes) {
@@ -464,6 +476,14 @@ Context* Context::scratch_clone() const {
     c->sp[1] = sp[1];
     c->sp_inv12 = sp_inv12;
     c->d_spmod = d_spmod;
+    c->wide = wide;
+    c->d_wp = d_wp;
+    c->d_wtables = d_wtables;
+    c->d_winvq = d_winvq;
+    c->d_wpmodq = d_wpmodq;
+    c->d_wpinvq = d_wpinvq;
+    c->d_wpaux = d_wpaux;
+    c->d_wphalf = d_wphalf;
     c->ks_method = ks_method;
     c->tune_group = tune_group;
     c->tune_dsplit = tune_dsplit;
@@ -478,7 +498,8 @@ Context::~Context() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (aux) (void)hipStreamDestroy(aux);
     if (!owns_tables) return;
-    void* ptrs[] = {d_primes, d_tables, d_invq, d_qwords, d_pwords, d_pmodq, d_pinvq, d_pmod_aux, d_spmod};
+    void* ptrs[] = {d_primes, d_tables,  d_invq,    d_qwords,  d_pwords,  d_pmodq, d_pinvq,
+                    d_pmod_aux, d_spmod, d_wp, d_wtables, d_winvq, d_wpmodq, d_wpinvq, d_wpaux, d_wphalf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }

@@ -64,6 +64,17 @@ struct Context {
     double sp_inv12 = 0;              // s1^{-1} mod s2 (centred)
     double* d_spmod = nullptr;        // [L+K][2]: s1 mod p_j, s1 s2 mod p_j (centred), per eval prime
     int ks_method = 0;                // keys created from now on: 0 exact convolution (hybrid, N >= 2^12), 1 digit NTT
+    // 64-bit integer arithmetic (wide.hpp): chosen when a modulus is at or above 2^50 + 2^46 (beyond
+    // the FP64-exact kernels), or on request.  Per-prime constants, twiddles and basis-conversion
+    // factors in Montgomery form (times R = 2^64), built on first use.
+    bool wide = false;
+    struct WidePrime* d_wp = nullptr;  // [L+K]
+    uint64_t* d_wtables = nullptr;     // [L+K][4][N]
+    uint64_t* d_winvq = nullptr;       // [L+K][L+K]: q_j^{-1} R mod q_i
+    uint64_t* d_wpmodq = nullptr;      // [L]: P mod q_i
+    uint64_t* d_wpinvq = nullptr;      // [L]: P^{-1} R mod q_i
+    uint64_t* d_wpaux = nullptr;       // [K][L]: p_k R mod q_i
+    uint64_t* d_wphalf = nullptr;      // [kMaxP]: floor(P / 2) as 64-bit words
     bool owns_tables = true;  // false for a scratch clone sharing another context's tables
     bool frozen = false;      // workspace may not grow (graph capture in progress)
     // keyswitch launch-shape tuning (0 = automatic): eval limbs per digit-MAC group, digit splits
@@ -84,6 +95,9 @@ struct Context {
     Context* scratch_clone() const;
     ~Context();
 };
+
+// switch a context to the 64-bit integer path (builds its tables on first use)
+void context_make_wide(Context& c);
 
 // errors
 void set_error(const std::string& msg);
@@ -106,6 +120,11 @@ struct Error {
     do {                                                    \
         if (!(cond)) throw ::rck::Error{-1, (message)};     \
     } while (0)
+
+// every FP64 kernel launcher starts with this: a 64-bit integer context must never reach one
+inline void require_fp64(const Context& c) {
+    RCK_REQUIRE(!c.wide, "this operation has no 64-bit integer path (moduli at or above 2^50 + 2^46)");
+}
 
 // Per-kernel timing (diagnostics): when enabled, every launch is bracketed by HIP events on its
 // own stream; prof_report() sums the elapsed times per kernel name.

@@ -17,6 +17,7 @@
 
 #include "../../include/rnsckks.h"
 #include "capi_types.hpp"
+#include "wide.hpp"
 
 struct rck_rng {
     uint64_t s[4];
@@ -115,7 +116,7 @@ __global__ void k_lift_small(const int64_t* __restrict__ v, uint64_t* __restrict
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
     if (i >= n) return;
-    const uint64_t q = (uint64_t)primes[pm(l)].q;
+    const uint64_t q = primes[pm(l)].qu;
     const int64_t x = v[i];
     uint64_t r = x >= 0 ? (uint64_t)x % q : q - ((uint64_t)(-x) % q);
     out[(size_t)l * n + i] = r == q ? 0 : r;
@@ -127,8 +128,8 @@ __global__ void k_small_from_limb0(const uint64_t* __restrict__ s0, uint64_t* __
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
     if (i >= n) return;
-    const uint64_t q0 = (uint64_t)primes[0].q;
-    const uint64_t q = (uint64_t)primes[pm(l)].q;
+    const uint64_t q0 = primes[0].qu;
+    const uint64_t q = primes[pm(l)].qu;
     const uint64_t x = s0[i];
     uint64_t r;
     if (x <= q0 / 2) {
@@ -157,6 +158,21 @@ __global__ void k_scale_add_sub(const uint64_t* __restrict__ a, const uint64_t* 
         if (r >= q) r -= q;
     }
     if (cc) r = r >= cc[k] ? r - cc[k] : r + q - cc[k];
+    out[k] = r;
+}
+
+// k_scale_add_sub for 64-bit integer contexts
+__global__ void k_w_scale_add_sub(const uint64_t* __restrict__ a, const uint64_t* __restrict__ f,
+                                  const uint64_t* __restrict__ b, const uint64_t* __restrict__ cc,
+                                  uint64_t* __restrict__ out, int n, PrimeMap pm, const WidePrime* __restrict__ wp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (i >= n) return;
+    const WidePrime P = wp[pm(l)];
+    const size_t k = (size_t)l * n + i;
+    uint64_t r = w_mul(w_mul(a[k], f[l], P.q, P.qn), P.r2, P.q, P.qn);
+    if (b) r = w_add(r, b[k], P.q);
+    if (cc) r = w_sub(r, cc[k], P.q);
     out[k] = r;
 }
 
@@ -200,8 +216,12 @@ void scale_add_sub(const Context& c, const uint64_t* a, const std::vector<uint64
                    const uint64_t* cc, uint64_t* out, int nq, int np) {
     DevBuf df(f.size() * 8);
     upload(df.p, f.data(), f.size() * 8);
-    hipLaunchKernelGGL(k_scale_add_sub, grid_of(c, nq + np), dim3(256), 0, nullptr, a, df.as<uint64_t>(), b, cc, out,
-                       (int)c.n, PrimeMap{nq, (int)c.L, 0}, c.d_primes);
+    if (c.wide)
+        hipLaunchKernelGGL(k_w_scale_add_sub, grid_of(c, nq + np), dim3(256), 0, nullptr, a, df.as<uint64_t>(), b, cc,
+                           out, (int)c.n, PrimeMap{nq, (int)c.L, 0}, c.d_wp);
+    else
+        hipLaunchKernelGGL(k_scale_add_sub, grid_of(c, nq + np), dim3(256), 0, nullptr, a, df.as<uint64_t>(), b, cc,
+                           out, (int)c.n, PrimeMap{nq, (int)c.L, 0}, c.d_primes);
     RCK_CHECK_HIP(hipGetLastError());
     RCK_CHECK_HIP(hipDeviceSynchronize());
 }
@@ -241,8 +261,19 @@ rck_ksk* keygen_switch(rck_context* ctx, rck_rng* r, double sigma, const uint64_
     k->aux = (uint32_t)K;
     k->limbs = (uint32_t)E;
     k->galois = galois;
+    // 64-bit integer contexts collect the key in coefficient form and convert it at the end
+    uint64_t* coef = nullptr;
+    auto store_digit = [&](uint32_t i, const uint64_t* a0) {
+        if (coef)
+            RCK_CHECK_HIP(hipMemcpy(coef + (size_t)i * 2 * P, a0, (size_t)2 * P * 8, hipMemcpyDeviceToDevice));
+        else
+            launch_fwd(c, kInU64, a0, n, P, k->data + (size_t)i * 2 * P, n, P, E, 2, pm, nullptr);
+    };
     try {
-        RCK_CHECK_HIP(hipMalloc(&k->data, (size_t)D * 2 * P * 8));
+        if (c.wide)
+            RCK_CHECK_HIP(hipMalloc(&coef, (size_t)D * 2 * P * 8));
+        else
+            RCK_CHECK_HIP(hipMalloc(&k->data, (size_t)D * 2 * P * 8));
         DevBuf stage(2 * P * 8);
         uint64_t* a0 = stage.as<uint64_t>();
         uint64_t* a1 = a0 + P;
@@ -268,7 +299,7 @@ rck_ksk* keygen_switch(rck_context* ctx, rck_rng* r, double sigma, const uint64_
                 lift_small(c, host_error(c, r, sigma), err.as<uint64_t>(), nq, K);
                 poly_mul(c, a1, P, tsec.as<uint64_t>(), 0, a1s.as<uint64_t>(), P, E, 1, pm, nullptr);
                 scale_add_sub(c, source, f, err.as<uint64_t>(), a1s.as<uint64_t>(), a0, nq, K);
-                launch_fwd(c, kInU64, a0, n, P, k->data + (size_t)i * 2 * P, n, P, E, 2, pm, nullptr);
+                store_digit(i, a0);
                 RCK_CHECK_HIP(hipDeviceSynchronize());
             }
         } else {
@@ -278,14 +309,22 @@ rck_ksk* keygen_switch(rck_context* ctx, rck_rng* r, double sigma, const uint64_
                 for (int l = 0; l < E; l++) f[l] = h_powmod(2 % c.hp[l].q, 30ull * i, c.hp[l].q);
                 scale_add_sub(c, source, f, nullptr, nullptr, pt.as<uint64_t>(), nq, 0);
                 encrypt_into(c, r, sigma, pt.as<uint64_t>(), pk, pk_limbs, nq, a0);
-                launch_fwd(c, kInU64, a0, n, P, k->data + (size_t)i * 2 * P, n, P, E, 2, pm, nullptr);
+                store_digit(i, a0);
                 RCK_CHECK_HIP(hipDeviceSynchronize());
             }
         }
-        if (c.ks_method == RCK_KS_EXACT_CONVOLUTION && exact_keyswitch_available(c, (uint32_t)K)) key_make_exact(c, k);
+        if (coef) {
+            wide_key_create(c, k, coef);
+            RCK_CHECK_HIP(hipFree(coef));
+            coef = nullptr;
+        } else if (c.ks_method == RCK_KS_EXACT_CONVOLUTION && exact_keyswitch_available(c, (uint32_t)K)) {
+            key_make_exact(c, k);
+        }
     } catch (...) {
+        if (coef) (void)hipFree(coef);
         if (k->data) (void)hipFree(k->data);
         if (k->sdata) (void)hipFree(k->sdata);
+        if (k->wdata) (void)hipFree(k->wdata);
         delete k;
         throw;
     }
@@ -471,8 +510,11 @@ int rck_keygen_rotation(rck_context* ctx, rck_rng* rng, double sigma, const uint
 int rck_ksk_export(rck_context* ctx, const rck_ksk* key, uint64_t* out) {
     return kguard([&] {
         Context& c = ctx_of(ctx);
-        RCK_REQUIRE(key != nullptr && (key->data != nullptr || key->sdata != nullptr), "null key");
+        RCK_REQUIRE(key != nullptr && (key->data != nullptr || key->sdata != nullptr || key->wdata != nullptr),
+                    "null key");
         check_ptr(out, "out");
+        if (key->wdata) return wide_key_export(c, key, out);
+        require_fp64(c);
         const long long n = c.n, P = (long long)key->limbs * n;
         const PrimeMap pm{(int)key->level + 1, (int)c.L, 0};
         DevBuf t(2 * P * 8), u(2 * P * 8);

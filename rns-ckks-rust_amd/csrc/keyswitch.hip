@@ -190,6 +190,7 @@ static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, in
 
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
                       long long ps, unsigned long long ginv) {
+    require_fp64(c);
     RCK_REQUIRE(nq >= 1 && nq <= (int)c.L, "bad limb count for decomposition");
     RCK_REQUIRE(levels >= 1, "decomposition needs at least one digit");
     // 30-bit words holding Q_nq (and the centred value in two's complement)
@@ -964,6 +965,7 @@ __global__ void k_mac_small(const double* __restrict__ dn, int D, const double* 
 void launch_digit_mac(const Context& c, const int32_t* digits, int B, int D, const double* key, int nq, int naux,
                       double* part, int dsplit, int group, int rows, int prefetch, int width, hipStream_t s,
                       const MacFold* inv) {
+    require_fp64(c);
     const int E = nq + naux;
     const long long n = c.n, EN = (long long)E * n;
     PrimeMap pm{nq, (int)c.L, 0};
@@ -1217,6 +1219,7 @@ __global__ __launch_bounds__(256) void k_inv_cols_md(const double* __restrict__ 
 void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B, const uint64_t* accP,
                              const uint64_t* add0, const uint64_t* add1, long long add_ps,
                              const std::vector<uint64_t*>& outs, hipStream_t s, unsigned long long ginv) {
+    require_fp64(c);
     RCK_REQUIRE(c.K > 0 && c.logn >= 12, "fused mod-down needs an aux basis and the two-pass NTT");
     const long long n = c.n, EN = (long long)(nq + c.K) * n, P = (long long)nq * n;
     int n1, n2;
@@ -1268,6 +1271,7 @@ void launch_inv_cols_moddown(const Context& c, const double* acc, int nq, int B,
 
 void launch_mod_down(const Context& c, const uint64_t* acc, int nq, const uint64_t* add0, const uint64_t* add1,
                      uint64_t* out, long long out_ps, hipStream_t s) {
+    require_fp64(c);
     RCK_REQUIRE(c.K > 0, "mod-down requires an auxiliary basis");
     ProfScope ps(s, "mod_down");
     k_mod_down<<<dim3((c.n + 255) / 256, 2), 256, 0, s>>>(acc, c.n, nq, c.K, c.L, c.L + c.K, add0, add1, out, out_ps,

@@ -355,6 +355,7 @@ bool exact_keyswitch_available(const Context& c, uint32_t aux) { return c.S == 2
 // Converts a key from its NTT form mod the eval primes (data) to the exact-convolution form (sdata);
 // data is released.
 void key_make_exact(Context& c, KeyDev* k) {
+    require_fp64(c);
     RCK_REQUIRE(k && k->data, "null key");
     RCK_REQUIRE(exact_keyswitch_available(c, k->aux), "exact-convolution keys need hybrid keys at N >= 2^12");
     const long long n = c.n;
@@ -421,6 +422,7 @@ void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* o
 void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
                      hipStream_t s, unsigned long long ginv, int bt, int jt) {
+    require_fp64(c);
     const long long n = c.n;
     const int E = limbs + (int)key->aux;
     const int gsize = exact_group_size(c, limbs);

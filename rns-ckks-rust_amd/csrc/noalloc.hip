@@ -68,6 +68,7 @@ int rck_poly_multiply_ntt_noalloc(rck_context* ctx, const uint64_t* a, const uin
     return nguard([&] {
         RCK_REQUIRE(ctx && ctx->c, "null context");
         Context& c = *ctx->c;
+        require_fp64(c);
         RCK_REQUIRE(nq >= 1 && nq <= c.L, "limb count must be in 1..=" + std::to_string(c.L) + ", got " +
                                               std::to_string(nq));
         RCK_REQUIRE(np == 0 || np == c.K, "aux limb count must be 0 or the aux basis size");

@@ -397,6 +397,7 @@ static void mult_rows_t(const Context& c, const double* xl, const double* xr, do
 
 void launch_mult_rows(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,
                       hipStream_t s, bool fold) {
+    require_fp64(c);
     check_dims(c, limbs, B);
     RCK_REQUIRE(!small_ring(c), "fused tensor rows need N >= 2^12");
     int n1, n2;
@@ -412,6 +413,7 @@ void launch_mult_rows(const Context& c, const double* xl, const double* xr, doub
 void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
                      long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s,
                      const void* in2, int zsplit) {
+    require_fp64(c);
     check_dims(c, limbs, polys);
     RCK_REQUIRE(in2 == nullptr || (zsplit > 0 && zsplit < polys), "bad second operand split");
     RCK_REQUIRE(!small_ring(c), "two-pass NTT needs N >= 2^12");
@@ -429,6 +431,7 @@ void launch_fwd_cols(const Context& c, int in_kind, const void* in, long long in
 
 void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
                      hipStream_t s) {
+    require_fp64(c);
     check_dims(c, limbs, polys);
     Batch<double> b{data, ls, ps};
     int n1, n2;
@@ -444,6 +447,7 @@ void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps,
 void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
                      hipStream_t s, int nsum, long long sstride, const double* addn, long long addn_bs,
                      long long addn_cs, int nq) {
+    require_fp64(c);
     check_dims(c, limbs, polys);
     RCK_REQUIRE(nsum >= 1 && nsum <= 8, "at most 8 partial sums");
     RCK_REQUIRE(addn == nullptr || (polys % 2 == 0 && nq <= limbs && nq <= (int)c.L), "bad folded addend");
@@ -460,6 +464,7 @@ void launch_inv_rows(const Context& c, double* data, long long ls, long long ps,
 
 void launch_inv_cols(const Context& c, const double* in, long long in_ls, long long in_ps, int out_kind, void* out,
                      long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    require_fp64(c);
     check_dims(c, limbs, polys);
     Batch<const double> bi{in, in_ls, in_ps};
     Batch<void> bo{out, out_ls, out_ps};
@@ -475,6 +480,7 @@ void launch_inv_cols(const Context& c, const double* in, long long in_ls, long l
 
 void launch_fwd(const Context& c, int in_kind, const void* in, long long in_ls, long long in_ps, double* out,
                 long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    require_fp64(c);
     check_dims(c, limbs, polys);
     if (small_ring(c)) {
         Batch<const void> bi{in, in_ls, in_ps};
@@ -497,6 +503,7 @@ void launch_fwd(const Context& c, int in_kind, const void* in, long long in_ls, 
 void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, int out_kind, void* out,
                 long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, int nsum,
                 long long sstride) {
+    require_fp64(c);
     check_dims(c, limbs, polys);
     if (small_ring(c)) {
         RCK_REQUIRE(nsum == 1, "partial sums need the two-pass NTT");

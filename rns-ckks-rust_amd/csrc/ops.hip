@@ -3,6 +3,7 @@
 // twist / bit-reversal permutations.  All are HBM-bound streaming kernels: one thread per
 // coefficient, grid.x over coefficients, grid.y over limbs, grid.z over polynomials.
 #include "ops.hpp"
+#include "wide.hpp"
 
 namespace rck {
 
@@ -157,6 +158,7 @@ static dim3 grid_for(int n, int limbs, int polys) { return dim3((n + 255) / 256,
 
 void launch_addsub(const Context& c, Batch<const uint64_t> a, Batch<const uint64_t> b, Batch<uint64_t> out, int limbs,
                    int polys, int sub, PrimeMap pm, hipStream_t s) {
+    if (c.wide) return wide_addsub(c, a, b, out, limbs, polys, sub, pm, s);
     ProfScope ps(s, "addsub");
     k_addsub<<<grid_for(c.n, limbs, polys), 256, 0, s>>>(a, b, out, c.n, sub, pm, c.d_primes);
     CHK();
@@ -173,6 +175,7 @@ uint64_t galois_inverse(const Context& c, uint64_t galois) {
 void launch_galois(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs, int polys,
                    uint64_t galois, PrimeMap pm, hipStream_t s) {
     const uint64_t ginv = galois_inverse(c, galois);
+    if (c.wide) return wide_galois(c, in, out, limbs, polys, ginv, pm, s);
     ProfScope ps(s, "galois");
     k_galois<<<grid_for(c.n, limbs, polys), 256, 0, s>>>(in, out, c.n, ginv, pm, c.d_primes);
     CHK();
@@ -181,6 +184,7 @@ void launch_galois(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> o
 void launch_rescale(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> out, int limbs_in, int polys,
                     hipStream_t s) {
     RCK_REQUIRE(limbs_in > 1, "rescale requires at least two levels");
+    if (c.wide) return wide_rescale(c, in, out, limbs_in, polys, s);
     ProfScope ps(s, "rescale");
     k_rescale<<<grid_for(c.n, limbs_in - 1, polys), 256, 0, s>>>(in, out, c.n, limbs_in - 1, c.L + c.K, c.d_primes,
                                                                    c.d_invq);
@@ -189,6 +193,7 @@ void launch_rescale(const Context& c, Batch<const uint64_t> in, Batch<uint64_t> 
 
 void launch_mul_add_u64(const Context& c, Batch<const uint64_t> a, Batch<const uint64_t> b, Batch<const uint64_t> cc,
                         bool has_c, Batch<uint64_t> out, int limbs, int polys, PrimeMap pm, hipStream_t s) {
+    if (c.wide) return wide_mul_add(c, a, b, cc, has_c, out, limbs, polys, pm, s);
     ProfScope ps(s, "mul_add_u64");
     k_mul_add_u64<<<grid_for(c.n, limbs, polys), 256, 0, s>>>(a, b, cc, has_c ? 1 : 0, out, c.n, pm, c.d_primes);
     CHK();
@@ -196,12 +201,14 @@ void launch_mul_add_u64(const Context& c, Batch<const uint64_t> a, Batch<const u
 
 void launch_mul_f64(const Context& c, Batch<const double> a, Batch<const double> b, Batch<double> out, int limbs,
                     int polys, PrimeMap pm, hipStream_t s) {
+    require_fp64(c);
     ProfScope ps(s, "mul_f64");
     k_mul_f64<<<grid_for(c.n, limbs, polys), 256, 0, s>>>(a, b, out, c.n, pm, c.d_primes);
     CHK();
 }
 
 void launch_tensor(const Context& c, const double* x, double* d, int limbs, PrimeMap pm, hipStream_t s) {
+    require_fp64(c);
     ProfScope ps(s, "tensor");
     k_tensor<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(x, d, (long long)limbs * c.n, c.n, pm, c.d_primes);
     CHK();
@@ -209,6 +216,7 @@ void launch_tensor(const Context& c, const double* x, double* d, int limbs, Prim
 
 void launch_twist_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
                      hipStream_t s) {
+    require_fp64(c);
     ProfScope ps(s, "twist_in");
     k_twist_in<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, pm, c.d_primes);
     CHK();
@@ -216,6 +224,7 @@ void launch_twist_in(const Context& c, Batch<const uint64_t> in, Batch<double> o
 
 void launch_brv_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
                     hipStream_t s) {
+    require_fp64(c);
     ProfScope ps(s, "brv_out");
     k_brv_out<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, c.logn, pm, c.d_primes);
     CHK();
@@ -223,6 +232,7 @@ void launch_brv_out(const Context& c, Batch<const double> in, Batch<uint64_t> ou
 
 void launch_brv_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
                    hipStream_t s) {
+    require_fp64(c);
     ProfScope ps(s, "brv_in");
     k_brv_in<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, c.logn, pm, c.d_primes);
     CHK();
@@ -230,6 +240,7 @@ void launch_brv_in(const Context& c, Batch<const uint64_t> in, Batch<double> out
 
 void launch_twist_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
                       hipStream_t s) {
+    require_fp64(c);
     ProfScope ps(s, "twist_out");
     k_twist_out<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, pm, c.d_primes);
     CHK();

@@ -9,6 +9,7 @@
 
 #include "../../include/rnsckks.h"
 #include "pipeline.hpp"
+#include "wide.hpp"
 
 namespace rck {
 
@@ -22,6 +23,10 @@ size_t poly_mul_scratch_bytes(const Context& c, long long b_ps, int limbs, int p
 
 void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
               long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, double* scratch) {
+    if (c.wide) {
+        RCK_REQUIRE(scratch == nullptr, "the caller-provided scratch serves the FP64 arithmetic only");
+        return wide_poly_mul(c, a, a_ps, b, b_ps, out, out_ps, limbs, polys, pm, s);
+    }
     const long long n = c.n, P = (long long)limbs * n;
     const int bpolys = b_ps ? polys : 1;
     double* t = scratch ? scratch
@@ -40,6 +45,11 @@ void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, 
 double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
                    uint64_t* out, long long out_ps, hipStream_t s, bool fold_ntt) {
     const long long n = c.n, P = (long long)limbs * n;
+    if (c.wide) {
+        RCK_REQUIRE(!fold_ntt, "the folded product serves the FP64 arithmetic only");
+        for (int b = 0; b < B; b++) wide_mult(c, lhs + b * in_ps, rhs + b * in_ps, limbs, out + b * out_ps, s);
+        return nullptr;
+    }
     PrimeMap pm{limbs, (int)c.L, 0};
     double* x = static_cast<double*>(c.workspace(0, (size_t)B * 4 * P * 8));
     double* d = static_cast<double*>(c.workspace(1, (size_t)B * 3 * P * 8));
@@ -103,7 +113,7 @@ void rotate_batch(Context& c, const std::vector<const uint64_t*>& as, int limbs,
     const int B = (int)as.size();
     const long long n = c.n, P = (long long)limbs * n;
     const long long st = B > 1 ? common_stride(as) : 2 * P;
-    if (c.logn >= 12 && key->aux > 0 && st > 0) {
+    if (c.logn >= 12 && key->aux > 0 && st > 0 && !c.wide) {
         // rotate = (sigma(c0), 0) + keyswitch(sigma(c1)): the decomposition reads c1 and the mod-down
         // epilogue reads c0 through the automorphism, so no rotated copy is written
         const uint64_t ginv = galois_inverse(c, key->galois);
@@ -122,7 +132,7 @@ void rotate_batch(Context& c, const std::vector<const uint64_t*>& as, int limbs,
 }
 
 bool mult_relin_folds(const Context& c, const KeyDev* key) {
-    return c.logn >= 12 && key && key->aux > 0 && key->data != nullptr;
+    return c.logn >= 12 && key && key->aux > 0 && key->data != nullptr && !c.wide;
 }
 
 void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
@@ -341,6 +351,15 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
         levels = active;  // keyswitching.rs:170
     }
     const int naux = key->aux;
+    if (c.wide) {
+        RCK_REQUIRE(addn == nullptr && ginv == 0, "folded addends and the fused automorphism serve the FP64 arithmetic only");
+        for (int b = 0; b < B; b++)
+            wide_keyswitch(c, poly + b * in_ps, limbs, key, add0 ? add0 + b * add_ps : nullptr,
+                           add1 ? add1 + b * add_ps : nullptr, outs[b], s);
+        return;
+    }
+    RCK_REQUIRE(key->data != nullptr || key->sdata != nullptr,
+                "keyswitch key was created for another arithmetic of this context");
     if (key->sdata) {
         RCK_REQUIRE(addn == nullptr, "folded addends need a digit-NTT key");
         const KsShape sh = shape.prefetch == kExactShape ? shape : ks_auto_shape(c, limbs, limbs + naux, levels, B, true);

@@ -11,6 +11,8 @@ struct KeyDev {
     // exact-convolution form instead (hybrid keys, N >= 2^12): sdata[digit][comp][limb][s][N], the NTT
     // forms under the two auxiliary primes s of the key's centred limbs (data is then null)
     double* sdata = nullptr;
+    // 64-bit integer contexts instead: wdata[digit][comp][limb][N], NTT form times R = 2^64 (wide.hpp)
+    uint64_t* wdata = nullptr;
     uint32_t digits = 0, level = 0, aux = 0, limbs = 0;
     uint64_t galois = 0;
 };

@@ -474,6 +474,7 @@ int rck_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t l
 int rck_server_register_ring(rck_server* s, rck_context* ctx, const rck_ksk* const* relin_keys, uint32_t n_keys) {
     return sguard([&] {
         RCK_REQUIRE(s && ctx && ctx->c, "null argument");
+        require_fp64(*ctx->c);
         std::lock_guard<std::mutex> lk(s->mu);
         RCK_REQUIRE(s->requests.empty(), "rings must be registered before the first request");
         Ring& g = s->rings[ctx];

@@ -19,6 +19,8 @@ u32p, u64p = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)
 
 KS_EXACT_CONVOLUTION = 0  # keyswitch methods (rck_context_set_keyswitch_method)
 KS_DIGIT_NTT = 1
+ARITH_FP64 = 0  # kernel arithmetic (rck_context_set_arithmetic)
+ARITH_INT64 = 1
 
 EXPORTS = {
     "rck_last_error": (ctypes.c_char_p, []),
@@ -43,6 +45,8 @@ EXPORTS = {
     "rck_context_set_keyswitch_method": (ctypes.c_int, [vp, ctypes.c_int]),
     "rck_ksk_method": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
     "rck_keyswitch_exact_groups": (ctypes.c_int, [vp, u32, u32p, u32p]),
+    "rck_context_set_arithmetic": (ctypes.c_int, [vp, ctypes.c_int]),
+    "rck_context_arithmetic": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
     "rck_galois_element_for_rotation": (u64, [u32, i64]),
     "rck_ntt_forward": (ctypes.c_int, [vp, vp, u32, u32, vp]),
     "rck_ntt_inverse": (ctypes.c_int, [vp, vp, u32, u32, vp]),
@@ -460,6 +464,16 @@ class Context:
     def set_keyswitch_method(self, method):
         """Keyswitch method of the keys created from now on (KS_EXACT_CONVOLUTION or KS_DIGIT_NTT)."""
         _check(lib().rck_context_set_keyswitch_method(self.ptr, method))
+
+    def set_arithmetic(self, arithmetic):
+        """ARITH_FP64 (moduli below 2^50 + 2^46 only) or ARITH_INT64 (any modulus below 2^64)."""
+        _check(lib().rck_context_set_arithmetic(self.ptr, arithmetic))
+
+    @property
+    def arithmetic(self):
+        m = ctypes.c_int()
+        _check(lib().rck_context_arithmetic(self.ptr, ctypes.byref(m)))
+        return m.value
 
     def exact_group_size(self, limbs):
         """(digits per group, groups) of the exact-convolution keyswitch at `limbs` Q limbs."""

@@ -507,16 +507,20 @@ def test_largest_allowed_primes_long_digit_runs(logn):
 
 
 def test_context_rejects_invalid_chains():
-    """RnsContext / NttPlan::new validation (math/ntt.rs:17-60) with the crate's error texts, and the
-    modulus bound of the FP64-exact arithmetic (2^50 + 2^46)."""
+    """RnsContext / NttPlan::new validation (math/ntt.rs:17-60) with the crate's error texts, at every
+    modulus size; the arithmetic follows the largest modulus (FP64-exact below 2^50 + 2^46)."""
     with pytest.raises(G.RckError, match="invalid RNS prime 41"):
         G.Context(poly_degree=8, primes=[41], roots=[6])
-    big = (1 << 50) + (1 << 46) + 17
-    with pytest.raises(G.RckError, match="moduli must be below 2"):
-        G.Context(poly_degree=8, primes=[big], roots=[3])
+    # 2^62 - 2^16 + 1 = 4611686018427322369 is prime with q = 1 mod 16; the root 1 generates no
+    # primitive 8th root of unity
+    with pytest.raises(G.RckError, match="invalid RNS prime 4611686018427322369"):
+        G.Context(poly_degree=8, primes=[4611686018427322369], roots=[1])
     q, g = chain("above_2^50", 1 << 16, 1)
-    ctx = G.Context(poly_degree=8, primes=q, roots=g)  # primes just above 2^50 are accepted
-    assert ctx is not None
+    ctx = G.Context(poly_degree=8, primes=q, roots=g)  # primes just above 2^50: FP64-exact
+    assert ctx.arithmetic == G.ARITH_FP64
+    big = PR.primes_below(64, 8, 1)[0]
+    ctx = G.Context(poly_degree=8, primes=[big], roots=[PR.smallest_primitive_root(big)])
+    assert ctx.arithmetic == G.ARITH_INT64
 
 
 def test_limb_counts_out_of_range_rejected():
