@@ -97,36 +97,13 @@ __global__ void k_key_from_special(const uint64_t* __restrict__ in, uint64_t* __
 // ciphertext group, prime and digit group, and each XCD takes a contiguous range of them (workgroup
 // w runs on XCD w % 8), so consecutive items on an XCD share their digit tiles in its L2.
 // ---------------------------------------------------------------------------------------
+// One wave's tile: ciphertexts b0 .. b0+BT-1 by key columns c0 .. c0+JT-1 over digits i0 .. i1-1 of
+// auxiliary prime s, for the 64 coefficients from x0.
 template <int BT, int JT>
-__global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__ dn, const double* __restrict__ key,
-                                                      double* __restrict__ acc, int n, int B, int D, int gsize,
-                                                      int cols, int WB, int njg, int nxb, int nbg, int items,
-                                                      int sbase, const DevPrime* __restrict__ primes) {
-    const int w = blockIdx.x;
-    const int per = gridDim.x >> 3;
-    const int item = (w & 7) * per + (w >> 3);
-    if (item >= items) return;
-    int r = item;
-    const int jg = r % njg;
-    r /= njg;
-    const int xb = r % nxb;
-    r /= nxb;
-    const int bg = r % nbg;
-    r /= nbg;
-    const int s = r & 1, g = r >> 1;
-    // wave-uniform quantities live in scalar registers: every load below is a scalar base plus the
-    // lane's offset, with no per-lane address arithmetic
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int WJ = 4 / WB;
-    const int wb = wave % WB, wj = wave / WB;
-    const int b0 = (bg * WB + wb) * BT;
-    const int c0 = (jg * WJ + wj) * JT;
-    if (b0 >= B || c0 >= cols) return;
-    const DevPrime P = primes[sbase + s];
-    const double q = P.q, qinv = P.qinv;
-    const int i0 = g * gsize, i1 = min(D, i0 + gsize);
-    const long long n2 = 2LL * n, x0 = (long long)xb * 64;
+__device__ __forceinline__ void mac_tile(const double* __restrict__ dn, const double* __restrict__ key,
+                                         double* __restrict__ acc, int n, int B, int D, int cols, int b0, int c0,
+                                         int s, int g, int i0, int i1, long long x0, int lane, double q, double qinv) {
+    const long long n2 = 2LL * n;
     // rows past the batch or past the key columns read the last valid one (their products are dropped);
     // every load is one of these wave-uniform offsets from a kernel argument plus the lane
     long long dbase[BT], kbase[JT];
@@ -191,6 +168,51 @@ __global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__
             acc[((((long long)g * B + b0 + bt) * cols + c0 + jt) * 2 + s) * n + x0 + lane] = a[bt][jt];
         }
     }
+}
+
+// The last column group of a row may hold fewer columns than JT: it runs the narrowest tile that still
+// covers them, so no products are computed for columns past the key's.
+template <int BT, int JT>
+__device__ __forceinline__ void mac_tile_fit(const double* __restrict__ dn, const double* __restrict__ key,
+                                             double* __restrict__ acc, int n, int B, int D, int cols, int b0, int c0,
+                                             int s, int g, int i0, int i1, long long x0, int lane, double q,
+                                             double qinv) {
+    if constexpr (JT > 4) {
+        if (cols - c0 <= JT / 2)
+            return mac_tile_fit<BT, JT / 2>(dn, key, acc, n, B, D, cols, b0, c0, s, g, i0, i1, x0, lane, q, qinv);
+    }
+    mac_tile<BT, JT>(dn, key, acc, n, B, D, cols, b0, c0, s, g, i0, i1, x0, lane, q, qinv);
+}
+
+template <int BT, int JT>
+__global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__ dn, const double* __restrict__ key,
+                                                      double* __restrict__ acc, int n, int B, int D, int gsize,
+                                                      int cols, int WB, int njg, int nxb, int nbg, int items,
+                                                      int sbase, const DevPrime* __restrict__ primes) {
+    const int w = blockIdx.x;
+    const int per = gridDim.x >> 3;
+    const int item = (w & 7) * per + (w >> 3);
+    if (item >= items) return;
+    int r = item;
+    const int jg = r % njg;
+    r /= njg;
+    const int xb = r % nxb;
+    r /= nxb;
+    const int bg = r % nbg;
+    r /= nbg;
+    const int s = r & 1, g = r >> 1;
+    // wave-uniform quantities live in scalar registers: every load below is a scalar base plus the
+    // lane's offset, with no per-lane address arithmetic
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int WJ = 4 / WB;
+    const int wb = wave % WB, wj = wave / WB;
+    const int b0 = (bg * WB + wb) * BT;
+    const int c0 = (jg * WJ + wj) * JT;
+    if (b0 >= B || c0 >= cols) return;
+    const DevPrime P = primes[sbase + s];
+    const int i0 = g * gsize, i1 = min(D, i0 + gsize);
+    mac_tile_fit<BT, JT>(dn, key, acc, n, B, D, cols, b0, c0, s, g, i0, i1, (long long)xb * 64, lane, P.q, P.qinv);
 }
 
 template <int BT, int JT>
@@ -432,12 +454,12 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     // workspace slots 30-32 (slots 20-26 hold the BSGS buffers of a transform whose rotations come here)
     double* dn = static_cast<double*>(c.workspace(30, (size_t)B * levels * 2 * n * 8));
     double* acc = static_cast<double*>(c.workspace(31, (size_t)G * B * 2 * E * 2 * n * 8));
-    uint64_t* accc = static_cast<uint64_t*>(c.workspace(32, (size_t)G * B * 2 * E * 2 * n * 8));
     launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps, ginv);
     const PrimeMap ps{0, (int)(c.L + c.K), 0};
     // every digit under s1 and s2: limb stride 0 on the input (the same digit row for both)
     launch_fwd(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
     launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, gs, G, E, s);
+    uint64_t* accc = static_cast<uint64_t*>(c.workspace(32, (size_t)G * B * 2 * E * 2 * n * 8));
     launch_inv(c, acc, n, 2 * n, kOutU64, accc, n, 2 * n, 2, G * B * 2 * E, ps, s);
     launch_crt_moddown(c, accc, limbs, B, G, add0, add1, add_ps, ginv, outs, s);
 }
