@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed batches (batched) / ops (per-op)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mode", default="batched", choices=["batched", "per-op", "serve", "mixed", "bsgs"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "per-op", "serve", "mixed", "bsgs", "wide"])
     ap.add_argument("--cts", type=int, default=256, help="mixed: ciphertexts in the batch")
     ap.add_argument("--dim", type=int, default=4096, help="bsgs: matrix dimension")
     ap.add_argument("--vectors", type=int, default=32, help="bsgs: encrypted vectors per batch")
@@ -256,6 +256,9 @@ def main():
         return
     if args.mode == "bsgs":
         bsgs_main(args, G, PR)
+        return
+    if args.mode == "wide":
+        wide_main(args, G, PR)
         return
 
     p = PR.config_per_op_gpu()
@@ -772,6 +775,60 @@ def mixed_main(args, G, PR):
                    "batched_vs_per_op": round(ms_op / ms, 3), "bit_exact_vs_per_op": bool(same)},
         "roofline": None, "cpu_baseline": None,
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def wide_main(args, G, PR):
+    """The 64-bit integer arithmetic at N=2^16: 20 primes below 2^60 (q = 1 mod 2^17) and 2 aux primes,
+    so D = 40 digits as at the headline workload; per-op mult+relinearize, rotation, and the crate's
+    NTT round trip of every limb, each against the same op on the FP64-exact kernels at the headline
+    workload (24 primes below 2^50)."""
+    n, L, K = 1 << 16, 20, 2
+    qs = PR.primes_below(60, n, L + K)
+    p = PR.RnsCkksParams.with_primes(n, qs[:L], qs[L:], 40)
+    ctx = G.Context(p)
+    assert ctx.arithmetic == G.ARITH_INT64
+    D = ctx.num_decomp_digits(L)
+    rs = np.random.default_rng(0)
+    P = L * n
+    lhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2))
+    rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2))
+    key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
+    ksk = ctx.ksk_create(key, L - 1, K)
+    gel = G.galois_element_for_rotation(n, 1)
+    rk = ctx.ksk_create(key, L - 1, K, gel)
+    del key
+    quad, out = G.DeviceBuffer((3, L, n)), G.DeviceBuffer((2, L, n))
+    stream = G.Stream()
+
+    def mr():
+        ctx.mult(lhs, rhs, L, 40, 40, quad)
+        ctx.relinearize(quad, L, 80, ksk, out)
+
+    for _ in range(args.warmup):
+        mr()
+    ms_mr = time_it(G, mr, args.steps) / args.steps
+    ms_rot = time_it(G, lambda: ctx.rotate(lhs, L, 40, rk, out), args.steps) / args.steps
+    ntt = ntt_entry(G, ctx, p, n, L, K)
+    G.profile_enable(True)
+    mr()
+    prof = G.profile_report()
+    G.profile_enable(False)
+    value = 1000.0 / ms_mr
+    log(f"[bench] 64-bit integer path, N=2^16, {L} primes < 2^60 + {K} aux, D={D}: mult+relin {ms_mr:.3f} ms "
+        f"({value:.1f} ops/s), rotate {ms_rot:.3f} ms")
+    for k, (cnt, tot) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        log(f"[bench]   {k:16s} launches={cnt:5d} total={tot:9.3f} ms")
+    line = {
+        "metric": "per-op ct x ct mult+relin ops/s with 60-bit moduli (64-bit integer arithmetic), N=2^16",
+        "value": round(value, 2), "unit": "ops/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_mr, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 residues (64-bit integer Montgomery arithmetic)", "data": "synthetic (seeded uniform residues)",
+        "config": {"workload": f"per-op mult+relinearize, N=2^16, L={L} primes < 2^60, K={K} aux, D={D} digits",
+                   "rotate_ms": round(ms_rot, 4)},
+        "roofline": None, "cpu_baseline": None, "ntt": ntt,
+        "kernels_ms_per_op": {k: round(v[1], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
     }
     print(json.dumps(line), flush=True)
 

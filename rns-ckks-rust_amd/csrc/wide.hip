@@ -4,9 +4,9 @@
 // multiplication on canonical u64 residues (wide.hpp).
 //
 // NTT: the merged-psi Cooley-Tukey forward transform (natural order in, bit-reversed out) and the
-// Gentleman-Sande inverse, as radix-2 stages.  Stages whose butterfly distance is at least the LDS
-// tile (4096 elements, 32 KiB) run as one streaming kernel per stage; the remaining stages of each
-// contiguous tile run in LDS in one kernel.  Negacyclic products, keys and accumulators use this
+// Gentleman-Sande inverse.  Stages whose butterfly distance is at least the LDS tile (4096 elements,
+// 32 KiB) run up to four at a time in registers (one streaming pass over the limb per four stages);
+// the remaining stages of each contiguous tile run in LDS in one kernel.  Negacyclic products, keys and accumulators use this
 // transform; the crate's cyclic NTT (src/math/ntt.rs:78-89) is a psi^{-j} twist, this transform and
 // a bit-reversal (and the reverse for the inverse).
 #include <algorithm>
@@ -135,31 +135,58 @@ void wide_tables_create(Context& c, const std::vector<uint64_t>& all, uint32_t T
 // ---------------------------------------------------------------------------------------
 // NTT
 // ---------------------------------------------------------------------------------------
-// One radix-2 stage of butterfly distance t = 2^lt >= tile over whole limbs: thread per butterfly.
-// Forward (CT): group i of the stage uses fw[N / 2t + i]; inverse (GS): iv[N / 2t + i].
-template <bool INV>
-__global__ __launch_bounds__(256) void k_w_ntt_stage(Batch<uint64_t> data, int logn, int lt, PrimeMap pm,
-                                                     const WidePrime* __restrict__ wp, int scale) {
-    const long long b = w_gidx();
-    if (b >= (1LL << (logn - 1))) return;
+// S consecutive radix-2 stages with butterfly distances from t = 2^lt (>= the tile) down to
+// t / 2^(S-1) (forward CT) or up from t / 2^(S-1) to t (inverse GS), in registers: a thread holds the
+// 2^S elements j = blk 2t + r + m delta (delta = 2t / 2^S, m < 2^S) that these stages combine.  The
+// stage of distance d uses table entry N / 2d + (j >> log2 2d) for element j.
+template <bool INV, int S>
+__global__ __launch_bounds__(256) void k_w_ntt_pass(Batch<uint64_t> data, int logn, int lt, PrimeMap pm,
+                                                    const WidePrime* __restrict__ wp, int scale) {
+    constexpr int M = 1 << S;
+    const long long gid = w_gidx();
+    if (gid >= (1LL << (logn - S))) return;
     const WidePrime P = wp[pm(blockIdx.y)];
     uint64_t* a = data.at(blockIdx.z, blockIdx.y);
-    const long long t = 1LL << lt, i = b >> lt, j = (i << (lt + 1)) + (b & (t - 1));
-    const long long m = 1LL << (logn - 1 - lt);
-    const uint64_t U = a[j], V = a[j + t];
+    const int ld = lt + 1 - S;  // log2 delta
+    const long long blk = gid >> ld, r = gid & ((1LL << ld) - 1);
+    const long long j0 = (blk << (lt + 1)) + r;
+    uint64_t v[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) v[m] = a[j0 + ((long long)m << ld)];
     if constexpr (!INV) {
-        const uint64_t W = w_mul(V, P.fw[m + i], P.q, P.qn);
-        a[j] = w_add(U, W, P.q);
-        a[j + t] = w_sub(U, W, P.q);
-    } else {
-        uint64_t x0 = w_add(U, V, P.q), x1 = w_mul(w_sub(U, V, P.q), P.iv[m + i], P.q, P.qn);
-        if (scale) {
-            x0 = w_mul(x0, P.ninv, P.q, P.qn);
-            x1 = w_mul(x1, P.ninv, P.q, P.qn);
+#pragma unroll
+        for (int k = 0; k < S; k++) {  // distance t >> k = 2^(ld + S - 1 - k) elements, 2^(S-1-k) registers
+            const int h = M >> (k + 1);
+            const long long base = (1LL << (logn - lt - 1 + k)) + (blk << k);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                if (m & h) continue;
+                const uint64_t W = w_mul(v[m + h], P.fw[base + (m >> (S - k))], P.q, P.qn);
+                const uint64_t U = v[m];
+                v[m] = w_add(U, W, P.q);
+                v[m + h] = w_sub(U, W, P.q);
+            }
         }
-        a[j] = x0;
-        a[j + t] = x1;
+    } else {
+#pragma unroll
+        for (int k = 0; k < S; k++) {  // distance delta 2^k
+            const int h = 1 << k;
+            const long long base = (1LL << (logn - ld - k - 1)) + (blk << (S - 1 - k));
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                if (m & h) continue;
+                const uint64_t U = v[m], V = v[m + h];
+                v[m] = w_add(U, V, P.q);
+                v[m + h] = w_mul(w_sub(U, V, P.q), P.iv[base + (m >> (k + 1))], P.q, P.qn);
+            }
+        }
+        if (scale) {
+#pragma unroll
+            for (int m = 0; m < M; m++) v[m] = w_mul(v[m], P.ninv, P.q, P.qn);
+        }
     }
+#pragma unroll
+    for (int m = 0; m < M; m++) a[j0 + ((long long)m << ld)] = v[m];
 }
 
 // The stages of butterfly distance below the tile (2^tb elements, contiguous) in LDS: forward
@@ -196,6 +223,23 @@ __global__ __launch_bounds__(256) void k_w_ntt_tile(Batch<uint64_t> data, int lo
     for (int i = threadIdx.x; i < tile; i += blockDim.x) src[i] = scale ? w_mul(a[i], P.ninv, P.q, P.qn) : a[i];
 }
 
+static void w_pass(bool inv, int st, Batch<uint64_t> d, int logn, int lt, int limbs, int polys, PrimeMap pm,
+                   const WidePrime* wp, int scale, hipStream_t s) {
+    const dim3 g = w_grid(1LL << (logn - st), limbs, polys);
+#define RCK_WP(I, S_) k_w_ntt_pass<I, S_><<<g, 256, 0, s>>>(d, logn, lt, pm, wp, scale)
+    switch (st * 2 + (inv ? 1 : 0)) {
+        case 2: RCK_WP(false, 1); break;
+        case 3: RCK_WP(true, 1); break;
+        case 4: RCK_WP(false, 2); break;
+        case 5: RCK_WP(true, 2); break;
+        case 6: RCK_WP(false, 3); break;
+        case 7: RCK_WP(true, 3); break;
+        case 8: RCK_WP(false, 4); break;
+        default: RCK_WP(true, 4); break;
+    }
+#undef RCK_WP
+}
+
 void wide_ntt(const Context& c, bool inverse, uint64_t* data, long long ls, long long ps, int limbs, int polys,
               PrimeMap pm, hipStream_t s, bool scale) {
     if (limbs <= 0 || polys <= 0) return;
@@ -203,15 +247,22 @@ void wide_ntt(const Context& c, bool inverse, uint64_t* data, long long ls, long
     const long long n = c.n;
     Batch<uint64_t> d{data, ls, ps};
     const dim3 gt((unsigned)(n >> tb), (unsigned)limbs, (unsigned)polys);
-    const dim3 gs = w_grid(n / 2, limbs, polys);
     ProfScope prof(s, inverse ? "wide_ntt_inv" : "wide_ntt_fwd");
     if (!inverse) {
-        for (int lt = logn - 1; lt >= tb; lt--) k_w_ntt_stage<false><<<gs, 256, 0, s>>>(d, logn, lt, pm, c.d_wp, 0);
+        // the stages above the tile, up to four per pass (largest distances first)
+        for (int lt = logn - 1; lt >= tb;) {
+            const int st = std::min(4, lt - tb + 1);
+            w_pass(false, st, d, logn, lt, limbs, polys, pm, c.d_wp, 0, s);
+            lt -= st;
+        }
         k_w_ntt_tile<false><<<gt, 256, 0, s>>>(d, logn, tb, pm, c.d_wp, 0);
     } else {
         k_w_ntt_tile<true><<<gt, 256, 0, s>>>(d, logn, tb, pm, c.d_wp, scale && tb == logn ? 1 : 0);
-        for (int lt = tb; lt < logn; lt++)
-            k_w_ntt_stage<true><<<gs, 256, 0, s>>>(d, logn, lt, pm, c.d_wp, scale && lt == logn - 1 ? 1 : 0);
+        for (int lo = tb; lo < logn;) {  // distances 2^lo .. 2^(lo + st - 1)
+            const int st = std::min(4, logn - lo);
+            w_pass(true, st, d, logn, lo + st - 1, limbs, polys, pm, c.d_wp, scale && lo + st == logn ? 1 : 0, s);
+            lo += st;
+        }
     }
     w_check();
 }
