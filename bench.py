@@ -30,6 +30,9 @@ CUS, NOMINAL_GHZ = 256, 2.4
 FP64_PER_BUTTERFLY = 8  # one exact FP64 modular product (6) + the add and the subtract
 
 
+KS_METHOD = {"exact": 0, "digit-ntt": 1, "auto": 2}  # rck_context_set_keyswitch_method values
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -201,7 +204,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed batches (batched) / ops (per-op)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mode", default="batched", choices=["batched", "per-op", "serve", "mixed", "bsgs", "wide"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "per-op", "serve", "mixed", "bsgs", "wide", "ntt"])
     ap.add_argument("--cts", type=int, default=256, help="mixed: ciphertexts in the batch")
     ap.add_argument("--dim", type=int, default=4096, help="bsgs: matrix dimension")
     ap.add_argument("--vectors", type=int, default=32, help="bsgs: encrypted vectors per batch")
@@ -210,8 +213,9 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=32.0, help="serve: working-set budget (GB)")
     ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent stream lanes")
     ap.add_argument("--batch", type=int, default=16, help="ciphertext pairs per batch")
-    ap.add_argument("--method", default="exact", choices=["exact", "digit-ntt"],
-                    help="keyswitch method of the measured keys (the other is timed beside it)")
+    ap.add_argument("--method", default="auto", choices=["auto", "exact", "digit-ntt"],
+                    help="keyswitch method of the measured keys (auto: keys hold both forms and each keyswitch "
+                         "takes the faster; the single methods are timed beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None,
                     help="oracle threads (default: the host cores this process may use)")
@@ -260,6 +264,9 @@ def main():
     if args.mode == "wide":
         wide_main(args, G, PR)
         return
+    if args.mode == "ntt":
+        ntt_main(args, G, PR)
+        return
 
     p = PR.config_per_op_gpu()
     n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
@@ -273,16 +280,16 @@ def main():
     rhs = G.DeviceBuffer.from_numpy(uniform(p.primes, n, rs, 2 * B))
     log(f"[bench] relin key: {D} digits x 2 x {L + K} limbs")
     key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
-    # the measured keys use --method; the other method's key is timed beside it (not in counter runs)
-    methods = {"exact": G.KS_EXACT_CONVOLUTION, "digit-ntt": G.KS_DIGIT_NTT}
-    other = "digit-ntt" if args.method == "exact" else "exact"
-    ctx.set_keyswitch_method(methods[args.method])
+    # the measured keys use --method; the single-method keys are timed beside it (not in counter runs)
+    ctx.set_keyswitch_method(KS_METHOD[args.method])
     ksk = ctx.ksk_create(key, L - 1, K)
-    ksk_alt = None
+    alts = {}
     if not args.counter_run:
-        ctx.set_keyswitch_method(methods[other])
-        ksk_alt = ctx.ksk_create(key, L - 1, K)
-        ctx.set_keyswitch_method(methods[args.method])
+        for other in ("exact", "digit-ntt"):
+            if other != args.method:
+                ctx.set_keyswitch_method(KS_METHOD[other])
+                alts[other] = ctx.ksk_create(key, L - 1, K)
+        ctx.set_keyswitch_method(KS_METHOD[args.method])
     del key
     quad = G.DeviceBuffer((3, L, n))
     out = G.DeviceBuffer((B, 2, L, n))
@@ -308,8 +315,8 @@ def main():
         log(f"[bench] per-op keyswitch tuned: { {k: v for k, v in tune1.items() if k != 'candidates'} }")
         for cand in sorted(tune1["candidates"], key=lambda c: c["ms"])[:8]:
             log(f"[bench]     candidate {cand}")
-        if ksk_alt is not None:
-            ctx.tune_keyswitch(L, ksk_alt, 1)
+        for k_alt in alts.values():
+            ctx.tune_keyswitch(L, k_alt, 1)
 
     bt = None
     if args.mode == "batched":
@@ -343,12 +350,12 @@ def main():
     per_op_steps = args.steps if args.mode == "per-op" else max(4, args.steps // 2)
     per_op_ms = time_it(G, per_op_step, per_op_steps) / per_op_steps
     per_op_by = {args.method: round(1000.0 / per_op_ms, 2)}
-    if ksk_alt is not None:
+    for other, k_alt in alts.items():
         for _ in range(args.warmup):
-            per_op_pair(0, one, ksk_alt)
-        ms_alt = time_it(G, lambda: per_op_pair(0, one, ksk_alt), per_op_steps) / per_op_steps
+            per_op_pair(0, one, k_alt)
+        ms_alt = time_it(G, lambda: per_op_pair(0, one, k_alt), per_op_steps) / per_op_steps
         per_op_by[other] = round(1000.0 / ms_alt, 2)
-    # the plain per-op path: the faster of the two methods
+    # the plain per-op path: the fastest of the methods
     per_op_value = max(per_op_by.values())
     log(f"[bench] per-op: {per_op_by} ops/s")
 
@@ -402,19 +409,22 @@ def main():
         value_rescale = B * 1000.0 / ms_r
         log(f"[bench] batched with rescale: {ms_r:.3f} ms per batch of {B} = {value_rescale:.1f} ops/s")
         del btr, out_r
-        # the same batch with the other method's key (same outputs)
-        out_a = G.DeviceBuffer((B, 2, L, n))
-        bta = G.Batch(ctx)
-        for b in range(B):
-            bta.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ksk_alt, out_a.offset(b * 2 * P))
-        bta.compile(tune=not args.no_tune)
-        for _ in range(args.warmup):
-            bta.run(stream)
-        ms_a = time_it(G, lambda: bta.run(stream), args.steps, stream) / args.steps
-        batched_by = {args.method: round(value, 2), other: round(B * 1000.0 / ms_a, 2)}
-        bit_exact = bit_exact and np.array_equal(out_a.numpy(), got)
+        # the same batch with the single-method keys (same outputs)
+        batched_by = {args.method: round(value, 2)}
+        for other, k_alt in alts.items():
+            out_a = G.DeviceBuffer((B, 2, L, n))
+            bta = G.Batch(ctx)
+            for b in range(B):
+                bta.add(G.OP_MULT_RELIN, L, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), k_alt,
+                        out_a.offset(b * 2 * P))
+            bta.compile(tune=not args.no_tune)
+            for _ in range(args.warmup):
+                bta.run(stream)
+            ms_a = time_it(G, lambda: bta.run(stream), args.steps, stream) / args.steps
+            batched_by[other] = round(B * 1000.0 / ms_a, 2)
+            bit_exact = bit_exact and np.array_equal(out_a.numpy(), got)
+            del bta, out_a
         log(f"[bench] batched by method: {batched_by} ops/s; outputs equal: {bit_exact}")
-        del bta, out_a
     else:
         ms_per_step, value = per_op_ms, per_op_value
         prof = profile_kernels(G, per_op_step, prof_steps)
@@ -605,6 +615,7 @@ def serve_main(args, G, PR):
     for n in sorted(need):
         p = PR.config_batched(n=n, limbs=24, aux=2)
         ctx = G.Context(p)
+        ctx.set_keyswitch_method(KS_METHOD[args.method])
         rng = G.Rng(n)
         s = ctx.sample_sparse_ternary(rng, 64, 24)
         pk = ctx.keygen_public(rng, s, 24, sigma=3.2)
@@ -689,6 +700,7 @@ def mixed_main(args, G, PR):
     n = 1 << 15
     p = PR.config_batched(n=n, limbs=20, aux=2)
     ctx = G.Context(p)
+    ctx.set_keyswitch_method(KS_METHOD[args.method])
     rs = np.random.default_rng(0)
     rng = G.Rng(3)
     t0 = time.perf_counter()
@@ -779,6 +791,44 @@ def mixed_main(args, G, PR):
     print(json.dumps(line), flush=True)
 
 
+def ntt_main(args, G, PR):
+    """NttPlan::forward + inverse (ntt.rs:78-89) of all 26 limbs of the headline chain (24 Q + 2 aux
+    primes below 2^50) at N=2^16 through rck_ntt_forward / rck_ntt_inverse, nothing else: one step is
+    one forward and one inverse transform of every limb (16 N bytes per limb and direction)."""
+    p = PR.config_per_op_gpu()
+    n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
+    ctx = G.Context(p)
+    rs = np.random.default_rng(9)
+    x = uniform(p.primes + p.aux_primes, n, rs, 1)[0]
+    buf = G.DeviceBuffer.from_numpy(x)
+    stream = G.Stream()
+
+    def step():
+        ctx.ntt_forward(buf, L, K, stream.ptr)
+        ctx.ntt_inverse(buf, L, K, stream.ptr)
+
+    for _ in range(args.warmup):
+        step()
+    ms = time_it(G, step, args.steps, stream) / args.steps
+    exact = bool(np.array_equal(buf.numpy().reshape(x.shape), x))
+    nb = 2 * (L + K) * 16 * n
+    gbs = nb / ms / 1e6
+    log(f"[bench] NTT round trip of {L + K} limbs at N=2^16: {ms * 1e3:.1f} us, {gbs:.1f} GB/s algorithmic, "
+        f"exact: {exact}")
+    line = {
+        "metric": "NTT forward+inverse of every limb (rck_ntt_forward/inverse), algorithmic GB/s",
+        "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 residues (FP64-exact modular arithmetic)", "data": "synthetic (seeded uniform residues)",
+        "config": {"workload": f"cyclic NTT round trip, N=2^16, {L} + {K} limbs (primes < 2^50)",
+                   "round_trip_exact": exact, "alg_bytes_per_step": nb},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def wide_main(args, G, PR):
     """The 64-bit integer arithmetic at N=2^16: 20 primes below 2^60 (q = 1 mod 2^17) and 2 aux primes,
     so D = 40 digits as at the headline workload; per-op mult+relinearize, rotation, and the crate's
@@ -840,6 +890,7 @@ def bsgs_main(args, G, PR):
     n, L, dim, B = 1 << 14, 12, args.dim, args.vectors
     p = PR.config_batched(n=n, limbs=L, aux=2)
     ctx = G.Context(p)
+    ctx.set_keyswitch_method(KS_METHOD[args.method])
     rs = np.random.default_rng(0)
     rng = G.Rng(4)
     t0 = time.perf_counter()

@@ -78,9 +78,13 @@ int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digi
  * convolution modulo the two primes' product (every digit transformed twice instead of once per eval
  * prime; the key takes twice the memory).  RCK_KS_DIGIT_NTT: keys kept in NTT form mod every eval
  * prime, every digit transformed under every eval prime.  Q-only keys and N <= 2^11 always use the
- * latter.  rck_ksk_method reports a key's method. */
+ * latter.  RCK_KS_AUTO (the default): such keys keep both forms (three times the memory of the
+ * digit-NTT form) and every keyswitch takes the faster method for its batch size (the tuned times of
+ * both when the tuning table has them, else the exact convolution from four ciphertexts up).
+ * rck_ksk_method reports a key's method (RCK_KS_AUTO when it holds both forms). */
 #define RCK_KS_EXACT_CONVOLUTION 0
 #define RCK_KS_DIGIT_NTT 1
+#define RCK_KS_AUTO 2
 int rck_context_set_keyswitch_method(rck_context* ctx, int method);
 int rck_ksk_method(const rck_ksk* key, int* method);
 /* Exact-convolution keyswitch at `limbs` Q limbs: the largest digit group whose exact sums stay below

@@ -192,14 +192,40 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
     const int E = (int)g.limbs + (int)g.key->aux;
     const int D = g.key->aux ? (int)((c.qbits[g.limbs] + 29) / 30) : (int)g.key->digits;
     const int B = (int)g.ops.size();
-    const bool exact = g.key->sdata != nullptr;
-    if (ks_tuned_find(c, (int)g.limbs, E, D, B, &g.shape, &g.tuned_ms, exact)) return;
+    // the key's methods: digit NTT (data), exact convolution (sdata), or both
+    std::vector<bool> methods;
+    if (g.key->data) methods.push_back(false);
+    if (g.key->sdata) methods.push_back(true);
+    {
+        // every method already tuned for this shape: take the fastest entry
+        bool all = true;
+        float best = 1e30f;
+        for (bool ex : methods) {
+            KsShape sh;
+            float ms = 0;
+            if (!ks_tuned_find(c, (int)g.limbs, E, D, B, &sh, &ms, ex)) {
+                all = false;
+                break;
+            }
+            if (ms < best) {
+                best = ms;
+                g.shape = sh;
+                g.tuned_ms = ms;
+            }
+        }
+        if (all) return;
+    }
     hipEvent_t e0, e1;
     RCK_CHECK_HIP(hipEventCreate(&e0));
     RCK_CHECK_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     KsShape best_shape{};
-    for (const KsShape& sh : ks_candidates(c, E, D, B, exact)) {
+    std::vector<KsShape> cands;
+    for (bool ex : methods)
+        for (const KsShape& sh : ks_candidates(c, E, D, B, ex)) cands.push_back(sh);
+    float best_m[2] = {1e30f, 1e30f};
+    KsShape shape_m[2] = {};
+    for (const KsShape& sh : cands) {
         run_group(bt, g, bt->stream, sh);  // warm-up (sizes the scratch)
         RCK_CHECK_HIP(hipEventRecord(e0, bt->stream));
         const int reps = 2;
@@ -210,6 +236,11 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
         RCK_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
         ms /= reps;
         g.tune_log += (g.tune_log.empty() ? "" : ", ") + shape_json(sh, ms);
+        const int m = sh.prefetch == kExactShape ? 1 : 0;
+        if (ms < best_m[m]) {
+            best_m[m] = ms;
+            shape_m[m] = sh;
+        }
         if (ms < best) {
             best = ms;
             best_shape = sh;
@@ -219,7 +250,7 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
     (void)hipEventDestroy(e1);
     g.shape = best_shape;
     g.tuned_ms = best;
-    ks_tuned_store(c, (int)g.limbs, E, D, B, best_shape, best, exact);
+    for (bool ex : methods) ks_tuned_store(c, (int)g.limbs, E, D, B, shape_m[ex ? 1 : 0], best_m[ex ? 1 : 0], ex);
 }
 
 // Per-op tuning: time every candidate shape of keyswitch_batch for B ciphertexts at `limbs` limbs
@@ -244,8 +275,15 @@ static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsSh
         for (int b = 0; b < B; b++) outs.push_back(out + b * 2 * P);
         float best = 1e30f;
         KsShape bs{};
-        const bool exact = key->sdata != nullptr;
-        for (const KsShape& sh : ks_candidates(c, E, D, B, exact)) {
+        std::vector<bool> methods;
+        if (key->data) methods.push_back(false);
+        if (key->sdata) methods.push_back(true);
+        std::vector<KsShape> cands;
+        for (bool ex : methods)
+            for (const KsShape& sh : ks_candidates(c, E, D, B, ex)) cands.push_back(sh);
+        float best_m[2] = {1e30f, 1e30f};
+        KsShape shape_m[2] = {};
+        for (const KsShape& sh : cands) {
             keyswitch_batch(c, in, P, B, limbs, key, nullptr, nullptr, 0, outs, s, sh);
             RCK_CHECK_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < 2; r++) keyswitch_batch(c, in, P, B, limbs, key, nullptr, nullptr, 0, outs, s, sh);
@@ -254,12 +292,17 @@ static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsSh
             float ms = 0;
             RCK_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (log) *log += (log->empty() ? "" : ", ") + shape_json(sh, ms / 2);
+            const int m = sh.prefetch == kExactShape ? 1 : 0;
+            if (ms / 2 < best_m[m]) {
+                best_m[m] = ms / 2;
+                shape_m[m] = sh;
+            }
             if (ms / 2 < best) {
                 best = ms / 2;
                 bs = sh;
             }
         }
-        ks_tuned_store(c, limbs, E, D, B, bs, best, exact);
+        for (bool ex : methods) ks_tuned_store(c, limbs, E, D, B, shape_m[ex ? 1 : 0], best_m[ex ? 1 : 0], ex);
         *best_out = bs;
         *best_ms = best;
     } catch (...) {

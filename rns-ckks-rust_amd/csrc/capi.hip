@@ -143,7 +143,8 @@ int rck_context_destroy(rck_context* ctx) {
 int rck_context_set_keyswitch_method(rck_context* ctx, int method) {
     return guard([&] {
         Context& c = C(ctx);
-        RCK_REQUIRE(method == RCK_KS_EXACT_CONVOLUTION || method == RCK_KS_DIGIT_NTT, "unknown keyswitch method");
+        RCK_REQUIRE(method == RCK_KS_EXACT_CONVOLUTION || method == RCK_KS_DIGIT_NTT || method == RCK_KS_AUTO,
+                    "unknown keyswitch method");
         c.ks_method = method;
     });
 }
@@ -182,7 +183,8 @@ int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digit
 int rck_ksk_method(const rck_ksk* key, int* method) {
     return guard([&] {
         RCK_REQUIRE(key != nullptr && method != nullptr, "null argument");
-        *method = key->sdata ? RCK_KS_EXACT_CONVOLUTION : RCK_KS_DIGIT_NTT;  // 64-bit integer keys: digit NTT
+        // 64-bit integer keys: digit NTT
+        *method = key->sdata ? (key->data ? RCK_KS_AUTO : RCK_KS_EXACT_CONVOLUTION) : RCK_KS_DIGIT_NTT;
     });
 }
 
@@ -421,7 +423,8 @@ static void ksk_create(rck_context* ctx, const uint64_t* key, bool on_device, ui
         }
         RCK_CHECK_HIP(hipDeviceSynchronize());
         if (stage) RCK_CHECK_HIP(hipFree(stage));
-        if (c.ks_method == RCK_KS_EXACT_CONVOLUTION && exact_keyswitch_available(c, aux)) key_make_exact(c, k);
+        if (c.ks_method != RCK_KS_DIGIT_NTT && exact_keyswitch_available(c, aux))
+            key_make_exact(c, k, c.ks_method == RCK_KS_AUTO);
     } catch (...) {
         if (k->data) (void)hipFree(k->data);
         if (k->sdata) (void)hipFree(k->sdata);

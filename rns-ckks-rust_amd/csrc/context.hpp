@@ -63,7 +63,8 @@ struct Context {
     uint64_t sp[2] = {0, 0};
     double sp_inv12 = 0;              // s1^{-1} mod s2 (centred)
     double* d_spmod = nullptr;        // [L+K][2]: s1 mod p_j, s1 s2 mod p_j (centred), per eval prime
-    int ks_method = 0;                // keys created from now on: 0 exact convolution (hybrid, N >= 2^12), 1 digit NTT
+    int ks_method = 2;                // keys created from now on: 0 exact convolution (hybrid, N >= 2^12), 1 digit
+                                      // NTT, 2 both forms (each keyswitch picks the faster for its batch size)
     // 64-bit integer arithmetic (wide.hpp): chosen when a modulus is at or above 2^50 + 2^46 (beyond
     // the FP64-exact kernels), or on request.  Per-prime constants, twiddles and basis-conversion
     // factors in Montgomery form (times R = 2^64), built on first use.

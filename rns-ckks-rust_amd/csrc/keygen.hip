@@ -317,8 +317,8 @@ rck_ksk* keygen_switch(rck_context* ctx, rck_rng* r, double sigma, const uint64_
             wide_key_create(c, k, coef);
             RCK_CHECK_HIP(hipFree(coef));
             coef = nullptr;
-        } else if (c.ks_method == RCK_KS_EXACT_CONVOLUTION && exact_keyswitch_available(c, (uint32_t)K)) {
-            key_make_exact(c, k);
+        } else if (c.ks_method != RCK_KS_DIGIT_NTT && exact_keyswitch_available(c, (uint32_t)K)) {
+            key_make_exact(c, k, c.ks_method == RCK_KS_AUTO);
         }
     } catch (...) {
         if (coef) (void)hipFree(coef);
@@ -518,7 +518,7 @@ int rck_ksk_export(rck_context* ctx, const rck_ksk* key, uint64_t* out) {
         const long long n = c.n, P = (long long)key->limbs * n;
         const PrimeMap pm{(int)key->level + 1, (int)c.L, 0};
         DevBuf t(2 * P * 8), u(2 * P * 8);
-        if (key->sdata) {
+        if (!key->data) {
             for (uint32_t d = 0; d < key->digits; d++) {
                 key_export_exact_digit(c, key, d, u.as<uint64_t>());
                 RCK_CHECK_HIP(hipMemcpy(out + (size_t)d * 2 * P, u.p, 2 * P * 8, hipMemcpyDeviceToHost));

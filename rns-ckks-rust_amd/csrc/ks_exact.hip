@@ -375,8 +375,8 @@ int exact_group_size(const Context& c, int nq) {
 bool exact_keyswitch_available(const Context& c, uint32_t aux) { return c.S == 2 && aux > 0 && c.logn >= 12; }
 
 // Converts a key from its NTT form mod the eval primes (data) to the exact-convolution form (sdata);
-// data is released.
-void key_make_exact(Context& c, KeyDev* k) {
+// data is released unless keep_data.
+void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
     require_fp64(c);
     RCK_REQUIRE(k && k->data, "null key");
     RCK_REQUIRE(exact_keyswitch_available(c, k->aux), "exact-convolution keys need hybrid keys at N >= 2^12");
@@ -409,8 +409,10 @@ void key_make_exact(Context& c, KeyDev* k) {
     }
     (void)hipFree(coeff);
     (void)hipFree(stage);
-    RCK_CHECK_HIP(hipFree(k->data));
-    k->data = nullptr;
+    if (!keep_data) {
+        RCK_CHECK_HIP(hipFree(k->data));
+        k->data = nullptr;
+    }
     k->sdata = sd;
 }
 

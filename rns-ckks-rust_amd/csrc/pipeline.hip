@@ -131,14 +131,17 @@ void rotate_batch(Context& c, const std::vector<const uint64_t*>& as, int limbs,
     keyswitch_batch(c, rot + P, 2 * P, B, limbs, key, rot, nullptr, 2 * P, outs, s, shape);
 }
 
-bool mult_relin_folds(const Context& c, const KeyDev* key) {
-    return c.logn >= 12 && key && key->aux > 0 && key->data != nullptr && !c.wide;
+// the relinearization fold needs the digit-NTT method on a hybrid key at a two-pass ring
+static bool mult_relin_folds(const Context& c, const KeyDev* key, int limbs, int B, const KsShape& shape) {
+    if (c.logn < 12 || !key || key->aux == 0 || key->data == nullptr || c.wide) return false;
+    const int D = (int)((c.qbits[limbs] + 29) / 30);
+    return !ks_use_exact(c, key, limbs, limbs + (int)key->aux, D, B, shape);
 }
 
 void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
                       const KeyDev* key, const std::vector<uint64_t*>& outs, hipStream_t s, KsShape shape) {
     const long long P = (long long)limbs * c.n;
-    if (mult_relin_folds(c, key)) {
+    if (mult_relin_folds(c, key, limbs, B, shape)) {
         // relinearize = (d0, d1) + keyswitch(d2): d0, d1 are added to the Q limbs of the keyswitch
         // accumulators in NTT form, times P, so mod-down by P returns keyswitch + (d0, d1) exactly
         uint64_t* c2 = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * P * 8));
@@ -272,6 +275,18 @@ int ks_tuned_import(const std::string& text) {
     return (int)rows.size();
 }
 
+bool ks_use_exact(const Context& c, const KeyDev* key, int limbs, int E, int D, int B, const KsShape& shape) {
+    if (!key->sdata) return false;
+    if (!key->data) return true;
+    if (shape.prefetch == kExactShape) return true;
+    if (shape.prefetch >= 0 || shape.group > 0 || shape.rows > 0) return false;  // an explicit digit-NTT shape
+    KsShape se, sd;
+    float me = 0, md = 0;
+    if (ks_tuned_find(c, limbs, E, D, B, &se, &me, true) && ks_tuned_find(c, limbs, E, D, B, &sd, &md, false))
+        return me < md;
+    return B >= 4;
+}
+
 KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B, bool exact) {
     KsShape r;
     if (ks_tuned_find(c, limbs, E, D, B, &r, nullptr, exact)) return r;
@@ -360,12 +375,12 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     }
     RCK_REQUIRE(key->data != nullptr || key->sdata != nullptr,
                 "keyswitch key was created for another arithmetic of this context");
-    if (key->sdata) {
-        RCK_REQUIRE(addn == nullptr, "folded addends need a digit-NTT key");
+    if (addn == nullptr && ks_use_exact(c, key, limbs, limbs + naux, levels, B, shape)) {
         const KsShape sh = shape.prefetch == kExactShape ? shape : ks_auto_shape(c, limbs, limbs + naux, levels, B, true);
         keyswitch_exact(c, poly, in_ps, B, limbs, key, levels, add0, add1, add_ps, outs, s, ginv, sh.rows, sh.width);
         return;
     }
+    RCK_REQUIRE(key->data != nullptr, "folded addends need a key with its digit-NTT form");
     RCK_REQUIRE(addn == nullptr || (naux > 0 && c.logn >= 12), "folded addends need hybrid keys and the two-pass NTT");
     RCK_REQUIRE(ginv == 0 || (naux > 0 && c.logn >= 12), "the fused automorphism needs hybrid keys and the two-pass NTT");
     const int E = limbs + naux;

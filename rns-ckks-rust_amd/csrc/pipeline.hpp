@@ -20,7 +20,8 @@ struct KeyDev {
 // exact-convolution keyswitch (ks_exact.hip)
 bool exact_keyswitch_available(const Context& c, uint32_t aux);
 int exact_group_size(const Context& c, int nq);
-void key_make_exact(Context& c, KeyDev* k);
+// keep_data: the key keeps its digit-NTT form as well (both methods available)
+void key_make_exact(Context& c, KeyDev* k, bool keep_data = false);
 void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out);
 void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
@@ -75,6 +76,10 @@ void keyswitch_impl(Context& c, const uint64_t* poly, int limbs, const KeyDev* k
 void run_op_group(Context& c, int kind, int limbs, const KeyDev* key, const std::vector<const uint64_t*>& as,
                   const std::vector<const uint64_t*>& bs, const std::vector<uint64_t*>& outs, hipStream_t s,
                   KsShape shape = {});
+// Does a keyswitch of B ciphertexts at `limbs` limbs (E eval limbs, D digits) with this key take the
+// exact convolution?  Keys with one form take it; for keys with both, an explicit shape decides, then
+// the tuned times of both methods, else B >= 4.
+bool ks_use_exact(const Context& c, const KeyDev* key, int limbs, int E, int D, int B, const KsShape& shape);
 // the automatic digit-MAC shape for B ciphertexts with E eval limbs and D digits: a tuned entry
 // when one exists for (N, limbs, E, D, B), otherwise a heuristic
 KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B, bool exact = false);

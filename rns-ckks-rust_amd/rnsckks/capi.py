@@ -19,6 +19,7 @@ u32p, u64p = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)
 
 KS_EXACT_CONVOLUTION = 0  # keyswitch methods (rck_context_set_keyswitch_method)
 KS_DIGIT_NTT = 1
+KS_AUTO = 2  # both key forms, the faster method per keyswitch
 ARITH_FP64 = 0  # kernel arithmetic (rck_context_set_arithmetic)
 ARITH_INT64 = 1
 
@@ -462,7 +463,7 @@ class Context:
         return self._scaled(lib().rck_rescale_quadratic, self.ptr, _p(quad), limbs, scale, _p(out))
 
     def set_keyswitch_method(self, method):
-        """Keyswitch method of the keys created from now on (KS_EXACT_CONVOLUTION or KS_DIGIT_NTT)."""
+        """Keyswitch method of the keys created from now on (KS_AUTO, KS_EXACT_CONVOLUTION or KS_DIGIT_NTT)."""
         _check(lib().rck_context_set_keyswitch_method(self.ptr, method))
 
     def set_arithmetic(self, arithmetic):

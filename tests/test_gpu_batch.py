@@ -185,9 +185,11 @@ def test_headline_workload_batched_matches_per_op_and_oracle():
     D = ctx.num_decomp_digits(L)
     assert D == 40
     key = np.stack([uniform(p.primes + p.aux_primes, n, rs, 2) for _ in range(D)])
-    k = ctx.ksk_create(key, L - 1, K)  # exact-convolution key (the default)
+    k = ctx.ksk_create(key, L - 1, K)  # both forms (the default)
     ctx.set_keyswitch_method(G.KS_DIGIT_NTT)
     kc = ctx.ksk_create(key, L - 1, K)
+    ctx.set_keyswitch_method(G.KS_EXACT_CONVOLUTION)
+    ke = ctx.ksk_create(key, L - 1, K)
     P = L * n
     lhs_h = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
     rhs_h = uniform(p.primes, n, rs, 2 * B).reshape(B, 2, L, n)
@@ -204,7 +206,7 @@ def test_headline_workload_batched_matches_per_op_and_oracle():
     O.set_threads(16)
     assert np.array_equal(want[0], O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs_h[0], rhs_h[0]), key))
     for pf in (5, 6, 7):  # the streamed digit-NTT MACs, then the exact-convolution method
-        key_used = k if pf == 7 else kc
+        key_used = ke if pf == 7 else kc
         G.tune_table_import(f"16 {L} {L + K} {D} {B} {L + K} 1 4 {pf} {8 if pf == 7 else 16} 1.0\n")
         out = G.DeviceBuffer((B, 2, L, n))
         bt = G.Batch(ctx)

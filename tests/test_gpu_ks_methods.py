@@ -1,9 +1,12 @@
 """The two keyswitch methods produce the crate's keyswitch bit for bit.
 
 * digit NTT: every digit transformed under every eval prime, key kept in NTT form mod each prime;
-* exact convolution (the default for hybrid keys at N >= 2^12): digits and centred key limbs
+* exact convolution (hybrid keys at N >= 2^12): digits and centred key limbs
   transformed under two auxiliary primes s1, s2, the products summed as exact integers modulo s1 s2
   in groups of digits small enough that no sum can wrap, then recombined and reduced mod each prime.
+
+Keys created with the automatic method (the default) hold both forms and every keyswitch takes one of
+them by batch size; both choices give the same bits.
 
 Both are checked against each other and the oracle (keyswitch_rns_hybrid, keyswitching.rs:159-194),
 including inputs built to drive the exact sums to their largest possible magnitude: every digit at
@@ -113,3 +116,39 @@ def test_exact_sums_at_their_largest_magnitude(limbs):
     assert np.array_equal(got, oc.numpy())
     O.set_threads(16)
     assert np.array_equal(got, O.relinearize(qb, eb, quad, key))
+
+
+def test_auto_keys_hold_both_forms_and_take_either_method():
+    """N = 2^13, 5 limbs: a key created with the automatic method keeps both forms; a single
+    mult+relin (digit NTT by default) and an untuned batch of 5 (exact convolution from four up) and a
+    tuned batch (both families timed) all equal the oracle's relinearization."""
+    n, L, K, B = 1 << 13, 5, 2, 5
+    p = PR.config_batched(n=1 << 16, limbs=L, aux=K)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    qb = O.Basis(n, p.primes, p.roots)
+    eb = O.Basis(n, p.primes + p.aux_primes, p.roots + p.aux_roots)
+    rs = np.random.default_rng(77)
+    D = ctx.num_decomp_digits(L)
+    key = np.stack([uniform(eb.moduli, n, rs, 2) for _ in range(D)])
+    k = ctx.ksk_create(key, L - 1, K)
+    assert ctx.ksk_method(k) == G.KS_AUTO
+    assert np.array_equal(ctx.ksk_export(k), key)
+    P = L * n
+    lhs = uniform(qb.moduli, n, rs, 2 * B).reshape(B, 2, L, n)
+    rhs = uniform(qb.moduli, n, rs, 2 * B).reshape(B, 2, L, n)
+    want = [O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs[b], rhs[b]), key) for b in range(B)]
+    quad, one = G.DeviceBuffer((3, L, n)), G.DeviceBuffer((2, L, n))
+    dl, dr = dev(lhs), dev(rhs)
+    for b in range(B):
+        ctx.mult(dl.offset(b * 2 * P), dr.offset(b * 2 * P), L, 40, 40, quad)
+        ctx.relinearize(quad, L, 80, k, one)
+        assert np.array_equal(one.numpy(), want[b]), b
+    for tune in (False, True):
+        out = G.DeviceBuffer((B, 2, L, n))
+        bt = G.Batch(ctx)
+        for b in range(B):
+            bt.add(G.OP_MULT_RELIN, L, dl.offset(b * 2 * P), dr.offset(b * 2 * P), k, out.offset(b * 2 * P))
+        bt.compile(tune=tune)
+        got = out.numpy()
+        for b in range(B):
+            assert np.array_equal(got[b], want[b]), (tune, b)
