@@ -450,9 +450,10 @@ def main():
     for k, (cnt, tot) in top:
         log(f"[bench]   {k:16s} launches={cnt:5d} total={tot:9.3f} ms avg={tot / cnt * 1e3:9.2f} us")
     kern_ms = {k: v[1] / prof_steps for k, v in top}
-    if args.mode == "batched":
-        # the batched mult's transforms: 4 L forward + L inverse (d2) limb NTTs per op, 16 N bytes each,
-        # over its forward column pass, fused rows + tensor, and inverse column passes
+    if args.mode == "batched" and "ks_mac" not in kern_ms:
+        # the batched mult's transforms (digit-NTT keyswitch): 4 L forward + L inverse (d2) limb NTTs per op,
+        # 16 N bytes each, over its forward column pass, fused rows + tensor, and inverse column passes
+        # (with the exact convolution the inverse column passes also carry the keyswitch accumulators)
         t_ms = sum(kern_ms.get(k, 0.0) for k in ("ntt_fwd_cols", "mult_rows", "ntt_inv_cols"))
         if t_ms > 0:
             nb = B * 5 * L * 16 * n
@@ -517,6 +518,23 @@ def main():
                "peak_basis": "39.3 T FP64 lane-instructions/s (78.6 TFLOP/s FMA) / 8 per butterfly "
                              "(one exact FP64 modular product + add + sub)",
                "frac": round(mm_total * value / mm_peak, 4)}
+    if rec:
+        # HBM bytes the NTT kernels move per launch (counters) over their launch time: the row passes,
+        # each a single kernel under its timing name
+        fr = {}
+        for tname, kname in (("ntt_inv_rows", "k_inv_rows"), ("ntt_fwd_rows", "k_fwd_rows"), ("mult_rows", "k_mult_rows")):
+            if tname not in prof:
+                continue
+            kc = [v for k, v in rec["kernels"].items() if kname in k and "fetch_bytes" in v and "write_bytes" in v]
+            if len(kc) != 1:
+                continue
+            cnt_t, tot_t = prof[tname]
+            nbytes = kc[0]["fetch_bytes"] + kc[0]["write_bytes"]
+            gbs = nbytes / (tot_t / cnt_t / 1e3) / 1e9
+            fr[tname] = {"hbm_bytes_per_launch": int(nbytes), "avg_launch_us": round(tot_t / cnt_t * 1e3, 2),
+                         "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if fr:
+            ntt["kernels_measured_traffic"] = fr
     if rec and rec.get("per_step"):
         f64_op = rec["per_step"]["valu_f64"] / B
         int_ops["fp64_wave_instr_per_op"] = int(f64_op)
