@@ -259,9 +259,80 @@ static void launch_mac_exact(const Context& c, int bt, int jt, const double* dn,
         mac_exact_j<1>(c, jt, dn, key, acc, B, D, gsize, G, E, s);
 }
 
+// Inverse column pass of the accumulators fused with the recombination: a workgroup transforms the
+// same S columns of one accumulator (digit group g, ciphertext b, key column jc) under s1 and then s2
+// (each exactly as k_inv_cols, with the N^{-1} scaling) and writes V mod p_j, V the group's exact sum
+// (crt_mod_q, |V mod p_j| <= p_j/2 + 1 as a double).  acc: [Z][2][N] after the inverse row pass,
+// Z = G B 2E; out: [Z][N].  Half the bytes of writing both residues for the recombination to read back.
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_inv_cols_crt(const double* __restrict__ acc, double* __restrict__ out,
+                                                      int logn, int E, int nq, int L, int sbase, SpecConst SK,
+                                                      const double* __restrict__ spmod,
+                                                      const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[R * S];
+    const int n = 1 << logn, N2 = n >> LOGR;
+    const int tid = threadIdx.x, col = tid % S, tau = tid / S;
+    const int r = blockIdx.x * S + col;
+    const long long z = blockIdx.y;
+    const int j = (int)(z % (2 * E)) % E;
+    const int pj = prime_of(j, nq, L);
+    // both primes' columns are loaded up front (the second set is in flight during the first transform)
+    double a1[16], a2[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) a1[e] = acc[(z * 2) * n + r + (long long)N2 * (16 * tau + e)];
+#pragma unroll
+    for (int e = 0; e < 16; e++) a2[e] = acc[(z * 2 + 1) * n + r + (long long)N2 * (16 * tau + e)];
+    auto column = [&](int s, double (&v)[16]) {
+        const DevPrime Ps = primes[sbase + s];
+        inv_phase_b<LOGR>(v, Ps.iv, 1, tau, Ps.q, Ps.qinv);
+        // LDS-only barriers: the second prime's loads stay in flight across the first exchange
+        if (s) lds_barrier();  // the first transform's reads of lds are done
+#pragma unroll
+        for (int e = 0; e < 16; e++) lds[(16 * tau + e) * S + col] = v[e];
+        lds_barrier();
+#pragma unroll
+        for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
+        inv_phase_a<kTwScalar>(v, Ps.iv, 1, Ps.q, Ps.qinv);
+#pragma unroll
+        for (int e = 0; e < 16; e++) v[e] = canon(mulmod(v[e], Ps.ninv, Ps.q, Ps.qinv), Ps.q, Ps.qinv);
+    };
+    column(0, a1);
+    column(1, a2);
+    const DevPrime Pj = primes[pj];
+    double* o = out + z * n;
+#pragma unroll
+    for (int e = 0; e < 16; e++)
+        o[r + (long long)N2 * (tau + TPS * e)] =
+            crt_mod_q(a1[e], a2[e], SK, Pj.q, Pj.qinv, spmod[2 * pj], spmod[2 * pj + 1]);
+}
+
+static void launch_inv_cols_crt(const Context& c, const double* acc, double* out, int nq, int Z, hipStream_t s) {
+    const int E = nq + (int)c.K;
+    const SpecConst SK = spec_const(c);
+    int n1, n2;
+    split_logn((int)c.logn, &n1, &n2);
+    const int S = 256 / ((1 << n1) / 16);
+    const dim3 grid((unsigned)((1 << n2) / S), (unsigned)Z);
+    ProfScope prof(s, "inv_cols_crt");
+    switch (n1) {
+#define RCK_IC(LR)                                                                                           \
+    case LR:                                                                                                 \
+        k_inv_cols_crt<LR><<<grid, 256, 0, s>>>(acc, out, (int)c.logn, E, nq, (int)c.L, (int)(c.L + c.K), SK, \
+                                                c.d_spmod, c.d_primes);                                      \
+        break;
+        RCK_IC(6)
+        RCK_IC(7)
+        RCK_IC(8)
+#undef RCK_IC
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+    RCK_CHECK_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------------------
-// Recombination and mod-down: one thread per (coefficient x, component, ciphertext).
-//   x_j = sum_g crt(acc[g][b][comp][j][s1], acc[..][s2]) mod p_j for every eval limb j;
+// Group sum and mod-down: one thread per (coefficient x, component, ciphertext).
+//   x_j = sum_g accq[g][b][comp][j] (each the recombined group sum mod p_j) for every eval limb j;
 //   the aux part through Garner (p_garner), then for each Q limb i
 //   out_i = (x_i - centred(x_P) mod q_i) P^{-1} mod q_i (+ addend)          (keyswitching.rs:278-322)
 // addends: add0 for component 0 (read through X -> X^g when ginv != 0, the rotated c0 of a rotation),
@@ -272,9 +343,8 @@ struct ExactOut {
 };
 
 template <int KT>
-__global__ __launch_bounds__(256) void k_crt_moddown(const uint64_t* __restrict__ accc, int n, int nq, int E, int B,
-                                                     int G, int L, int T, SpecConst SK,
-                                                     const double* __restrict__ spmod,
+__global__ __launch_bounds__(256) void k_crt_moddown(const double* __restrict__ accq, int n, int nq, int E, int B,
+                                                     int G, int L, int T,
                                                      const uint64_t* __restrict__ add0,
                                                      const uint64_t* __restrict__ add1, long long add_ps,
                                                      unsigned long long ginv, ExactOut outs, int b0,
@@ -289,21 +359,16 @@ __global__ __launch_bounds__(256) void k_crt_moddown(const uint64_t* __restrict_
     const int z = blockIdx.y, bl = z >> 1, comp = z & 1, b = b0 + bl;
     const int cols = 2 * E;
     // x_j for eval limb j (prime index pj), summed over the digit groups
-    auto xj = [&](int j, int pj) -> double {
-        const DevPrime Pj = primes[pj];
-        const double q = Pj.q, qinv = Pj.qinv, p1m = spmod[2 * pj], p12m = spmod[2 * pj + 1];
+    auto xj = [&](int j) -> double {
         double v = 0.0;
-        for (int g = 0; g < G; g++) {
-            const long long base = ((((long long)g * B + b) * cols + comp * E + j) * 2) * n + x;
-            v += crt_mod_q((double)accc[base], (double)accc[base + n], SK, q, qinv, p1m, p12m);
-        }
+        for (int g = 0; g < G; g++) v += accq[(((long long)g * B + b) * cols + comp * E + j) * n + x];
         return v;  // |v| <= G (q/2 + 1)
     };
     double r[KT], m[KT];
 #pragma unroll
     for (int k = 0; k < KT; k++) {
         const DevPrime Pk = primes[L + k];
-        r[k] = canon(xj(nq + k, L + k), Pk.q, Pk.qinv);
+        r[k] = canon(xj(nq + k), Pk.q, Pk.qinv);
     }
     const bool gt = p_garner(r, KT, L, T, m, primes, invq, pwords);
     const uint64_t* add = comp == 0 ? add0 : add1;
@@ -311,7 +376,7 @@ __global__ __launch_bounds__(256) void k_crt_moddown(const uint64_t* __restrict_
     for (int i = 0; i < nq; i++) {
         const DevPrime Pi = primes[i];
         const double q = Pi.q, qinv = Pi.qinv;
-        const double xi = reduce(xj(i, i), q, qinv);
+        const double xi = reduce(xj(i), q, qinv);
         double cv = reduce(m[KT - 1], q, qinv);
 #pragma unroll
         for (int kk = KT - 2; kk >= 0; kk--)
@@ -332,11 +397,10 @@ __global__ __launch_bounds__(256) void k_crt_moddown(const uint64_t* __restrict_
     }
 }
 
-static void launch_crt_moddown(const Context& c, const uint64_t* accc, int nq, int B, int G, const uint64_t* add0,
+static void launch_crt_moddown(const Context& c, const double* accq, int nq, int B, int G, const uint64_t* add0,
                                const uint64_t* add1, long long add_ps, unsigned long long ginv,
                                const std::vector<uint64_t*>& outs, hipStream_t s) {
     const int E = nq + (int)c.K;
-    const SpecConst SK = spec_const(c);
     for (int b0 = 0; b0 < B; b0 += 32) {
         const int bc = std::min(32, B - b0);
         ExactOut op{};
@@ -344,9 +408,9 @@ static void launch_crt_moddown(const Context& c, const uint64_t* accc, int nq, i
         dim3 grid((c.n + 255) / 256, 2 * bc);
         ProfScope ps(s, "crt_moddown");
 #define RCK_CM(KK)                                                                                             \
-    k_crt_moddown<KK><<<grid, 256, 0, s>>>(accc, (int)c.n, nq, E, B, G, (int)c.L, (int)(c.L + c.K), SK, c.d_spmod, \
-                                           add0, add1, add_ps, ginv, op, b0, c.d_primes, c.d_invq, c.d_pwords,  \
-                                           c.d_pmodq, c.d_pinvq, c.d_pmod_aux)
+    k_crt_moddown<KK><<<grid, 256, 0, s>>>(accq, (int)c.n, nq, E, B, G, (int)c.L, (int)(c.L + c.K), add0, add1,    \
+                                           add_ps, ginv, op, b0, c.d_primes, c.d_invq, c.d_pwords, c.d_pmodq,   \
+                                           c.d_pinvq, c.d_pmod_aux)
         switch (c.K) {
             case 1: RCK_CM(1); break;
             case 2: RCK_CM(2); break;
@@ -461,9 +525,12 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     // every digit under s1 and s2: limb stride 0 on the input (the same digit row for both)
     launch_fwd(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
     launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, gs, G, E, s);
-    uint64_t* accc = static_cast<uint64_t*>(c.workspace(32, (size_t)G * B * 2 * E * 2 * n * 8));
-    launch_inv(c, acc, n, 2 * n, kOutU64, accc, n, 2 * n, 2, G * B * 2 * E, ps, s);
-    launch_crt_moddown(c, accc, limbs, B, G, add0, add1, add_ps, ginv, outs, s);
+    // inverse rows under s1, s2; inverse columns fused with the recombination (one value per
+    // accumulator and coefficient: the group's exact sum mod p_j); then the group sum and mod-down
+    double* accq = static_cast<double*>(c.workspace(32, (size_t)G * B * 2 * E * n * 8));
+    launch_inv_rows(c, acc, n, 2 * n, 2, G * B * 2 * E, ps, s);
+    launch_inv_cols_crt(c, acc, accq, limbs, G * B * 2 * E, s);
+    launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s);
 }
 
 }  // namespace rck
