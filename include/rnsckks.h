@@ -87,8 +87,11 @@ int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digi
 #define RCK_KS_AUTO 2
 int rck_context_set_keyswitch_method(rck_context* ctx, int method);
 int rck_ksk_method(const rck_ksk* key, int* method);
-/* Exact-convolution keyswitch at `limbs` Q limbs: the largest digit group whose exact sums stay below
- * half the auxiliary primes' product, and the number of groups the active digits take. */
+/* Exact-convolution keyswitch at `limbs` Q limbs: the digit grouping it runs (digits per group, groups).
+ * A group's exact sums must stay below half the auxiliary primes' product.  When the digits of typical
+ * (uniform) ciphertexts allow it this is one group: the decomposition then measures every ciphertext's
+ * digits, and a batch with one whose digits could leave that range is recomputed on the device with
+ * the worst-case grouping (every digit at 2^29), so the results are exact for every input. */
 int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digits_per_group, uint32_t* groups);
 /* Arithmetic of the context's kernels; no counterpart in the crate, whose u128 products both reproduce
  * bit for bit (src/math/ntt.rs:132-150).  RCK_ARITH_FP64: FP64-exact modular products (every modulus

@@ -174,9 +174,11 @@ int rck_keyswitch_exact_groups(rck_context* ctx, uint32_t limbs, uint32_t* digit
         Context& c = C(ctx);
         RCK_REQUIRE(limbs >= 1 && limbs <= c.L, "bad limb count");
         RCK_REQUIRE(exact_keyswitch_available(c, c.K), "the context has no exact-convolution keyswitch");
-        const uint32_t D = (c.qbits[limbs] + 29) / 30, g = (uint32_t)exact_group_size(c, (int)limbs);
-        if (digits_per_group) *digits_per_group = g;
-        if (groups) *groups = (D + g - 1) / g;
+        const int D = (int)(c.qbits[limbs] + 29) / 30;
+        int g = 0, G = 0;
+        exact_grouping(c, (int)limbs, D, &g, &G);
+        if (digits_per_group) *digits_per_group = (uint32_t)g;
+        if (groups) *groups = (uint32_t)G;
     });
 }
 

@@ -28,13 +28,16 @@ constexpr int kDecBS = 64;
 // parked words of X and more waves stay resident.  NQM = 0: the digits live in LDS (any nq).
 // ginv != 0: the input is read through the automorphism X -> X^g (g * ginv = 1 mod 2N), i.e. the
 // decomposition of sigma_g(poly): coefficient c is (+/-) poly[c * ginv mod 2N] (as k_galois).
+// l1 (nullable; needs n % kDecBS == 0): l1[b][block] = sum of |d| over the block's coefficients and
+// every digit, an integer below 2^41 (exact as a double).
 template <int W, int NQM>
 __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict__ poly, long long ps, int n, int nq,
                                                       int levels,
                                                       const DevPrime* __restrict__ primes,
                                                       const double* __restrict__ invq, int T,
                                                       const uint32_t* __restrict__ qw, const uint32_t* __restrict__ hw,
-                                                      int32_t* __restrict__ digits, unsigned long long ginv) {
+                                                      int32_t* __restrict__ digits, unsigned long long ginv,
+                                                      double* __restrict__ l1) {
     // LDS (NQM == 0 only): [nq][kDecBS] Garner digits
     extern __shared__ double lds_m[];
     const int tid = threadIdx.x;
@@ -146,29 +149,40 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     // extension); d >= 2^29 becomes d - 2^30 with a carry into the next digit
     const uint32_t sx = (x[W - 1] >> 29) ? 0x3fffffffu : 0u;
     int carry = 0;
+    long long mag = 0;  // sum of |d| over the digits (< 2^35)
 #pragma unroll
     for (int i = 0; i < W; i++) {
         if (i < levels) {
             int d = (int)x[i] + carry;
             carry = d >= (1 << 29);
-            digits[(long long)i * n + c] = carry ? d - (1 << 30) : d;
+            d = carry ? d - (1 << 30) : d;
+            digits[(long long)i * n + c] = d;
+            mag += d < 0 ? -d : d;
         }
     }
     for (int i = W; i < levels; i++) {
         int d = (int)sx + carry;
         carry = d >= (1 << 29);
-        digits[(long long)i * n + c] = carry ? d - (1 << 30) : d;
+        d = carry ? d - (1 << 30) : d;
+        digits[(long long)i * n + c] = d;
+        mag += d < 0 ? -d : d;
+    }
+    if (l1 != nullptr) {
+        double t = (double)mag;
+#pragma unroll
+        for (int off = kDecBS / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, kDecBS);
+        if (tid == 0) l1[(long long)blockIdx.y * gridDim.x + blockIdx.x] = t;
     }
 }
 
 template <int W, int NQM>
 static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
-                   long long ps, unsigned long long ginv) {
+                   long long ps, unsigned long long ginv, double* l1) {
     const uint32_t* qw = c.d_qwords + (size_t)nq * 2 * kWords30;
     ProfScope prof(s, "decompose");
     const size_t lds = NQM == 0 ? (size_t)nq * kDecBS * 8 : 0;
     k_decompose<W, NQM><<<dim3((c.n + kDecBS - 1) / kDecBS, B), kDecBS, lds, s>>>(
-        poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords30, digits, ginv);
+        poly, ps, c.n, nq, levels, c.d_primes, c.d_invq, c.L + c.K, qw, qw + kWords30, digits, ginv, l1);
     RCK_CHECK_HIP(hipGetLastError());
 }
 
@@ -178,39 +192,40 @@ static void dec_tq(const Context& c, const uint64_t* poly, int nq, int levels, i
 // bits of Q, so NQM limbs of at most 51 bits need W >= 51 NQM / 30: W > 48 only pairs with NQM = 0.
 template <int W>
 static void dec_t(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
-                  long long ps, unsigned long long ginv) {
+                  long long ps, unsigned long long ginv, double* l1) {
     if constexpr (W <= 48) {
-        if (!c.garner_lazy) return dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv);
-        if (nq <= 8) return dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps, ginv);
-        if (nq <= 16) return dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps, ginv);
-        if (nq <= 24) return dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        if (!c.garner_lazy) return dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
+        if (nq <= 8) return dec_tq<W, 8>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
+        if (nq <= 16) return dec_tq<W, 16>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
+        if (nq <= 24) return dec_tq<W, 24>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     }
-    dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv);
+    dec_tq<W, 0>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
 }
 
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s, int B,
-                      long long ps, unsigned long long ginv) {
+                      long long ps, unsigned long long ginv, double* l1) {
     require_fp64(c);
+    RCK_REQUIRE(l1 == nullptr || c.n % kDecBS == 0, "digit magnitudes need whole blocks");
     RCK_REQUIRE(nq >= 1 && nq <= (int)c.L, "bad limb count for decomposition");
     RCK_REQUIRE(levels >= 1, "decomposition needs at least one digit");
     // 30-bit words holding Q_nq (and the centred value in two's complement)
     const int need = std::max(1, (int)(c.qbits[nq] + 29) / 30);
     if (need <= 8)
-        dec_t<8>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<8>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else if (need <= 16)
-        dec_t<16>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<16>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else if (need <= 24)
-        dec_t<24>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<24>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else if (need <= 32)
-        dec_t<32>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<32>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else if (need <= 40)
-        dec_t<40>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<40>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else if (need <= 48)
-        dec_t<48>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<48>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else if (need <= 64)
-        dec_t<64>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<64>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else if (need <= kWords30)
-        dec_t<kWords30>(c, poly, nq, levels, digits, s, B, ps, ginv);
+        dec_t<kWords30>(c, poly, nq, levels, digits, s, B, ps, ginv, l1);
     else
         RCK_REQUIRE(false, "decomposition needs more than 2160 bits");
 }

@@ -188,7 +188,9 @@ template <int BT, int JT>
 __global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__ dn, const double* __restrict__ key,
                                                       double* __restrict__ acc, int n, int B, int D, int gsize,
                                                       int cols, int WB, int njg, int nxb, int nbg, int items,
-                                                      int sbase, const DevPrime* __restrict__ primes) {
+                                                      int sbase, const DevPrime* __restrict__ primes,
+                                                      const int* __restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;
     const int w = blockIdx.x;
     const int per = gridDim.x >> 3;
     const int item = (w & 7) * per + (w >> 3);
@@ -217,7 +219,7 @@ __global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__
 
 template <int BT, int JT>
 static void mac_exact_t(const Context& c, const double* dn, const double* key, double* acc, int B, int D, int gsize,
-                        int G, int E, hipStream_t s) {
+                        int G, int E, hipStream_t s, const int* gate) {
     const int n = (int)c.n;
     const int nbt = (B + BT - 1) / BT;  // b-tiles; WB of them per workgroup (1, 2 or 4 waves along them)
     const int WB = nbt >= 3 ? 4 : nbt;
@@ -230,33 +232,34 @@ static void mac_exact_t(const Context& c, const double* dn, const double* key, d
     const long long items = (long long)njg * nxb * nbg * 2 * G;
     RCK_REQUIRE(items < (1LL << 30), "exact keyswitch launch too large");
     const int per = (int)((items + 7) / 8);
-    ProfScope ps(s, "ks_mac");
+    ProfScope ps(s, gate ? "ks_mac_gated" : "ks_mac");
     k_ks_mac_exact<BT, JT><<<dim3(per * 8), 256, 0, s>>>(dn, key, acc, n, B, D, gsize, cols, WB, njg, nxb, nbg,
-                                                         (int)items, (int)(c.L + c.K), c.d_primes);
+                                                         (int)items, (int)(c.L + c.K), c.d_primes, gate);
     RCK_CHECK_HIP(hipGetLastError());
 }
 
 template <int BT>
 static void mac_exact_j(const Context& c, int jt, const double* dn, const double* key, double* acc, int B, int D,
-                        int gsize, int G, int E, hipStream_t s) {
+                        int gsize, int G, int E, hipStream_t s, const int* gate) {
     if (jt == 4)
-        mac_exact_t<BT, 4>(c, dn, key, acc, B, D, gsize, G, E, s);
+        mac_exact_t<BT, 4>(c, dn, key, acc, B, D, gsize, G, E, s, gate);
     else if (jt == 16)
-        mac_exact_t<BT, 16>(c, dn, key, acc, B, D, gsize, G, E, s);
+        mac_exact_t<BT, 16>(c, dn, key, acc, B, D, gsize, G, E, s, gate);
     else
-        mac_exact_t<BT, 8>(c, dn, key, acc, B, D, gsize, G, E, s);
+        mac_exact_t<BT, 8>(c, dn, key, acc, B, D, gsize, G, E, s, gate);
 }
 
-// bt ciphertexts by jt key columns per thread (bt in {1, 2, 4}, jt in {4, 8, 16})
+// bt ciphertexts by jt key columns per thread (bt in {1, 2, 4}, jt in {4, 8, 16}); gate (nullable,
+// device): the launch does nothing unless *gate != 0
 static void launch_mac_exact(const Context& c, int bt, int jt, const double* dn, const double* key, double* acc, int B,
-                             int D, int gsize, int G, int E, hipStream_t s) {
+                             int D, int gsize, int G, int E, hipStream_t s, const int* gate = nullptr) {
     RCK_REQUIRE((bt == 1 || bt == 2 || bt == 4) && (jt == 4 || jt == 8 || jt == 16), "bad exact keyswitch tile");
     if (bt == 4)
-        mac_exact_j<4>(c, jt, dn, key, acc, B, D, gsize, G, E, s);
+        mac_exact_j<4>(c, jt, dn, key, acc, B, D, gsize, G, E, s, gate);
     else if (bt == 2)
-        mac_exact_j<2>(c, jt, dn, key, acc, B, D, gsize, G, E, s);
+        mac_exact_j<2>(c, jt, dn, key, acc, B, D, gsize, G, E, s, gate);
     else
-        mac_exact_j<1>(c, jt, dn, key, acc, B, D, gsize, G, E, s);
+        mac_exact_j<1>(c, jt, dn, key, acc, B, D, gsize, G, E, s, gate);
 }
 
 // Inverse column pass of the accumulators fused with the recombination: a workgroup transforms the
@@ -268,9 +271,11 @@ template <int LOGR>
 __global__ __launch_bounds__(256) void k_inv_cols_crt(const double* __restrict__ acc, double* __restrict__ out,
                                                       int logn, int E, int nq, int L, int sbase, SpecConst SK,
                                                       const double* __restrict__ spmod,
-                                                      const DevPrime* __restrict__ primes) {
+                                                      const DevPrime* __restrict__ primes,
+                                                      const int* __restrict__ gate) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[R * S];
+    if (gate != nullptr && *gate == 0) return;
     const int n = 1 << logn, N2 = n >> LOGR;
     const int tid = threadIdx.x, col = tid % S, tau = tid / S;
     const int r = blockIdx.x * S + col;
@@ -307,19 +312,20 @@ __global__ __launch_bounds__(256) void k_inv_cols_crt(const double* __restrict__
             crt_mod_q(a1[e], a2[e], SK, Pj.q, Pj.qinv, spmod[2 * pj], spmod[2 * pj + 1]);
 }
 
-static void launch_inv_cols_crt(const Context& c, const double* acc, double* out, int nq, int Z, hipStream_t s) {
+static void launch_inv_cols_crt(const Context& c, const double* acc, double* out, int nq, int Z, hipStream_t s,
+                                const int* gate = nullptr) {
     const int E = nq + (int)c.K;
     const SpecConst SK = spec_const(c);
     int n1, n2;
     split_logn((int)c.logn, &n1, &n2);
     const int S = 256 / ((1 << n1) / 16);
     const dim3 grid((unsigned)((1 << n2) / S), (unsigned)Z);
-    ProfScope prof(s, "inv_cols_crt");
+    ProfScope prof(s, gate ? "inv_cols_crt_gated" : "inv_cols_crt");
     switch (n1) {
 #define RCK_IC(LR)                                                                                           \
     case LR:                                                                                                 \
         k_inv_cols_crt<LR><<<grid, 256, 0, s>>>(acc, out, (int)c.logn, E, nq, (int)c.L, (int)(c.L + c.K), SK, \
-                                                c.d_spmod, c.d_primes);                                      \
+                                                c.d_spmod, c.d_primes, gate);                                \
         break;
         RCK_IC(6)
         RCK_IC(7)
@@ -353,7 +359,9 @@ __global__ __launch_bounds__(256) void k_crt_moddown(const double* __restrict__ 
                                                      const uint32_t* __restrict__ pwords,
                                                      const double* __restrict__ pmodq,
                                                      const double* __restrict__ pinvq,
-                                                     const double* __restrict__ pmod_aux) {
+                                                     const double* __restrict__ pmod_aux,
+                                                     const int* __restrict__ gate, int want) {
+    if (gate != nullptr && (*gate != 0) != (want != 0)) return;
     const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
     const int z = blockIdx.y, bl = z >> 1, comp = z & 1, b = b0 + bl;
@@ -397,20 +405,22 @@ __global__ __launch_bounds__(256) void k_crt_moddown(const double* __restrict__ 
     }
 }
 
+// gate (nullable, device): the launch does nothing unless (*gate != 0) == want
 static void launch_crt_moddown(const Context& c, const double* accq, int nq, int B, int G, const uint64_t* add0,
                                const uint64_t* add1, long long add_ps, unsigned long long ginv,
-                               const std::vector<uint64_t*>& outs, hipStream_t s) {
+                               const std::vector<uint64_t*>& outs, hipStream_t s, const int* gate = nullptr,
+                               int want = 0) {
     const int E = nq + (int)c.K;
     for (int b0 = 0; b0 < B; b0 += 32) {
         const int bc = std::min(32, B - b0);
         ExactOut op{};
         for (int b = 0; b < bc; b++) op.p[b] = outs[b0 + b];
         dim3 grid((c.n + 255) / 256, 2 * bc);
-        ProfScope ps(s, "crt_moddown");
+        ProfScope ps(s, gate && want ? "crt_moddown_gated" : "crt_moddown");
 #define RCK_CM(KK)                                                                                             \
     k_crt_moddown<KK><<<grid, 256, 0, s>>>(accq, (int)c.n, nq, E, B, G, (int)c.L, (int)(c.L + c.K), add0, add1,    \
                                            add_ps, ginv, op, b0, c.d_primes, c.d_invq, c.d_pwords, c.d_pmodq,   \
-                                           c.d_pinvq, c.d_pmod_aux)
+                                           c.d_pinvq, c.d_pmod_aux, gate, want)
         switch (c.K) {
             case 1: RCK_CM(1); break;
             case 2: RCK_CM(2); break;
@@ -434,6 +444,66 @@ int exact_group_size(const Context& c, int nq) {
     const u128 g = lim / per;
     RCK_REQUIRE(g >= 1, "auxiliary primes too small for one digit");
     return (int)std::min<u128>(g, 1 << 20);
+}
+
+// One group for a batch whose digits allow it.  A ciphertext's one-group sums satisfy |V_j| <= S maxhalf,
+// S = sum_i ||d_i||_1 (the L1 norm of all its digits), so they are exact when S <= floor(lim / maxhalf),
+// lim = (s1 s2 - 1)/2.  The digits of a uniform residue average 2^28 in magnitude (S near levels N 2^28),
+// so when levels N 2^28 (5/4) is below that threshold the keyswitch runs one group, the decomposition
+// records every ciphertext's S, and a batch with some S above the threshold is recomputed with the
+// worst-case groups (the gate is set on the device; the launches of the other path do nothing).
+struct ExactPlan {
+    int G, gsize;    // grouping of the main path
+    int Gw, gsw;     // worst-case grouping (the recomputation when gated)
+    bool gated;      // main path one group, checked per batch
+    double thr;      // the threshold on S
+};
+
+static ExactPlan exact_plan(const Context& c, int nq, int levels) {
+    using u128 = unsigned __int128;
+    ExactPlan pl{};
+    const int gw = exact_group_size(c, nq);
+    pl.Gw = (levels + gw - 1) / gw;
+    pl.gsw = (levels + pl.Gw - 1) / pl.Gw;  // balanced groups
+    pl.G = pl.Gw;
+    pl.gsize = pl.gsw;
+    if (pl.Gw == 1 || c.n % 64 != 0) return pl;
+    uint64_t maxhalf = 0;
+    for (int j = 0; j < nq; j++) maxhalf = std::max(maxhalf, (c.hp[j].q - 1) / 2);
+    for (uint32_t k = 0; k < c.K; k++) maxhalf = std::max(maxhalf, (c.hp[c.L + k].q - 1) / 2);
+    const u128 lim = ((u128)c.sp[0] * c.sp[1] - 1) / 2;
+    const u128 thr = lim / maxhalf;
+    const u128 typical = (u128)levels * c.n * ((u128)1 << 28) * 5 / 4;
+    if (typical > thr) return pl;
+    pl.gated = true;
+    pl.G = 1;
+    pl.gsize = levels;
+    pl.thr = thr >= ((u128)1 << 53) ? 9007199254740992.0 : (double)(uint64_t)thr;  // exact below 2^53
+    return pl;
+}
+
+void exact_grouping(const Context& c, int nq, int levels, int* gsize, int* G) {
+    const ExactPlan pl = exact_plan(c, nq, levels);
+    *gsize = pl.gsize;
+    *G = pl.G;
+}
+
+// gate[0] = 1 when some ciphertext's S = sum of l1[b][0 .. nblk) exceeds thr, else 0 (every partial
+// is an integer and every sum stays below 2^53, so the sums are exact in any order).  Wave w of the
+// 16 sums ciphertexts w, w + 16, ...
+__global__ __launch_bounds__(1024) void k_exact_gate(const double* __restrict__ l1, int B, int nblk, double thr,
+                                                     int* __restrict__ gate) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int flag = 0;
+    for (int b = wave; b < B; b += 16) {
+        double t = 0.0;
+        for (int i = lane; i < nblk; i += 64) t += l1[(long long)b * nblk + i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        flag |= t > thr;
+    }
+    flag = __syncthreads_or(flag);
+    if (threadIdx.x == 0) gate[0] = flag;
 }
 
 bool exact_keyswitch_available(const Context& c, uint32_t aux) { return c.S == 2 && aux > 0 && c.logn >= 12; }
@@ -513,24 +583,43 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     require_fp64(c);
     const long long n = c.n;
     const int E = limbs + (int)key->aux;
-    const int gsize = exact_group_size(c, limbs);
-    const int G = (levels + gsize - 1) / gsize;
-    const int gs = (levels + G - 1) / G;  // balanced groups
+    const ExactPlan pl = exact_plan(c, limbs, levels);
+    const int G = pl.G, Gw = pl.Gw;
     int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
-    // workspace slots 30-32 (slots 20-26 hold the BSGS buffers of a transform whose rotations come here)
+    // workspace slots 30-34 (slots 20-26 hold the BSGS buffers of a transform whose rotations come here);
+    // the accumulators are sized for the worst-case groups, which the recomputation reuses
     double* dn = static_cast<double*>(c.workspace(30, (size_t)B * levels * 2 * n * 8));
-    double* acc = static_cast<double*>(c.workspace(31, (size_t)G * B * 2 * E * 2 * n * 8));
-    launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps, ginv);
+    double* acc = static_cast<double*>(c.workspace(31, (size_t)Gw * B * 2 * E * 2 * n * 8));
+    double* accq = static_cast<double*>(c.workspace(32, (size_t)Gw * B * 2 * E * n * 8));
+    double* l1 = nullptr;
+    int* gate = nullptr;
+    if (pl.gated) {
+        l1 = static_cast<double*>(c.workspace(33, (size_t)B * (n / 64) * 8));
+        gate = static_cast<int*>(c.workspace(34, 256));
+    }
+    launch_decompose(c, poly, limbs, levels, digits, s, B, in_ps, ginv, l1);
+    if (pl.gated) {
+        ProfScope prof(s, "exact_gate");
+        k_exact_gate<<<1, 1024, 0, s>>>(l1, B, (int)(n / 64), pl.thr, gate);
+        RCK_CHECK_HIP(hipGetLastError());
+    }
     const PrimeMap ps{0, (int)(c.L + c.K), 0};
     // every digit under s1 and s2: limb stride 0 on the input (the same digit row for both)
     launch_fwd(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
-    launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, gs, G, E, s);
+    launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, pl.gsize, G, E, s);
     // inverse rows under s1, s2; inverse columns fused with the recombination (one value per
     // accumulator and coefficient: the group's exact sum mod p_j); then the group sum and mod-down
-    double* accq = static_cast<double*>(c.workspace(32, (size_t)G * B * 2 * E * n * 8));
+    // (skipped when the gate is set)
     launch_inv_rows(c, acc, n, 2 * n, 2, G * B * 2 * E, ps, s);
     launch_inv_cols_crt(c, acc, accq, limbs, G * B * 2 * E, s);
-    launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s);
+    launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s, gate, 0);
+    if (pl.gated) {
+        // some ciphertext's digits exceed the one-group bound: the batch again with the worst-case groups
+        launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, pl.gsw, Gw, E, s, gate);
+        launch_inv_rows(c, acc, n, 2 * n, 2, Gw * B * 2 * E, ps, s, 1, 0, nullptr, 0, 0, 0, gate);
+        launch_inv_cols_crt(c, acc, accq, limbs, Gw * B * 2 * E, s, gate);
+        launch_crt_moddown(c, accq, limbs, B, Gw, add0, add1, add_ps, ginv, outs, s, gate, 1);
+    }
 }
 
 }  // namespace rck

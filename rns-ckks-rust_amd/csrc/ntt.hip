@@ -85,13 +85,16 @@ __global__ __launch_bounds__(256) void k_fwd_rows(Batch<double> data, int logn, 
 // addn (nullable): for the first nq limbs, P * addn[z >> 1][z & 1] (NTT form, P mod q_y = pmodq[y]) is
 // added too: the relinearization terms folded into keyswitch accumulators [B][2][E][N] (mod-down by P
 // then returns keyswitch + addend exactly).
+// gate (nullable): the launch does nothing unless *gate != 0 (a recomputation chosen on the device).
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_inv_rows(Batch<double> data, int logn, PrimeMap pm,
                                                   const DevPrime* __restrict__ primes, int nsum, long long sstride,
                                                   const double* __restrict__ addn, long long addn_bs,
-                                                  long long addn_cs, int nq, const double* __restrict__ pmodq) {
+                                                  long long addn_cs, int nq, const double* __restrict__ pmodq,
+                                                  const int* __restrict__ gate) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[S * row_pitch<R>()];
+    if (gate != nullptr && *gate == 0) return;
     const int N1 = 1 << (logn - LOGR);
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
     const int b = blockIdx.x * S + row;
@@ -344,12 +347,13 @@ static void fwd_rows_t(const Context& c, Batch<double> d, int limbs, int polys, 
 
 template <int LOGR>
 static void inv_rows_t(const Context& c, Batch<double> d, int limbs, int polys, PrimeMap pm, hipStream_t s, int nsum,
-                       long long sstride, const double* addn, long long addn_bs, long long addn_cs, int nq) {
+                       long long sstride, const double* addn, long long addn_bs, long long addn_cs, int nq,
+                       const int* gate) {
     constexpr int S = 256 / ((1 << LOGR) / 16);
     const int n1 = 1 << (c.logn - LOGR);
-    ProfScope ps(s, "ntt_inv_rows");
+    ProfScope ps(s, gate ? "ntt_inv_rows_gated" : "ntt_inv_rows");
     k_inv_rows<LOGR><<<dim3(n1 / S, limbs, polys), 256, 0, s>>>(d, c.logn, pm, c.d_primes, nsum, sstride, addn,
-                                                                addn_bs, addn_cs, nq, c.d_pmodq);
+                                                                addn_bs, addn_cs, nq, c.d_pmodq, gate);
     RCK_LAUNCH_CHECK();
 }
 
@@ -446,7 +450,7 @@ void launch_fwd_rows(const Context& c, double* data, long long ls, long long ps,
 
 void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
                      hipStream_t s, int nsum, long long sstride, const double* addn, long long addn_bs,
-                     long long addn_cs, int nq) {
+                     long long addn_cs, int nq, const int* gate) {
     require_fp64(c);
     check_dims(c, limbs, polys);
     RCK_REQUIRE(nsum >= 1 && nsum <= 8, "at most 8 partial sums");
@@ -455,9 +459,9 @@ void launch_inv_rows(const Context& c, double* data, long long ls, long long ps,
     int n1, n2;
     split_logn(c.logn, &n1, &n2);
     switch (n2) {
-        case 6: inv_rows_t<6>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq); break;
-        case 7: inv_rows_t<7>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq); break;
-        case 8: inv_rows_t<8>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq); break;
+        case 6: inv_rows_t<6>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq, gate); break;
+        case 7: inv_rows_t<7>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq, gate); break;
+        case 8: inv_rows_t<8>(c, b, limbs, polys, pm, s, nsum, sstride, addn, addn_bs, addn_cs, nq, gate); break;
         default: RCK_REQUIRE(false, "unsupported ring degree");
     }
 }

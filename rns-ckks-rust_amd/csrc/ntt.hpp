@@ -237,9 +237,10 @@ void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, 
                 long long sstride = 0);
 // addn (nullable): for polys = 2B keyswitch accumulators [B][2][limbs][N], adds P * addn (NTT form, ciphertext
 // stride addn_bs, component stride addn_cs, limb stride N) to the first nq limbs
+// gate (nullable, device): the launch does nothing unless *gate != 0
 void launch_inv_rows(const Context& c, double* data, long long ls, long long ps, int limbs, int polys, PrimeMap pm,
                      hipStream_t s, int nsum = 1, long long sstride = 0, const double* addn = nullptr,
-                     long long addn_bs = 0, long long addn_cs = 0, int nq = 0);
+                     long long addn_bs = 0, long long addn_cs = 0, int nq = 0, const int* gate = nullptr);
 // xl: B x [a0 a1], xr: B x [b0 b1] after the forward column pass (limbs limbs each) -> d: B x [d0 d1 d2]
 // after the inverse row pass (tensor fused between the row passes); fold: d0, d1 stay in NTT form
 void launch_mult_rows(const Context& c, const double* xl, const double* xr, double* d, int limbs, int B, PrimeMap pm,

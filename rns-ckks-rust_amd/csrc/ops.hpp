@@ -35,8 +35,9 @@ void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // `levels` signed digits per coefficient: digits[i * N + c] (keyswitching.rs:196-236, 381-467).
 // B polynomials at poly stride `ps` elements -> digits [B][levels][N].
 // ginv != 0: decompose sigma_g(poly) instead (coefficients gathered through X -> X^g, g * ginv = 1 mod 2N).
+// l1 (nullable): [B][N / 64] partial sums of |digit| (each over 64 coefficients and every digit).
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s,
-                      int B = 1, long long ps = 0, unsigned long long ginv = 0);
+                      int B = 1, long long ps = 0, unsigned long long ginv = 0, double* l1 = nullptr);
 // For B ciphertexts sharing one key: acc_{0,1}[j] = sum_i NTT_j(digit_i) * key[i][{0,1}][j] for every eval
 // prime j (Q-prefix nq, then naux aux primes), NTT form.  digits: [B][D][N]; key: [>=D][2][nq+naux][N]
 // NTT-form doubles; part: [dsplit][B][2][nq+naux][N] partial sums over digit ranges.  Eval limbs

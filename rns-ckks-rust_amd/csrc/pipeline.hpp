@@ -19,7 +19,11 @@ struct KeyDev {
 
 // exact-convolution keyswitch (ks_exact.hip)
 bool exact_keyswitch_available(const Context& c, uint32_t aux);
+// worst-case digits per group (every digit at 2^29)
 int exact_group_size(const Context& c, int nq);
+// grouping of `levels` digits the keyswitch runs for ciphertexts within the one-group bound: *G groups
+// of *gsize digits (one group when typical digit magnitudes allow it, else the worst-case groups)
+void exact_grouping(const Context& c, int nq, int levels, int* gsize, int* G);
 // keep_data: the key keeps its digit-NTT form as well (both methods available)
 void key_make_exact(Context& c, KeyDev* k, bool keep_data = false);
 void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out);

@@ -4,6 +4,8 @@
 * exact convolution (hybrid keys at N >= 2^12): digits and centred key limbs
   transformed under two auxiliary primes s1, s2, the products summed as exact integers modulo s1 s2
   in groups of digits small enough that no sum can wrap, then recombined and reduced mod each prime.
+  When typical digits allow, that is one group, with the digits' L1 norms checked on the device and
+  a batch whose norms could wrap a sum recomputed with the worst-case groups.
 
 Keys created with the automatic method (the default) hold both forms and every keyswitch takes one of
 them by batch size; both choices give the same bits.
@@ -11,7 +13,8 @@ them by batch size; both choices give the same bits.
 Both are checked against each other and the oracle (keyswitch_rns_hybrid, keyswitching.rs:159-194),
 including inputs built to drive the exact sums to their largest possible magnitude: every digit at
 -2^29 and every key coefficient at (p_j - 1)/2 on the largest primes the context accepts, with the
-group size at its limit (one group) and split in two."""
+group size at its limit (one group), split in two, and beyond the one-group bound inside a batch of
+uniform ciphertexts."""
 import numpy as np
 import pytest
 
@@ -79,7 +82,9 @@ def test_methods_agree_with_each_other_and_the_oracle(logn, limbs, K):
 def test_exact_sums_at_their_largest_magnitude(limbs):
     """N = 2^16, the largest primes below 2^50 + 2^46, every digit of every coefficient at -2^29 (the top
     digit 0) and every key coefficient at (p_j - 1)/2: each exact sum reaches D - 1 = 33 (one group of
-    the largest size the auxiliary primes allow) or 35 (two groups) times N 2^29 (p_j - 1)/2."""
+    the largest size the auxiliary primes allow) or 35 times N 2^29 (p_j - 1)/2.  At 35 the keyswitch
+    plans one group (typical digits fit), the decomposition finds these digits beyond the one-group
+    bound and the batch is recomputed with two groups of 18; a one-group result would be wrong."""
     n, K = 1 << 16, 2
     step, top = 2 * n, (1 << 50) + (1 << 46)
     q, cand = [], (top - 1) // step * step + 1
@@ -95,6 +100,7 @@ def test_exact_sums_at_their_largest_magnitude(limbs):
     eb = O.Basis(n, q[:limbs + K], g[:limbs + K])
     D = ctx.num_decomp_digits(limbs)
     assert D == (34 if limbs == 20 else 36)
+    assert ctx.exact_group_size(limbs) == ((34, 1) if limbs == 20 else (36, 1))
     Q = 1
     for x in q[:limbs]:
         Q *= x
@@ -152,3 +158,63 @@ def test_auto_keys_hold_both_forms_and_take_either_method():
         got = out.numpy()
         for b in range(B):
             assert np.array_equal(got[b], want[b]), (tune, b)
+
+
+def test_one_group_with_a_device_checked_recomputation():
+    """N = 2^16, 24 primes below 2^50, 40 digits: two groups in the worst case, and the exact
+    keyswitch runs one (the digits of uniform ciphertexts leave a margin of about 1.8), recomputing a
+    batch with two groups when some ciphertext's digit L1 norm could carry its sums out of range.
+    Op 2 of a batch of four has a1 = 1 and b1 = X (every digit 2^29 - 1, the top digit 0) and the key
+    every coefficient at (p_j - 1)/2: its one-group sum at coefficient N - 1 is
+    39 (2^29 - 1) N (p_j - 1)/2, beyond (s1 s2 - 1)/2 for any auxiliary primes below 2^50 + 2^46,
+    so only the recomputation gives the crate's result.  Batched, alone, and mixed with uniform
+    ciphertexts, every output equals the digit-NTT method's; op 2 also the oracle's."""
+    p = PR.config_per_op_gpu()
+    n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
+    ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
+    D = ctx.num_decomp_digits(L)
+    assert D == 40 and ctx.exact_group_size(L) == (40, 1)
+    mods = p.primes + p.aux_primes
+    assert 39 * (2 ** 29 - 1) * n * ((min(mods) - 1) // 2) > ((1 << 50) + (1 << 46)) ** 2 // 2
+    key = np.empty((D, 2, L + K, n), dtype=np.uint64)
+    for j, m in enumerate(mods):
+        key[:, :, j, :] = (m - 1) // 2
+    Q = 1
+    for x in p.primes:
+        Q *= x
+    X = (2 ** 29 - 1) * sum(2 ** (30 * i) for i in range(D - 1))
+    assert X < Q // 2
+    qb = O.Basis(n, p.primes, p.roots)
+    eb = O.Basis(n, mods, p.roots + p.aux_roots)
+    rs = np.random.default_rng(41)
+    B, P = 4, L * n
+    lhs = uniform(qb.moduli, n, rs, 2 * B).reshape(B, 2, L, n)
+    rhs = uniform(qb.moduli, n, rs, 2 * B).reshape(B, 2, L, n)
+    lhs[2, 1] = 0
+    lhs[2, 1, :, 0] = 1
+    rhs[2, 1] = np.array([[X % m] * n for m in p.primes], dtype=np.uint64)
+    sd = O.decompose_signed(qb, rhs[2, 1], D)
+    assert (sd[: D - 1] == 2 ** 29 - 1).all() and (sd[D - 1] == 0).all()
+    kc, ke = both_keys(ctx, key, L - 1, K)
+    dl, dr = dev(lhs), dev(rhs)
+    want, quad, one = [], G.DeviceBuffer((3, L, n)), G.DeviceBuffer((2, L, n))
+    for b in range(B):
+        ctx.mult(dl.offset(b * 2 * P), dr.offset(b * 2 * P), L, 40, 40, quad)
+        ctx.relinearize(quad, L, 80, kc, one)
+        want.append(one.numpy())
+        ctx.relinearize(quad, L, 80, ke, one)
+        assert np.array_equal(one.numpy(), want[b]), ("alone", b)
+    for ops in ([0, 1, 3], [0, 1, 2, 3]):
+        out = G.DeviceBuffer((B, 2, L, n))
+        bt = G.Batch(ctx)
+        for b in ops:
+            bt.add(G.OP_MULT_RELIN, L, dl.offset(b * 2 * P), dr.offset(b * 2 * P), ke, out.offset(b * 2 * P))
+        bt.compile(tune=False)
+        for rep in range(2):
+            bt.run()
+            bt.synchronize()
+            got = out.numpy()
+            for b in ops:
+                assert np.array_equal(got[b], want[b]), (ops, rep, b)
+    O.set_threads(16)
+    assert np.array_equal(want[2], O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs[2], rhs[2]), key))
