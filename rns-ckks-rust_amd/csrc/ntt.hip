@@ -172,16 +172,23 @@ __device__ __forceinline__ void row_inv_store(double (&v)[16], double* p, double
 // FOLD: d0 and d1 are stored in NTT form (NTT-domain row order) for a keyswitch that folds them into
 // its accumulators; only d2 gets the inverse row stages.  NT threads per workgroup (NT / (R/16) rows):
 // 128 for small batches, so that the grid still covers every CU.
+// Work items (row block, limb, ciphertext) are numbered ciphertext fastest, and each XCD (the 8 take
+// consecutive workgroups in turn) runs a contiguous range of them: the B ciphertexts of a row block
+// share its twiddles in one L2 instead of every XCD fetching them again for each ciphertext.
 template <int LOGR, bool FOLD, int NT>
 __global__ __launch_bounds__(NT) void k_mult_rows(const double* __restrict__ xl, const double* __restrict__ xr,
                                                   double* __restrict__ d, long long P, int logn, PrimeMap pm,
-                                                  const DevPrime* __restrict__ primes) {
+                                                  const DevPrime* __restrict__ primes, int B, int nrb,
+                                                  int items) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = NT / TPS;
     __shared__ double lds[S * row_pitch<R>()];
+    const int w = blockIdx.x, per = gridDim.x >> 3;
+    const int item = (w & 7) * per + (w >> 3);
+    if (item >= items) return;
+    const int z = item % B, rb = (item / B) % nrb, y = item / B / nrb;
     const int n = 1 << logn, N1 = n >> LOGR;
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
-    const int b = blockIdx.x * S + row;
-    const int y = blockIdx.y, z = blockIdx.z;
+    const int b = rb * S + row;
     const DevPrime Pr = primes[pm(y)];
     const double q = Pr.q, qinv = Pr.qinv;
     const long long off = (long long)y * n + (long long)b * R;
@@ -378,11 +385,16 @@ static void mult_rows_nt(const Context& c, const double* xl, const double* xr, d
                          PrimeMap pm, hipStream_t s, bool fold) {
     constexpr int S = NT / ((1 << LOGR) / 16);
     const int n1 = 1 << (c.logn - LOGR);
-    const dim3 grid(n1 / S, limbs, B);
+    const int nrb = n1 / S;
+    const long long items = (long long)nrb * limbs * B;
+    RCK_REQUIRE(items < (1LL << 30), "tensor launch too large");
+    const dim3 grid((unsigned)((items + 7) / 8 * 8));
     if (fold)
-        k_mult_rows<LOGR, true, NT><<<grid, NT, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
+        k_mult_rows<LOGR, true, NT><<<grid, NT, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes, B,
+                                                        nrb, (int)items);
     else
-        k_mult_rows<LOGR, false, NT><<<grid, NT, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes);
+        k_mult_rows<LOGR, false, NT><<<grid, NT, 0, s>>>(xl, xr, d, (long long)limbs * c.n, c.logn, pm, c.d_primes,
+                                                         B, nrb, (int)items);
 }
 
 template <int LOGR>
