@@ -184,6 +184,142 @@ __device__ __forceinline__ void mac_tile_fit(const double* __restrict__ dn, cons
     mac_tile<BT, JT>(dn, key, acc, n, B, D, cols, b0, c0, s, g, i0, i1, x0, lane, q, qinv);
 }
 
+// The same tile when the four waves of a workgroup take four b-tiles of the same key columns: the
+// workgroup's key tile (JT columns x 64 coefficients of one digit) is copied into LDS once, by
+// global_load_lds (each lane 16 bytes, a wave two columns), and every wave reads it from there.
+// The waves' own vector loads are then only their digits, BT per digit instead of BT + JT: the
+// texture address and data units, which each vector load instruction occupies, bound the kernel.
+// Per digit t: wait for this wave's loads of t, barrier (every wave's copy of tile t has landed and
+// every read of the other buffer is done), start the copy of tile t + 1 and the digit loads of t + 1,
+// then read tile t and multiply.
+constexpr int kWaitVm0Exact = 0x0F70;  // s_waitcnt vmcnt(0), the other counters at their maximum
+
+template <int BT, int JT>
+__device__ __forceinline__ void mac_tile_lds(const double* __restrict__ dn, const double* __restrict__ key,
+                                             double* __restrict__ acc, double* kt, int n, int B, int D, int cols,
+                                             int b0, int c0, int s, int g, int i0, int i1, long long x0, int wave,
+                                             int lane, double q, double qinv) {
+    const long long n2 = 2LL * n;
+    long long dbase[BT];
+#pragma unroll
+    for (int bt = 0; bt < BT; bt++) dbase[bt] = ((long long)min(b0 + bt, B - 1) * D * 2 + s) * n + x0;
+    const long long kstep = (long long)cols * n2;  // key digit i -> i + 1
+    // column pair m of the tile (m = wave, wave + 4, ..): lanes 0-31 copy column c0 + 2m, lanes 32-63
+    // column c0 + 2m + 1 (columns past the key's read its last one; their products are dropped)
+    const int half = lane >> 5, sub = 2 * (lane & 31);
+    auto copy = [&](int t, int buf) {
+#pragma unroll
+        for (int m = wave; m < JT / 2; m += 4) {
+            const int col = min(c0 + 2 * m + half, cols - 1);
+            const double* src = key + (long long)t * kstep + (long long)col * n2 + (long long)s * n + x0 + sub;
+            double* dst = kt + (buf * JT + 2 * m) * 64;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+        }
+    };
+    auto load_d = [&](int t, double (&d)[BT]) {
+#pragma unroll
+        for (int bt = 0; bt < BT; bt++) d[bt] = (dn + dbase[bt] + t * n2)[lane];
+    };
+    double a[BT][JT];
+#pragma unroll
+    for (int bt = 0; bt < BT; bt++)
+#pragma unroll
+        for (int jt = 0; jt < JT; jt++) a[bt][jt] = 0.0;
+    // reduced operands: each product is <= 0.57 s, so eight of them on a reduced accumulator stay
+    // below 5.1 s < 2^53 / s
+    auto fold = [&]() {
+#pragma unroll
+        for (int bt = 0; bt < BT; bt++)
+#pragma unroll
+            for (int jt = 0; jt < JT; jt++) a[bt][jt] = reduce(a[bt][jt], q, qinv);
+    };
+    auto step = [&](int t, const double (&cur)[BT], double (&nxt)[BT], int buf) {
+        __builtin_amdgcn_s_waitcnt(kWaitVm0Exact);
+        __syncthreads();
+        if (t + 1 < i1) {
+            copy(t + 1, buf ^ 1);
+            load_d(t + 1, nxt);
+        }
+        // keeps the loads ahead of the multiplies (the compiler would otherwise sink them to their
+        // first use); neither barrier reads a register, so neither adds a wait
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        double k[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; jt++) k[jt] = kt[(buf * JT + jt) * 64 + lane];
+#pragma unroll
+        for (int bt = 0; bt < BT; bt++)
+#pragma unroll
+            for (int jt = 0; jt < JT; jt++) a[bt][jt] += mulmod(cur[bt], k[jt], q, qinv);
+        if (((t - i0) & 7) == 7) fold();
+    };
+    double dA[BT], dB[BT];
+    copy(i0, 0);
+    load_d(i0, dA);
+    for (int t = i0; t < i1; t += 2) {
+        step(t, dA, dB, 0);
+        if (t + 1 < i1) step(t + 1, dB, dA, 1);
+    }
+    fold();
+#pragma unroll
+    for (int bt = 0; bt < BT; bt++) {
+        if (b0 + bt >= B) continue;
+#pragma unroll
+        for (int jt = 0; jt < JT; jt++) {
+            if (c0 + jt >= cols) continue;
+            acc[((((long long)g * B + b0 + bt) * cols + c0 + jt) * 2 + s) * n + x0 + lane] = a[bt][jt];
+        }
+    }
+}
+
+// the narrowest tile that still covers the last column group (uniform over the workgroup)
+template <int BT, int JT>
+__device__ __forceinline__ void mac_tile_lds_fit(const double* __restrict__ dn, const double* __restrict__ key,
+                                                 double* __restrict__ acc, double* kt, int n, int B, int D, int cols,
+                                                 int b0, int c0, int s, int g, int i0, int i1, long long x0, int wave,
+                                                 int lane, double q, double qinv) {
+    if constexpr (JT > 4) {
+        if (cols - c0 <= JT / 2)
+            return mac_tile_lds_fit<BT, JT / 2>(dn, key, acc, kt, n, B, D, cols, b0, c0, s, g, i0, i1, x0, wave, lane,
+                                                q, qinv);
+    }
+    mac_tile_lds<BT, JT>(dn, key, acc, kt, n, B, D, cols, b0, c0, s, g, i0, i1, x0, wave, lane, q, qinv);
+}
+
+// Four waves along the ciphertexts (WB = 4): work items as in k_ks_mac_exact.  A wave whose b-tile
+// lies past the batch still takes part in every barrier and copy; it stores nothing.
+template <int BT, int JT>
+__global__ __launch_bounds__(256) void k_ks_mac_exact_lds(const double* __restrict__ dn,
+                                                          const double* __restrict__ key, double* __restrict__ acc,
+                                                          int n, int B, int D, int gsize, int cols, int njg, int nxb,
+                                                          int nbg, int items, int sbase,
+                                                          const DevPrime* __restrict__ primes,
+                                                          const int* __restrict__ gate) {
+    __shared__ double kt[2 * JT * 64];
+    if (gate != nullptr && *gate == 0) return;
+    const int w = blockIdx.x;
+    const int per = gridDim.x >> 3;
+    const int item = (w & 7) * per + (w >> 3);
+    if (item >= items) return;
+    int r = item;
+    const int jg = r % njg;
+    r /= njg;
+    const int xb = r % nxb;
+    r /= nxb;
+    const int bg = r % nbg;
+    r /= nbg;
+    const int s = r & 1, g = r >> 1;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int b0 = (bg * 4 + wave) * BT;
+    const int c0 = jg * JT;
+    const DevPrime P = primes[sbase + s];
+    const int i0 = g * gsize, i1 = min(D, i0 + gsize);
+    mac_tile_lds_fit<BT, JT>(dn, key, acc, kt, n, B, D, cols, b0, c0, s, g, i0, i1, (long long)xb * 64, wave, lane,
+                             P.q, P.qinv);
+}
+
 template <int BT, int JT>
 __global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__ dn, const double* __restrict__ key,
                                                       double* __restrict__ acc, int n, int B, int D, int gsize,
@@ -233,8 +369,12 @@ static void mac_exact_t(const Context& c, const double* dn, const double* key, d
     RCK_REQUIRE(items < (1LL << 30), "exact keyswitch launch too large");
     const int per = (int)((items + 7) / 8);
     ProfScope ps(s, gate ? "ks_mac_gated" : "ks_mac");
-    k_ks_mac_exact<BT, JT><<<dim3(per * 8), 256, 0, s>>>(dn, key, acc, n, B, D, gsize, cols, WB, njg, nxb, nbg,
-                                                         (int)items, (int)(c.L + c.K), c.d_primes, gate);
+    if (WB == 4)
+        k_ks_mac_exact_lds<BT, JT><<<dim3(per * 8), 256, 0, s>>>(dn, key, acc, n, B, D, gsize, cols, njg, nxb, nbg,
+                                                                 (int)items, (int)(c.L + c.K), c.d_primes, gate);
+    else
+        k_ks_mac_exact<BT, JT><<<dim3(per * 8), 256, 0, s>>>(dn, key, acc, n, B, D, gsize, cols, WB, njg, nxb, nbg,
+                                                             (int)items, (int)(c.L + c.K), c.d_primes, gate);
     RCK_CHECK_HIP(hipGetLastError());
 }
 

@@ -89,7 +89,7 @@ def test_mac_streamed_batch_bit_identical(logn, limbs):
             assert np.array_equal(got[b], want[b]), (ds, pf, b)
 
 
-@pytest.mark.parametrize("logn,limbs,B", [(12, 4, 1), (13, 6, 5), (16, 5, 3)])
+@pytest.mark.parametrize("logn,limbs,B", [(12, 4, 1), (13, 6, 5), (16, 5, 3), (14, 5, 9)])
 def test_exact_mac_tiles_bit_identical(logn, limbs, B):
     """Every multiply-accumulate tile of the exact-convolution keyswitch (ciphertexts 1 / 2 / 4 by key
     columns 4 / 8 / 16 per thread), forced through the tuning table, gives the digit-NTT result."""
