@@ -226,9 +226,7 @@ int rck_ntt_forward(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, 
         PrimeMap pm{(int)nq, (int)c.L, 0};
         if (c.wide) return wide_ntt_cyclic(c, false, data, limbs, pm, S(stream));
         double* t = static_cast<double*>(c.workspace(8, (size_t)limbs * n * 8));
-        launch_twist_in(c, {data, n, 0}, {t, n, 0}, limbs, pm, S(stream));
-        launch_fwd(c, kInF64, t, n, 0, t, n, 0, limbs, 1, pm, S(stream));
-        launch_brv_out(c, {t, n, 0}, {data, n, 0}, limbs, pm, S(stream));
+        launch_ntt_cyclic(c, false, data, t, limbs, pm, S(stream));
     });
 }
 
@@ -241,11 +239,8 @@ int rck_ntt_inverse(rck_context* ctx, uint64_t* data, uint32_t nq, uint32_t np, 
         const long long n = c.n;
         PrimeMap pm{(int)nq, (int)c.L, 0};
         if (c.wide) return wide_ntt_cyclic(c, true, data, limbs, pm, S(stream));
-        double* t = static_cast<double*>(c.workspace(8, (size_t)2 * limbs * n * 8));
-        double* t2 = t + (size_t)limbs * n;
-        launch_brv_in(c, {data, n, 0}, {t, n, 0}, limbs, pm, S(stream));
-        launch_inv(c, t, n, 0, kOutF64, t2, n, 0, limbs, 1, pm, S(stream));
-        launch_twist_out(c, {t2, n, 0}, {data, n, 0}, limbs, pm, S(stream));
+        double* t = static_cast<double*>(c.workspace(8, (size_t)limbs * n * 8));
+        launch_ntt_cyclic(c, true, data, t, limbs, pm, S(stream));
     });
 }
 

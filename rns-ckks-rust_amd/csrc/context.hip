@@ -298,26 +298,31 @@ Context* context_create(uint32_t n, const uint64_t* q, const uint64_t* g, uint32
             std::string err = plan_check(n, all[i], roots[i], &c->hp[i]);
             RCK_REQUIRE(err.empty(), "invalid RNS prime " + std::to_string(all[i]) + ": " + err);
         }
-        // twiddle tables: per prime fw, iv (bit-reversed) and tw, twi (natural order)
+        // twiddle tables per prime: fw, iv (merged negacyclic) and fwc, ivc (cyclic), all in the
+        // Cooley-Tukey group order k = 2^s + g.  fw[k] = psi^{brv_logN(k)} with
+        // brv_logN(k) = (2 brv_s(g) + 1) N / 2^{s+1}; the cyclic table drops the odd part:
+        // fwc[k] = psi^{2 brv_s(g) N / 2^{s+1}} = psi^{brv_logN(k) - N / 2^{s+1}}.  The same merged
+        // loops then compute the cyclic DFT in bit-reversed order (and its inverse from that order).
         std::vector<double> tables((size_t)TS * 4 * n);
         for (uint32_t i = 0; i < TS; i++) {
             const HostPrime& h = c->hp[i];
             double* fw = &tables[(size_t)i * 4 * n];
-            double *iv = fw + n, *tw = iv + n, *twi = tw + n;
-            std::vector<uint64_t> pw(n), pwi(n);
-            uint64_t a = 1, b = 1;
-            for (uint32_t k = 0; k < n; k++) {
+            double *iv = fw + n, *fwc = iv + n, *ivc = fwc + n;
+            std::vector<uint64_t> pw(2 * n);  // psi^e for e < 2N (psi^{2N} = 1)
+            uint64_t a = 1;
+            for (uint32_t k = 0; k < 2 * n; k++) {
                 pw[k] = a;
-                pwi[k] = b;
                 a = h_mulmod(a, h.psi, h.q);
-                b = h_mulmod(b, h.psi_inv, h.q);
             }
             for (uint32_t k = 0; k < n; k++) {
-                uint32_t r = bitrev(k, c->logn);
+                const uint32_t r = bitrev(k, c->logn);
                 fw[k] = h_centred(pw[r], h.q);
-                iv[k] = h_centred(pwi[r], h.q);
-                tw[k] = h_centred(pwi[k], h.q);
-                twi[k] = h_centred(pw[k], h.q);
+                iv[k] = h_centred(pw[(2 * n - r) % (2 * n)], h.q);
+                uint32_t s = 0;
+                while (k >> (s + 1)) s++;
+                const uint32_t e = k == 0 ? 0 : r - (n >> (s + 1));
+                fwc[k] = h_centred(pw[e], h.q);
+                ivc[k] = h_centred(pw[(2 * n - e) % (2 * n)], h.q);
             }
         }
         c->d_tables = upload(tables);

@@ -22,8 +22,8 @@ struct DevPrime {
     uint64_t qu;       // q as an integer (wave-uniform integer arithmetic)
     const double* fw;  // forward twiddles: fw[k] = psi^{bitrev_logN(k)}, centred, k < N
     const double* iv;  // inverse twiddles: iv[k] = psi^{-bitrev_logN(k)}, centred
-    const double* tw;  // natural-order psi^{-j} (cyclic-mode pre-twist), centred
-    const double* twi; // natural-order psi^{j}  (cyclic-mode post-twist), centred
+    const double* fwc; // cyclic forward twiddles: fwc[2^s + g] = omega^{bitrev_s(g) N / 2^{s+1}}, centred
+    const double* ivc; // cyclic inverse twiddles: ivc[k] = fwc[k]^{-1}, centred
 };
 
 // Decomposition / basis-conversion constants (keyswitching.rs:196-322).

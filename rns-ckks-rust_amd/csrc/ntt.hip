@@ -18,7 +18,7 @@ __device__ __forceinline__ double load_in(const void* p, long long idx, double q
 // Column pass, forward: stages 0..LOGR-1 on the columns x = r + N2 c.
 // Thread (tau, col): tid = tau * S + col, so consecutive lanes touch consecutive columns.
 // ---------------------------------------------------------------------------------------
-template <int LOGR, int IN>
+template <int LOGR, int IN, bool CYC = false>
 __global__ __launch_bounds__(256) void k_fwd_cols(Batch<const void> in, int in_elem, Batch<double> out, int logn,
                                                   PrimeMap pm, const DevPrime* __restrict__ primes,
                                                   const void* in2, int zsplit) {
@@ -37,13 +37,14 @@ __global__ __launch_bounds__(256) void k_fwd_cols(Batch<const void> in, int in_e
     double v[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = load_in<IN>(src, r + (long long)N2 * (tau + TPS * e), q);
-    fwd_phase_a<kTwScalar>(v, P.fw, 1, q, qinv);
+    const double* fw = CYC ? P.fwc : P.fw;
+    fwd_phase_a<kTwScalar>(v, fw, 1, q, qinv);
 #pragma unroll
     for (int e = 0; e < 16; e++) lds[(tau + TPS * e) * S + col] = v[e];
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[(16 * tau + e) * S + col];
-    fwd_phase_b<LOGR>(v, P.fw, 1, tau, q, qinv);
+    fwd_phase_b<LOGR>(v, fw, 1, tau, q, qinv);
     double* dst = out.at(z, y);
 #pragma unroll
     for (int e = 0; e < 16; e++) dst[r + (long long)N2 * (16 * tau + e)] = v[e];
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(NT) void k_mult_rows(const double* __restrict__ xl,
 }
 
 // Column pass, inverse, with the N^{-1} scaling; writes canonical u64 or reduced doubles.
-template <int LOGR, int OUT>
+template <int LOGR, int OUT, bool CYC = false>
 __global__ __launch_bounds__(256) void k_inv_cols(Batch<const double> in, Batch<void> out, int out_elem, int logn,
                                                   PrimeMap pm, const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
@@ -236,13 +237,14 @@ __global__ __launch_bounds__(256) void k_inv_cols(Batch<const double> in, Batch<
     double v[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = src[r + (long long)N2 * (16 * tau + e)];
-    inv_phase_b<LOGR>(v, P.iv, 1, tau, q, qinv);
+    const double* iv = CYC ? P.ivc : P.iv;
+    inv_phase_b<LOGR>(v, iv, 1, tau, q, qinv);
 #pragma unroll
     for (int e = 0; e < 16; e++) lds[(16 * tau + e) * S + col] = v[e];
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
-    inv_phase_a<kTwScalar>(v, P.iv, 1, q, qinv);
+    inv_phase_a<kTwScalar>(v, iv, 1, q, qinv);
     char* dst = static_cast<char*>(out.base) + ((long long)z * out.ps + (long long)y * out.ls) * out_elem;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
@@ -253,6 +255,132 @@ __global__ __launch_bounds__(256) void k_inv_cols(Batch<const double> in, Batch<
         } else {
             reinterpret_cast<double*>(dst)[idx] = reduce(x, q, qinv);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Cyclic NTT, natural order in and out (NttPlan::forward / inverse, src/math/ntt.rs:78-89).
+// The column and row passes above run unchanged with the cyclic tables fwc / ivc (context.hip),
+// which makes slot i of the row-major result hold X_{brv_logN(i)}.  For slot i = b N2 + x that is
+// X_k with k = brv(x) N1 + brv(b) (brv over log2 N2 and log2 N1 bits).  A workgroup therefore takes
+// the S rows b = brv(beta), beta = beta0 .. beta0 + S - 1, so that for every x its S outputs are the
+// consecutive k = brv(x) N1 + beta: after the row stages the tile goes through LDS once more and
+// each store instruction writes S-element runs.  The inverse row pass gathers the same way.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ int brv_bits(int x, int bits) { return (int)(__brev((unsigned)x) >> (32 - bits)); }
+
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_cyc_fwd_rows(const double* __restrict__ in, uint64_t* __restrict__ out,
+                                                      int logn, PrimeMap pm, const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[S * row_pitch<R>()];
+    const int n1b = logn - LOGR, N1 = 1 << n1b;
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int beta0 = blockIdx.x * S, b = brv_bits(beta0 + row, n1b);
+    const int y = blockIdx.y;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    const long long limb = (long long)y << logn;
+    const double* p = in + limb + (long long)b * R;
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
+    fwd_phase_a(v, P.fwc, N1 + b, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, tau + TPS * e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
+    fwd_phase_b<LOGR>(v, P.fwc, N1 + b, tau, q, qinv);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, 16 * tau + e)] = v[e];
+    __syncthreads();
+    // output run: lane -> (row offset bo = tid % S, m = tid / S + TPS e), k = m N1 + beta0 + bo
+    const int bo = tid % S, m0 = tid / S;
+    uint64_t* dst = out + limb + beta0 + bo;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int m = m0 + TPS * e;
+        dst[(long long)m * N1] = f64_to_u64(canon(lds[row_lds<R>(bo, brv_bits(m, LOGR))], q, qinv));
+    }
+}
+
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_cyc_inv_rows(const uint64_t* __restrict__ in, double* __restrict__ out,
+                                                      int logn, PrimeMap pm, const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[S * row_pitch<R>()];
+    const int n1b = logn - LOGR, N1 = 1 << n1b;
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int beta0 = blockIdx.x * S, b = brv_bits(beta0 + row, n1b);
+    const int y = blockIdx.y;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    const long long limb = (long long)y << logn;
+    const int bo = tid % S, m0 = tid / S;
+    const uint64_t* src = in + limb + beta0 + bo;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int m = m0 + TPS * e;
+        lds[row_lds<R>(bo, brv_bits(m, LOGR))] = load_centred(src[(long long)m * N1], q);
+    }
+    __syncthreads();
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
+    inv_phase_b<LOGR>(v, P.ivc, N1 + b, tau, q, qinv);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, 16 * tau + e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, tau + TPS * e)];
+    inv_phase_a(v, P.ivc, N1 + b, q, qinv);
+    double* p = out + limb + (long long)b * R;
+#pragma unroll
+    for (int e = 0; e < 16; e++) p[tau + TPS * e] = v[e];
+}
+
+// Small rings: the whole limb in LDS, in place, the bit reversal folded into the load (inverse) or the
+// store (forward).
+template <bool INV>
+__global__ __launch_bounds__(256) void k_small_cyc(uint64_t* data, int logn, PrimeMap pm,
+                                                   const DevPrime* __restrict__ primes) {
+    __shared__ double a[2048];
+    const int n = 1 << logn, y = blockIdx.y;
+    const DevPrime P = primes[pm(y)];
+    const double q = P.q, qinv = P.qinv;
+    uint64_t* d = data + ((long long)y << logn);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[INV ? brv_bits(i, logn) : i] = load_centred(d[i], q);
+    __syncthreads();
+    if constexpr (!INV) {
+        for (int s = 0; s < logn; s++) {
+            const int t = n >> (s + 1);
+            for (int k = threadIdx.x; k < n / 2; k += blockDim.x) {
+                const int g = k / t, j = k % t, x = 2 * g * t + j;
+                double u = a[x], v = a[x + t];
+                ct_bfly(u, v, P.fwc[(1 << s) + g], q, qinv);
+                a[x] = reduce(u, q, qinv);
+                a[x + t] = reduce(v, q, qinv);
+            }
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = f64_to_u64(canon(a[brv_bits(i, logn)], q, qinv));
+    } else {
+        for (int s = logn - 1; s >= 0; s--) {
+            const int t = n >> (s + 1);
+            for (int k = threadIdx.x; k < n / 2; k += blockDim.x) {
+                const int g = k / t, j = k % t, x = 2 * g * t + j;
+                double u = a[x], v = a[x + t];
+                gs_bfly(u, v, P.ivc[(1 << s) + g], q, qinv);
+                a[x] = reduce(u, q, qinv);
+                a[x + t] = reduce(v, q, qinv);
+            }
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            d[i] = f64_to_u64(canon(mulmod(a[i], P.ninv, q, qinv), q, qinv));
     }
 }
 
@@ -536,6 +664,58 @@ void launch_inv(const Context& c, double* in, long long in_ls, long long in_ps, 
     }
     launch_inv_rows(c, in, in_ls, in_ps, limbs, polys, pm, s, nsum, sstride);
     launch_inv_cols(c, in, in_ls, in_ps, out_kind, out, out_ls, out_ps, limbs, polys, pm, s);
+}
+
+template <int N1B, int N2B>
+static void cyclic_t(const Context& c, bool inverse, uint64_t* data, double* t, int limbs, PrimeMap pm,
+                     hipStream_t s) {
+    constexpr int Sc = 256 / ((1 << N1B) / 16), Sr = 256 / ((1 << N2B) / 16);
+    const dim3 gc((1 << N2B) / Sc, limbs, 1), gr((1 << N1B) / Sr, limbs, 1);
+    Batch<const void> bi{data, c.n, 0};
+    Batch<double> bt{t, c.n, 0};
+    if (!inverse) {
+        {
+            ProfScope ps(s, "ntt_cyc_fwd_cols");
+            k_fwd_cols<N1B, kInU64, true><<<gc, 256, 0, s>>>(bi, 8, bt, c.logn, pm, c.d_primes, nullptr, 0);
+            RCK_LAUNCH_CHECK();
+        }
+        ProfScope ps(s, "ntt_cyc_fwd_rows");
+        k_cyc_fwd_rows<N2B><<<gr, 256, 0, s>>>(t, data, c.logn, pm, c.d_primes);
+        RCK_LAUNCH_CHECK();
+    } else {
+        {
+            ProfScope ps(s, "ntt_cyc_inv_rows");
+            k_cyc_inv_rows<N2B><<<gr, 256, 0, s>>>(data, t, c.logn, pm, c.d_primes);
+            RCK_LAUNCH_CHECK();
+        }
+        ProfScope ps(s, "ntt_cyc_inv_cols");
+        k_inv_cols<N1B, kOutU64, true><<<gc, 256, 0, s>>>({t, c.n, 0}, {data, c.n, 0}, 8, c.logn, pm, c.d_primes);
+        RCK_LAUNCH_CHECK();
+    }
+}
+
+void launch_ntt_cyclic(const Context& c, bool inverse, uint64_t* data, double* t, int limbs, PrimeMap pm,
+                       hipStream_t s) {
+    require_fp64(c);
+    check_dims(c, limbs, 1);
+    if (small_ring(c)) {
+        ProfScope ps(s, inverse ? "ntt_small_cyc_inv" : "ntt_small_cyc_fwd");
+        if (inverse)
+            k_small_cyc<true><<<dim3(1, limbs, 1), 256, 0, s>>>(data, c.logn, pm, c.d_primes);
+        else
+            k_small_cyc<false><<<dim3(1, limbs, 1), 256, 0, s>>>(data, c.logn, pm, c.d_primes);
+        RCK_LAUNCH_CHECK();
+        return;
+    }
+    RCK_REQUIRE(t != nullptr, "cyclic NTT needs a workspace");
+    switch (c.logn) {
+        case 12: cyclic_t<6, 6>(c, inverse, data, t, limbs, pm, s); break;
+        case 13: cyclic_t<6, 7>(c, inverse, data, t, limbs, pm, s); break;
+        case 14: cyclic_t<7, 7>(c, inverse, data, t, limbs, pm, s); break;
+        case 15: cyclic_t<7, 8>(c, inverse, data, t, limbs, pm, s); break;
+        case 16: cyclic_t<8, 8>(c, inverse, data, t, limbs, pm, s); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
 }
 
 }  // namespace rck

@@ -248,4 +248,9 @@ void launch_mult_rows(const Context& c, const double* xl, const double* xr, doub
 void launch_inv_cols(const Context& c, const double* in, long long in_ls, long long in_ps, int out_kind, void* out,
                      long long out_ls, long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s);
 
+// The crate's cyclic NTT (natural order in and out) of `limbs` canonical u64 limbs, in place; t: a
+// workspace of limbs * N doubles (unused for N <= 2^11).
+void launch_ntt_cyclic(const Context& c, bool inverse, uint64_t* data, double* t, int limbs, PrimeMap pm,
+                       hipStream_t s);
+
 }  // namespace rck

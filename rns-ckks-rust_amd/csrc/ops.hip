@@ -1,6 +1,6 @@
 // Elementwise kernels: coefficient-domain add/sub/automorphism/rescale/multiply-add on
-// canonical u64 limbs, NTT-domain pointwise products on doubles, and the cyclic-mode
-// twist / bit-reversal permutations.  All are HBM-bound streaming kernels: one thread per
+// canonical u64 limbs and NTT-domain pointwise products on doubles.  All are HBM-bound
+// streaming kernels: one thread per
 // coefficient, grid.x over coefficients, grid.y over limbs, grid.z over polynomials.
 #include "ops.hpp"
 #include "wide.hpp"
@@ -99,45 +99,6 @@ __global__ void k_tensor(const double* __restrict__ x, double* __restrict__ d, l
     d[2 * poly + o] = reduce(mulmod(a1, b1, P.q, P.qinv), P.q, P.qinv);
 }
 
-// cyclic-mode helpers (NttPlan::forward/inverse parity, src/math/ntt.rs:78-89)
-__global__ void k_twist_in(Batch<const uint64_t> in, Batch<double> out, int n, PrimeMap pm,
-                           const DevPrime* __restrict__ primes) {
-    const long long j = gidx();
-    if (j >= n) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const DevPrime P = primes[pm(y)];
-    out.at(z, y)[j] = reduce(mulmod(load_centred(in.at(z, y)[j], P.q), P.tw[j], P.q, P.qinv), P.q, P.qinv);
-}
-
-__global__ void k_brv_out(Batch<const double> in, Batch<uint64_t> out, int n, int logn, PrimeMap pm,
-                          const DevPrime* __restrict__ primes) {
-    const long long k = gidx();
-    if (k >= n) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const DevPrime P = primes[pm(y)];
-    const unsigned r = __brev((unsigned)k) >> (32 - logn);
-    out.at(z, y)[k] = f64_to_u64(canon(in.at(z, y)[ntt_phys(r, logn)], P.q, P.qinv));
-}
-
-__global__ void k_brv_in(Batch<const uint64_t> in, Batch<double> out, int n, int logn, PrimeMap pm,
-                         const DevPrime* __restrict__ primes) {
-    const long long k = gidx();
-    if (k >= n) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const DevPrime P = primes[pm(y)];
-    const unsigned r = __brev((unsigned)k) >> (32 - logn);
-    out.at(z, y)[ntt_phys(r, logn)] = load_centred(in.at(z, y)[k], P.q);
-}
-
-__global__ void k_twist_out(Batch<const double> in, Batch<uint64_t> out, int n, PrimeMap pm,
-                            const DevPrime* __restrict__ primes) {
-    const long long j = gidx();
-    if (j >= n) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const DevPrime P = primes[pm(y)];
-    out.at(z, y)[j] = f64_to_u64(canon(mulmod(in.at(z, y)[j], P.twi[j], P.q, P.qinv), P.q, P.qinv));
-}
-
 // plain streaming copy (diagnostics: calibrates hardware byte counters against a known volume)
 __global__ void k_copy(const double4* __restrict__ src, double4* __restrict__ dst, long long n4) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
@@ -211,38 +172,6 @@ void launch_tensor(const Context& c, const double* x, double* d, int limbs, Prim
     require_fp64(c);
     ProfScope ps(s, "tensor");
     k_tensor<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(x, d, (long long)limbs * c.n, c.n, pm, c.d_primes);
-    CHK();
-}
-
-void launch_twist_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
-                     hipStream_t s) {
-    require_fp64(c);
-    ProfScope ps(s, "twist_in");
-    k_twist_in<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, pm, c.d_primes);
-    CHK();
-}
-
-void launch_brv_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
-                    hipStream_t s) {
-    require_fp64(c);
-    ProfScope ps(s, "brv_out");
-    k_brv_out<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, c.logn, pm, c.d_primes);
-    CHK();
-}
-
-void launch_brv_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
-                   hipStream_t s) {
-    require_fp64(c);
-    ProfScope ps(s, "brv_in");
-    k_brv_in<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, c.logn, pm, c.d_primes);
-    CHK();
-}
-
-void launch_twist_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
-                      hipStream_t s) {
-    require_fp64(c);
-    ProfScope ps(s, "twist_out");
-    k_twist_out<<<grid_for(c.n, limbs, 1), 256, 0, s>>>(in, out, c.n, pm, c.d_primes);
     CHK();
 }
 

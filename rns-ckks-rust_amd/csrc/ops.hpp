@@ -19,15 +19,6 @@ void launch_mul_add_u64(const Context& c, Batch<const uint64_t> a, Batch<const u
 void launch_mul_f64(const Context& c, Batch<const double> a, Batch<const double> b, Batch<double> out, int limbs,
                     int polys, PrimeMap pm, hipStream_t s);
 void launch_tensor(const Context& c, const double* x, double* d, int limbs, PrimeMap pm, hipStream_t s);
-void launch_twist_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
-                     hipStream_t s);
-void launch_brv_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
-                    hipStream_t s);
-void launch_brv_in(const Context& c, Batch<const uint64_t> in, Batch<double> out, int limbs, PrimeMap pm,
-                   hipStream_t s);
-void launch_twist_out(const Context& c, Batch<const double> in, Batch<uint64_t> out, int limbs, PrimeMap pm,
-                      hipStream_t s);
-
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // keyswitch pipeline
