@@ -192,10 +192,10 @@ static void autotune(rck_batch* bt, BatchGroup& g) {
     const int E = (int)g.limbs + (int)g.key->aux;
     const int D = g.key->aux ? (int)((c.qbits[g.limbs] + 29) / 30) : (int)g.key->digits;
     const int B = (int)g.ops.size();
-    // the key's methods: digit NTT (data), exact convolution (sdata), or both
+    // the key's methods: digit NTT (data), exact convolution (mdata), or both
     std::vector<bool> methods;
     if (g.key->data) methods.push_back(false);
-    if (g.key->sdata) methods.push_back(true);
+    if (g.key->mdata) methods.push_back(true);
     {
         // every method already tuned for this shape: take the fastest entry
         bool all = true;
@@ -277,7 +277,7 @@ static void tune_keyswitch(Context& c, int limbs, const KeyDev* key, int B, KsSh
         KsShape bs{};
         std::vector<bool> methods;
         if (key->data) methods.push_back(false);
-        if (key->sdata) methods.push_back(true);
+        if (key->mdata) methods.push_back(true);
         std::vector<KsShape> cands;
         for (bool ex : methods)
             for (const KsShape& sh : ks_candidates(c, E, D, B, ex)) cands.push_back(sh);

@@ -186,7 +186,7 @@ int rck_ksk_method(const rck_ksk* key, int* method) {
     return guard([&] {
         RCK_REQUIRE(key != nullptr && method != nullptr, "null argument");
         // 64-bit integer keys: digit NTT
-        *method = key->sdata ? (key->data ? RCK_KS_AUTO : RCK_KS_EXACT_CONVOLUTION) : RCK_KS_DIGIT_NTT;
+        *method = key->mdata ? (key->data ? RCK_KS_AUTO : RCK_KS_EXACT_CONVOLUTION) : RCK_KS_DIGIT_NTT;
     });
 }
 
@@ -420,11 +420,13 @@ static void ksk_create(rck_context* ctx, const uint64_t* key, bool on_device, ui
         }
         RCK_CHECK_HIP(hipDeviceSynchronize());
         if (stage) RCK_CHECK_HIP(hipFree(stage));
-        if (c.ks_method != RCK_KS_DIGIT_NTT && exact_keyswitch_available(c, aux))
+        // automatic keys take the exact-convolution form where it applies (at most 64 digits)
+        if (c.ks_method != RCK_KS_DIGIT_NTT && exact_keyswitch_available(c, aux) &&
+            (c.ks_method != RCK_KS_AUTO || digits <= 64))
             key_make_exact(c, k, c.ks_method == RCK_KS_AUTO);
     } catch (...) {
         if (k->data) (void)hipFree(k->data);
-        if (k->sdata) (void)hipFree(k->sdata);
+        if (k->mdata) (void)hipFree(k->mdata);
         if (k->wdata) (void)hipFree(k->wdata);
         delete k;
         throw;
@@ -447,7 +449,7 @@ int rck_ksk_destroy(rck_ksk* key) {
         if (!key) return;
         RCK_CHECK_HIP(hipDeviceSynchronize());
         if (key->data) RCK_CHECK_HIP(hipFree(key->data));
-        if (key->sdata) RCK_CHECK_HIP(hipFree(key->sdata));
+        if (key->mdata) RCK_CHECK_HIP(hipFree(key->mdata));
         if (key->wdata) RCK_CHECK_HIP(hipFree(key->wdata));
         delete key;
     });

@@ -317,13 +317,14 @@ rck_ksk* keygen_switch(rck_context* ctx, rck_rng* r, double sigma, const uint64_
             wide_key_create(c, k, coef);
             RCK_CHECK_HIP(hipFree(coef));
             coef = nullptr;
-        } else if (c.ks_method != RCK_KS_DIGIT_NTT && exact_keyswitch_available(c, (uint32_t)K)) {
+        } else if (c.ks_method != RCK_KS_DIGIT_NTT && exact_keyswitch_available(c, (uint32_t)K) &&
+                   (c.ks_method != RCK_KS_AUTO || k->digits <= 64)) {
             key_make_exact(c, k, c.ks_method == RCK_KS_AUTO);
         }
     } catch (...) {
         if (coef) (void)hipFree(coef);
         if (k->data) (void)hipFree(k->data);
-        if (k->sdata) (void)hipFree(k->sdata);
+        if (k->mdata) (void)hipFree(k->mdata);
         if (k->wdata) (void)hipFree(k->wdata);
         delete k;
         throw;
@@ -510,7 +511,7 @@ int rck_keygen_rotation(rck_context* ctx, rck_rng* rng, double sigma, const uint
 int rck_ksk_export(rck_context* ctx, const rck_ksk* key, uint64_t* out) {
     return kguard([&] {
         Context& c = ctx_of(ctx);
-        RCK_REQUIRE(key != nullptr && (key->data != nullptr || key->sdata != nullptr || key->wdata != nullptr),
+        RCK_REQUIRE(key != nullptr && (key->data != nullptr || key->mdata != nullptr || key->wdata != nullptr),
                     "null key");
         check_ptr(out, "out");
         if (key->wdata) return wide_key_export(c, key, out);

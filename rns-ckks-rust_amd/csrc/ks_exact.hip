@@ -12,9 +12,9 @@
 // crate's per-prime products give, bit for bit.
 //
 // So each digit is transformed under s1 and s2 only (2 limb NTTs instead of one per eval prime), the
-// key is stored as the s1/s2 NTT forms of its centred limbs ([digit][comp][limb][s][N]), the products
-// are a per-coefficient multiply-accumulate over the digits of a group (every digit value reused for
-// all 2 (l + K) key columns, every key value for all ciphertexts of a batch), and the accumulators are
+// key is stored once as the s1/s2 NTT forms of its centred limbs, the products are, at every NTT slot,
+// a (ciphertexts x digits) by (digits x 2 (l + K) key columns) matrix product mod s computed exactly on
+// the int8 matrix cores (every key value serves all ciphertexts of a batch), and the accumulators are
 // transformed back under s1, s2, recombined, reduced mod p_j and taken through mod-down by P.
 #include <algorithm>
 
@@ -88,318 +88,372 @@ __global__ void k_key_from_special(const uint64_t* __restrict__ in, uint64_t* __
 }
 
 // ---------------------------------------------------------------------------------------
-// Multiply-accumulate: for coefficient x, auxiliary prime s and digit group g,
-//   acc[g][b][jc][s][x] = sum_{i in g} dn[b][i][s][x] * key[i][jc][s][x]  mod s,  jc = comp * E + j.
-// A workgroup is four waves over 64 consecutive coefficients; wave w takes BT ciphertexts of its
-// b-tile and JT key columns of its j-tile (WB waves along the ciphertexts, 4 / WB along the
-// columns), so the waves of a workgroup read the same key values (served by the CU's caches) or the
-// same digit values.  Work items are numbered key-column group fastest, then coefficient block,
-// ciphertext group, prime and digit group, and each XCD takes a contiguous range of them (workgroup
-// w runs on XCD w % 8), so consecutive items on an XCD share their digit tiles in its L2.
+// Multiply-accumulate on the int8 matrix cores.
+//
+// At NTT slot x under auxiliary prime s the accumulators of a batch are one matrix product mod s:
+//   acc[b][jc] = sum_i dn[b][i] * key[i][jc]   (B ciphertexts x D digits, by D digits x 2E key columns),
+// every operand a residue mod s < 2^50.1, centred.  Each operand is written in balanced signed bytes,
+// x = sum_{u<7} x_u 256^u with x_u in [-128, 128) and byte 7 zero.  For output byte position
+// p = u + v (0 .. 12) the integer sums
+//   C_p[b][jc] = sum_i sum_{u+v=p} dn_u[b][i] key_v[i][jc]      (|C_p| <= 8 D 2^14)
+// are products of int8 matrices, and acc = sum_p C_p 256^p exactly.  One v_mfma_i32_16x16x64_i8 takes
+// 16 ciphertexts by 16 key columns over 8 digits x 8 byte pairs: lane l holds row (A) or column (B)
+// l % 16 and the K-items 16 (l / 16) + t, t < 16, here two digits x 8 bytes; output row 4 (l / 16) + r,
+// column l % 16 in register r.  The key's bytes are stored reversed (item t of a digit = key byte
+// 7 - t) and the digit's operand for position p is its word shifted by p - 7 bytes (item t = digit
+// byte p - 7 + t, zero outside 0 .. 6), so item t pairs digit byte u = p - 7 + t with key byte
+// v = 7 - t: u + v = p.  The 13 sums are recombined in FP64: H_k = sum_{t<4} C_{4k+t} 2^{8t}
+// (|H| < 2^47.1, exact), V = ((H_3 c + H_2) c + H_1) c + H_0 mod s with c = 2^32 mod s (three exact
+// modular products), then reduced: the value the crate's per-prime products sum to, mod s.
 // ---------------------------------------------------------------------------------------
-// One wave's tile: ciphertexts b0 .. b0+BT-1 by key columns c0 .. c0+JT-1 over digits i0 .. i1-1 of
-// auxiliary prime s, for the 64 coefficients from x0.
-template <int BT, int JT>
-__device__ __forceinline__ void mac_tile(const double* __restrict__ dn, const double* __restrict__ key,
-                                         double* __restrict__ acc, int n, int B, int D, int cols, int b0, int c0,
-                                         int s, int g, int i0, int i1, long long x0, int lane, double q, double qinv) {
-    const long long n2 = 2LL * n;
-    // rows past the batch or past the key columns read the last valid one (their products are dropped);
-    // every load is one of these wave-uniform offsets from a kernel argument plus the lane
-    long long dbase[BT], kbase[JT];
-#pragma unroll
-    for (int bt = 0; bt < BT; bt++) dbase[bt] = ((long long)min(b0 + bt, B - 1) * D * 2 + s) * n + x0;
-#pragma unroll
-    for (int jt = 0; jt < JT; jt++) kbase[jt] = (long long)min(c0 + jt, cols - 1) * n2 + s * n + x0;
-    const long long kstep = (long long)cols * n2;  // key digit i -> i + 1
-    double a[BT][JT];
-#pragma unroll
-    for (int bt = 0; bt < BT; bt++)
-#pragma unroll
-        for (int jt = 0; jt < JT; jt++) a[bt][jt] = 0.0;
-    auto load = [&](int t, double (&d)[BT], double (&k)[JT]) {
-#pragma unroll
-        for (int bt = 0; bt < BT; bt++) d[bt] = (dn + dbase[bt] + t * n2)[lane];
-#pragma unroll
-        for (int jt = 0; jt < JT; jt++) k[jt] = (key + kbase[jt] + t * kstep)[lane];
-        // keeps the loads ahead of the multiplies that follow (the compiler would otherwise sink them to
-        // their first use); neither barrier reads a register, so neither adds a wait
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_sched_barrier(0);
-    };
-    auto mac = [&](const double (&d)[BT], const double (&k)[JT]) {
-#pragma unroll
-        for (int bt = 0; bt < BT; bt++)
-#pragma unroll
-            for (int jt = 0; jt < JT; jt++) a[bt][jt] += mulmod(d[bt], k[jt], q, qinv);
-    };
-    // reduced operands: each product is <= 0.57 s, so eight of them on a reduced accumulator stay
-    // below 5.1 s < 2^53 / s
-    auto fold = [&]() {
-#pragma unroll
-        for (int bt = 0; bt < BT; bt++)
-#pragma unroll
-            for (int jt = 0; jt < JT; jt++) a[bt][jt] = reduce(a[bt][jt], q, qinv);
-    };
-    // two register buffers, unrolled by two: the digit after the one being multiplied is always in
-    // flight (its index clamped to the group, so every pass issues the same loads and the only wait
-    // is on loads issued one digit earlier)
-    double dA[BT], kA[JT], dB[BT], kB[JT];
-    load(i0, dA, kA);
-    for (int t = i0; t < i1; t += 2) {
-        load(min(t + 1, i1 - 1), dB, kB);
-        mac(dA, kA);
-        load(min(t + 2, i1 - 1), dA, kA);
-        // an odd group's last pass has no second digit: its key operands become 0, and every product
-        // with 0 is exactly 0
-        const bool none = t + 1 == i1;
-#pragma unroll
-        for (int jt = 0; jt < JT; jt++) kB[jt] = none ? 0.0 : kB[jt];
-        mac(dB, kB);
-        if (((t - i0) & 7) == 6) fold();
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned long long v2u __attribute__((ext_vector_type(2)));
+
+constexpr uint64_t kByteBias = 0x0080808080808080ull;  // 128 in bytes 0 .. 6
+
+// integer-valued double |v| < 2^51 -> int64: v + 1.5 2^52 has the bit pattern 0x4338000000000000 + v
+__device__ __forceinline__ int64_t f64_to_i64(double v) {
+    return (int64_t)__double_as_longlong(v + kShift) - 0x4338000000000000ll;
+}
+// balanced signed-byte word of |x| < 2^55 (byte u = x_u as two's complement, byte 7 zero) and back
+__device__ __forceinline__ uint64_t sbyte_word(int64_t x) { return ((uint64_t)x + kByteBias) ^ kByteBias; }
+__device__ __forceinline__ int64_t sbyte_value(uint64_t w) { return (int64_t)((w ^ kByteBias) - kByteBias); }
+
+struct MfmaArgs {
+    const uint64_t* dsb;  // [2][N][B][Dp] digit words (slot-major)
+    const uint64_t* key;  // [2][N][units] key operand units of 16 bytes (two u64)
+    double* acc;          // [G][2][N][B cols] accumulators (slot-major, reduced)
+    int n, B, Dp, kbk, cols, D, gsize;
+    double q[2], qinv[2], c32[2];  // the auxiliary primes, 1/s, 2^32 mod s (centred)
+};
+
+// Digit operand for position P: both digit words of the lane shifted by P - 7 bytes.
+template <int P>
+__device__ __forceinline__ v4i digit_window(const uint64_t (&d)[2]) {
+    uint64_t w0, w1;
+    if constexpr (P <= 7) {
+        w0 = d[0] << (8 * (7 - P));
+        w1 = d[1] << (8 * (7 - P));
+    } else {
+        w0 = d[0] >> (8 * (P - 7));
+        w1 = d[1] >> (8 * (P - 7));
     }
-    fold();
+    v4i r;
+    r.x = (int)(uint32_t)w0;
+    r.y = (int)(uint32_t)(w0 >> 32);
+    r.z = (int)(uint32_t)w1;
+    r.w = (int)(uint32_t)(w1 >> 32);
+    return r;
+}
+
+// Four consecutive positions P0 .. P0+3 (four independent accumulators) folded into H[r] exactly.
+template <int P0, int KB>
+__device__ __forceinline__ void mfma_positions(const uint64_t (&a)[KB][2], const v4i (&bk)[KB], double (&H)[4]) {
+    v4i c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
 #pragma unroll
-    for (int bt = 0; bt < BT; bt++) {
-        if (b0 + bt >= B) continue;
+    for (int kb = 0; kb < KB; kb++) {
+        c0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0>(a[kb]), bk[kb], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0 + 1>(a[kb]), bk[kb], c1, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0 + 2>(a[kb]), bk[kb], c2, 0, 0, 0);
+        c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0 + 3>(a[kb]), bk[kb], c3, 0, 0, 0);
+    }
 #pragma unroll
-        for (int jt = 0; jt < JT; jt++) {
-            if (c0 + jt >= cols) continue;
-            acc[((((long long)g * B + b0 + bt) * cols + c0 + jt) * 2 + s) * n + x0 + lane] = a[bt][jt];
+    for (int r = 0; r < 4; r++)
+        H[r] = __builtin_fma((double)c3[r], 16777216.0,
+                             __builtin_fma((double)c2[r], 65536.0, __builtin_fma((double)c1[r], 256.0, (double)c0[r])));
+}
+
+// One slot: every M-tile of 16 ciphertexts by every N-tile of 16 key columns over the K-blocks
+// kb0 .. kb0+KB-1 (8 digits each), digits outside [i0, i1) masked to zero.
+template <int KB>
+__device__ __forceinline__ void mfma_slot(const MfmaArgs& A, int s, long long x, int g, int kb0, int i0, int i1,
+                                          int lane) {
+    const int j = lane & 15, gq = lane >> 4;
+    const long long slot = (long long)s * A.n + x;
+    const uint64_t* drow = A.dsb + slot * A.B * A.Dp;
+    const long long units = 4LL * A.kbk * A.cols;
+    const uint64_t* krow = A.key + slot * units * 2;
+    double* arow = A.acc + ((long long)g * 2 * A.n + slot) * ((long long)A.B * A.cols);
+    const double q = A.q[s], qinv = A.qinv[s], c32 = A.c32[s];
+    for (int m0 = 0; m0 < A.B; m0 += 16) {
+        const int b = min(m0 + j, A.B - 1);  // rows past the batch repeat the last one; they are not stored
+        uint64_t a[KB][2];
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+            const int d0 = 8 * (kb0 + kb) + 2 * gq;
+            const bool in0 = d0 >= i0 && d0 < i1, in1 = d0 + 1 >= i0 && d0 + 1 < i1;
+            v2u v = {0ull, 0ull};
+            if (in0 || in1) v = *reinterpret_cast<const v2u*>(drow + (long long)b * A.Dp + d0);
+            a[kb][0] = in0 ? v.x : 0ull;
+            a[kb][1] = in1 ? v.y : 0ull;
+        }
+        for (int nt = 0; 16 * nt < A.cols; nt++) {
+            // the digit words are opaque to the compiler here, so the 13 KB shifted operands are formed in
+            // each N-tile (a few shifts) rather than all held in registers across the loop
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) asm volatile("" : "+v"(a[kb][0]), "+v"(a[kb][1]));
+            const int w = min(16, A.cols - 16 * nt);
+            const uint64_t* kt = krow + 2LL * 16 * nt * A.kbk * 4;
+            v4i bk[KB];
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) {
+                const bool ok = j < w && kb0 + kb < A.kbk;
+                bk[kb] = ok ? *reinterpret_cast<const v4i*>(kt + 2LL * (((kb0 + kb) * 4 + gq) * w + j))
+                            : v4i{0, 0, 0, 0};
+            }
+            double V[4], H[4];
+            {
+                v4i c = {0, 0, 0, 0};
+#pragma unroll
+                for (int kb = 0; kb < KB; kb++)
+                    c = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<12>(a[kb]), bk[kb], c, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; r++) V[r] = (double)c[r];
+            }
+            mfma_positions<8, KB>(a, bk, H);
+#pragma unroll
+            for (int r = 0; r < 4; r++) V[r] = mulmod(V[r], c32, q, qinv) + H[r];
+            mfma_positions<4, KB>(a, bk, H);
+#pragma unroll
+            for (int r = 0; r < 4; r++) V[r] = mulmod(V[r], c32, q, qinv) + H[r];
+            mfma_positions<0, KB>(a, bk, H);
+#pragma unroll
+            for (int r = 0; r < 4; r++) V[r] = reduce(mulmod(V[r], c32, q, qinv) + H[r], q, qinv);
+            if (j < w) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int bb = m0 + 4 * gq + r;
+                    if (bb < A.B) arow[(long long)bb * A.cols + 16 * nt + j] = V[r];
+                }
+            }
         }
     }
 }
 
-// The last column group of a row may hold fewer columns than JT: it runs the narrowest tile that still
-// covers them, so no products are computed for columns past the key's.
-template <int BT, int JT>
-__device__ __forceinline__ void mac_tile_fit(const double* __restrict__ dn, const double* __restrict__ key,
-                                             double* __restrict__ acc, int n, int B, int D, int cols, int b0, int c0,
-                                             int s, int g, int i0, int i1, long long x0, int lane, double q,
-                                             double qinv) {
-    if constexpr (JT > 4) {
-        if (cols - c0 <= JT / 2)
-            return mac_tile_fit<BT, JT / 2>(dn, key, acc, n, B, D, cols, b0, c0, s, g, i0, i1, x0, lane, q, qinv);
-    }
-    mac_tile<BT, JT>(dn, key, acc, n, B, D, cols, b0, c0, s, g, i0, i1, x0, lane, q, qinv);
-}
-
-// The same tile when the four waves of a workgroup take four b-tiles of the same key columns: the
-// workgroup's key tile (JT columns x 64 coefficients of one digit) is copied into LDS once, by
-// global_load_lds (each lane 16 bytes, a wave two columns), and every wave reads it from there.
-// The waves' own vector loads are then only their digits, BT per digit instead of BT + JT: the
-// texture address and data units, which each vector load instruction occupies, bound the kernel.
-// Per digit t: wait for this wave's loads of t, barrier (every wave's copy of tile t has landed and
-// every read of the other buffer is done), start the copy of tile t + 1 and the digit loads of t + 1,
-// then read tile t and multiply.
-constexpr int kWaitVm0Exact = 0x0F70;  // s_waitcnt vmcnt(0), the other counters at their maximum
-
-template <int BT, int JT>
-__device__ __forceinline__ void mac_tile_lds(const double* __restrict__ dn, const double* __restrict__ key,
-                                             double* __restrict__ acc, double* kt, int n, int B, int D, int cols,
-                                             int b0, int c0, int s, int g, int i0, int i1, long long x0, int wave,
-                                             int lane, double q, double qinv) {
-    const long long n2 = 2LL * n;
-    long long dbase[BT];
-#pragma unroll
-    for (int bt = 0; bt < BT; bt++) dbase[bt] = ((long long)min(b0 + bt, B - 1) * D * 2 + s) * n + x0;
-    const long long kstep = (long long)cols * n2;  // key digit i -> i + 1
-    // column pair m of the tile (m = wave, wave + 4, ..): lanes 0-31 copy column c0 + 2m, lanes 32-63
-    // column c0 + 2m + 1 (columns past the key's read its last one; their products are dropped)
-    const int half = lane >> 5, sub = 2 * (lane & 31);
-    auto copy = [&](int t, int buf) {
-#pragma unroll
-        for (int m = wave; m < JT / 2; m += 4) {
-            const int col = min(c0 + 2 * m + half, cols - 1);
-            const double* src = key + (long long)t * kstep + (long long)col * n2 + (long long)s * n + x0 + sub;
-            double* dst = kt + (buf * JT + 2 * m) * 64;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-        }
-    };
-    auto load_d = [&](int t, double (&d)[BT]) {
-#pragma unroll
-        for (int bt = 0; bt < BT; bt++) d[bt] = (dn + dbase[bt] + t * n2)[lane];
-    };
-    double a[BT][JT];
-#pragma unroll
-    for (int bt = 0; bt < BT; bt++)
-#pragma unroll
-        for (int jt = 0; jt < JT; jt++) a[bt][jt] = 0.0;
-    // reduced operands: each product is <= 0.57 s, so eight of them on a reduced accumulator stay
-    // below 5.1 s < 2^53 / s
-    auto fold = [&]() {
-#pragma unroll
-        for (int bt = 0; bt < BT; bt++)
-#pragma unroll
-            for (int jt = 0; jt < JT; jt++) a[bt][jt] = reduce(a[bt][jt], q, qinv);
-    };
-    auto step = [&](int t, const double (&cur)[BT], double (&nxt)[BT], int buf) {
-        __builtin_amdgcn_s_waitcnt(kWaitVm0Exact);
-        __syncthreads();
-        if (t + 1 < i1) {
-            copy(t + 1, buf ^ 1);
-            load_d(t + 1, nxt);
-        }
-        // keeps the loads ahead of the multiplies (the compiler would otherwise sink them to their
-        // first use); neither barrier reads a register, so neither adds a wait
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_sched_barrier(0);
-        double k[JT];
-#pragma unroll
-        for (int jt = 0; jt < JT; jt++) k[jt] = kt[(buf * JT + jt) * 64 + lane];
-#pragma unroll
-        for (int bt = 0; bt < BT; bt++)
-#pragma unroll
-            for (int jt = 0; jt < JT; jt++) a[bt][jt] += mulmod(cur[bt], k[jt], q, qinv);
-        if (((t - i0) & 7) == 7) fold();
-    };
-    double dA[BT], dB[BT];
-    copy(i0, 0);
-    load_d(i0, dA);
-    for (int t = i0; t < i1; t += 2) {
-        step(t, dA, dB, 0);
-        if (t + 1 < i1) step(t + 1, dB, dA, 1);
-    }
-    fold();
-#pragma unroll
-    for (int bt = 0; bt < BT; bt++) {
-        if (b0 + bt >= B) continue;
-#pragma unroll
-        for (int jt = 0; jt < JT; jt++) {
-            if (c0 + jt >= cols) continue;
-            acc[((((long long)g * B + b0 + bt) * cols + c0 + jt) * 2 + s) * n + x0 + lane] = a[bt][jt];
-        }
-    }
-}
-
-// the narrowest tile that still covers the last column group (uniform over the workgroup)
-template <int BT, int JT>
-__device__ __forceinline__ void mac_tile_lds_fit(const double* __restrict__ dn, const double* __restrict__ key,
-                                                 double* __restrict__ acc, double* kt, int n, int B, int D, int cols,
-                                                 int b0, int c0, int s, int g, int i0, int i1, long long x0, int wave,
-                                                 int lane, double q, double qinv) {
-    if constexpr (JT > 4) {
-        if (cols - c0 <= JT / 2)
-            return mac_tile_lds_fit<BT, JT / 2>(dn, key, acc, kt, n, B, D, cols, b0, c0, s, g, i0, i1, x0, wave, lane,
-                                                q, qinv);
-    }
-    mac_tile_lds<BT, JT>(dn, key, acc, kt, n, B, D, cols, b0, c0, s, g, i0, i1, x0, wave, lane, q, qinv);
-}
-
-// Four waves along the ciphertexts (WB = 4): work items as in k_ks_mac_exact.  A wave whose b-tile
-// lies past the batch still takes part in every barrier and copy; it stores nothing.
-template <int BT, int JT>
-__global__ __launch_bounds__(256) void k_ks_mac_exact_lds(const double* __restrict__ dn,
-                                                          const double* __restrict__ key, double* __restrict__ acc,
-                                                          int n, int B, int D, int gsize, int cols, int njg, int nxb,
-                                                          int nbg, int items, int sbase,
-                                                          const DevPrime* __restrict__ primes,
-                                                          const int* __restrict__ gate) {
-    __shared__ double kt[2 * JT * 64];
+// Work item (digit group g, prime s, block of 4 spw slots); each of the four waves takes spw consecutive
+// slots.  gate (nullable, device): the launch does nothing unless *gate != 0.
+template <int KB>
+__global__ __launch_bounds__(256) void k_ks_mac_mfma(MfmaArgs A, int spw, int nxb, const int* __restrict__ gate) {
     if (gate != nullptr && *gate == 0) return;
-    const int w = blockIdx.x;
-    const int per = gridDim.x >> 3;
-    const int item = (w & 7) * per + (w >> 3);
-    if (item >= items) return;
-    int r = item;
-    const int jg = r % njg;
-    r /= njg;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int r = blockIdx.x;
     const int xb = r % nxb;
     r /= nxb;
-    const int bg = r % nbg;
-    r /= nbg;
     const int s = r & 1, g = r >> 1;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int b0 = (bg * 4 + wave) * BT;
-    const int c0 = jg * JT;
-    const DevPrime P = primes[sbase + s];
-    const int i0 = g * gsize, i1 = min(D, i0 + gsize);
-    mac_tile_lds_fit<BT, JT>(dn, key, acc, kt, n, B, D, cols, b0, c0, s, g, i0, i1, (long long)xb * 64, wave, lane,
-                             P.q, P.qinv);
+    const int i0 = g * A.gsize, i1 = min(A.D, i0 + A.gsize);
+    const long long x0 = ((long long)xb * 4 + wave) * spw;
+    for (int t = 0; t < spw; t++) mfma_slot<KB>(A, s, x0 + t, g, i0 >> 3, i0, i1, lane);
 }
 
-template <int BT, int JT>
-__global__ __launch_bounds__(256) void k_ks_mac_exact(const double* __restrict__ dn, const double* __restrict__ key,
-                                                      double* __restrict__ acc, int n, int B, int D, int gsize,
-                                                      int cols, int WB, int njg, int nxb, int nbg, int items,
-                                                      int sbase, const DevPrime* __restrict__ primes,
-                                                      const int* __restrict__ gate) {
-    if (gate != nullptr && *gate == 0) return;
-    const int w = blockIdx.x;
-    const int per = gridDim.x >> 3;
-    const int item = (w & 7) * per + (w >> 3);
-    if (item >= items) return;
-    int r = item;
-    const int jg = r % njg;
-    r /= njg;
-    const int xb = r % nxb;
-    r /= nxb;
-    const int bg = r % nbg;
-    r /= nbg;
-    const int s = r & 1, g = r >> 1;
-    // wave-uniform quantities live in scalar registers: every load below is a scalar base plus the
-    // lane's offset, with no per-lane address arithmetic
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int WJ = 4 / WB;
-    const int wb = wave % WB, wj = wave / WB;
-    const int b0 = (bg * WB + wb) * BT;
-    const int c0 = (jg * WJ + wj) * JT;
-    if (b0 >= B || c0 >= cols) return;
-    const DevPrime P = primes[sbase + s];
-    const int i0 = g * gsize, i1 = min(D, i0 + gsize);
-    mac_tile_fit<BT, JT>(dn, key, acc, n, B, D, cols, b0, c0, s, g, i0, i1, (long long)xb * 64, lane, P.q, P.qinv);
+static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, int B, int D,
+                          int Dp, int kbk, int cols, int gsize) {
+    MfmaArgs A{};
+    A.dsb = dsb;
+    A.key = key;
+    A.acc = acc;
+    A.n = (int)c.n;
+    A.B = B;
+    A.Dp = Dp;
+    A.kbk = kbk;
+    A.cols = cols;
+    A.D = D;
+    A.gsize = gsize;
+    for (int s = 0; s < 2; s++) {
+        const uint64_t sp = c.sp[s];
+        A.q[s] = (double)sp;
+        A.qinv[s] = 1.0 / (double)sp;
+        const uint64_t r = (uint64_t)(((unsigned __int128)1 << 32) % sp);
+        A.c32[s] = r > sp / 2 ? (double)r - (double)sp : (double)r;
+    }
+    return A;
 }
 
-template <int BT, int JT>
-static void mac_exact_t(const Context& c, const double* dn, const double* key, double* acc, int B, int D, int gsize,
-                        int G, int E, hipStream_t s, const int* gate) {
-    const int n = (int)c.n;
-    const int nbt = (B + BT - 1) / BT;  // b-tiles; WB of them per workgroup (1, 2 or 4 waves along them)
-    const int WB = nbt >= 3 ? 4 : nbt;
-    const int WJ = 4 / WB;
-    const int cols = 2 * E;
-    const int njt = (cols + JT - 1) / JT;
-    const int njg = (njt + WJ - 1) / WJ;
-    const int nxb = n / 64;
-    const int nbg = (B + WB * BT - 1) / (WB * BT);
-    const long long items = (long long)njg * nxb * nbg * 2 * G;
-    RCK_REQUIRE(items < (1LL << 30), "exact keyswitch launch too large");
-    const int per = (int)((items + 7) / 8);
+// spw slots per wave (1, 2 or 4); digits in groups of gsize (G groups); gate as in k_ks_mac_mfma
+static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, int B, int D,
+                            int Dp, int kbk, int cols, int gsize, int G, int spw, hipStream_t s,
+                            const int* gate = nullptr) {
+    RCK_REQUIRE(spw == 1 || spw == 2 || spw == 4, "bad exact keyswitch shape");
+    RCK_REQUIRE(c.n % (4 * spw) == 0 && B >= 1 && 8 * kbk >= D && Dp >= 8 * ((D + 7) / 8), "bad exact keyswitch size");
+    // K-blocks one group spans (uniform over the groups' starts)
+    int span = 1;
+    for (int g = 0; g < G; g++) {
+        const int i0 = g * gsize, i1 = std::min(D, i0 + gsize);
+        if (i1 > i0) span = std::max(span, (i1 - 1) / 8 - i0 / 8 + 1);
+    }
+    const MfmaArgs A = mfma_args(c, dsb, key, acc, B, D, Dp, kbk, cols, gsize);
+    const int nxb = (int)(c.n / (4 * spw));
+    const long long blocks = (long long)G * 2 * nxb;
+    RCK_REQUIRE(blocks < (1LL << 31), "exact keyswitch launch too large");
     ProfScope ps(s, gate ? "ks_mac_gated" : "ks_mac");
-    if (WB == 4)
-        k_ks_mac_exact_lds<BT, JT><<<dim3(per * 8), 256, 0, s>>>(dn, key, acc, n, B, D, gsize, cols, njg, nxb, nbg,
-                                                                 (int)items, (int)(c.L + c.K), c.d_primes, gate);
-    else
-        k_ks_mac_exact<BT, JT><<<dim3(per * 8), 256, 0, s>>>(dn, key, acc, n, B, D, gsize, cols, WB, njg, nxb, nbg,
-                                                             (int)items, (int)(c.L + c.K), c.d_primes, gate);
+    switch (span) {
+#define RCK_MM(K)                                                                   \
+    case K: k_ks_mac_mfma<K><<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate); break;
+        RCK_MM(1)
+        RCK_MM(2)
+        RCK_MM(3)
+        RCK_MM(4)
+        RCK_MM(5)
+        RCK_MM(6)
+        RCK_MM(7)
+        RCK_MM(8)
+#undef RCK_MM
+        default: RCK_REQUIRE(false, "exact keyswitch supports at most 64 digits per group");
+    }
     RCK_CHECK_HIP(hipGetLastError());
 }
 
-template <int BT>
-static void mac_exact_j(const Context& c, int jt, const double* dn, const double* key, double* acc, int B, int D,
-                        int gsize, int G, int E, hipStream_t s, const int* gate) {
-    if (jt == 4)
-        mac_exact_t<BT, 4>(c, dn, key, acc, B, D, gsize, G, E, s, gate);
-    else if (jt == 16)
-        mac_exact_t<BT, 16>(c, dn, key, acc, B, D, gsize, G, E, s, gate);
-    else
-        mac_exact_t<BT, 8>(c, dn, key, acc, B, D, gsize, G, E, s, gate);
+// ---------------------------------------------------------------------------------------
+// Row passes between the standard limb rows and the slot-major operands of the matrix-core MAC.
+// A workgroup takes ring row rb of S consecutive rows z under one prime; the tile goes through LDS
+// once more so that each store (or load) instruction covers S consecutive z of a few slots.
+// ---------------------------------------------------------------------------------------
+// Forward row stages of the digit transforms: in rows (z, s) at in + (z 2 + s) N after the column pass,
+// z = b D + i (Z = B D rows) -> out[(s N + x) B Dp + b Dp + i] as signed-byte words (x the NTT-domain
+// position).
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_fwd_rows_sm(const double* __restrict__ in, uint64_t* __restrict__ out,
+                                                     int logn, int Z, int D, int Dp, int B, int sbase,
+                                                     const DevPrime* __restrict__ primes) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[S * row_pitch<R>()];
+    const int n = 1 << logn, N1 = n >> LOGR;
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int z0 = blockIdx.x * S, rb = blockIdx.y, s = blockIdx.z;
+    const DevPrime P = primes[sbase + s];
+    const double q = P.q, qinv = P.qinv;
+    const int z = min(z0 + row, Z - 1);  // rows past Z transform the last one; nothing of theirs is stored
+    const double* p = in + ((long long)z * 2 + s) * n + (long long)rb * R;
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = p[tau + TPS * e];
+    fwd_phase_a(v, P.fw, N1 + rb, q, qinv);
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, tau + TPS * e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, 16 * tau + e)];
+    fwd_phase_b<LOGR>(v, P.fw, N1 + rb, tau, q, qinv);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, e * TPS + tau)] = v[e];  // NTT-domain position e TPS + tau
+    __syncthreads();
+    const int zz = tid % S, pos0 = tid / S, zo = z0 + zz;
+    if (zo >= Z) return;
+    const long long pitch = (long long)B * Dp;
+    uint64_t* o = out + ((long long)s * n + (long long)rb * R) * pitch + (long long)(zo / D) * Dp + zo % D;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int pos = pos0 + TPS * e;
+        o[(long long)pos * pitch] = sbyte_word(f64_to_i64(lds[row_lds<R>(zz, pos)]));
+    }
 }
 
-// bt ciphertexts by jt key columns per thread (bt in {1, 2, 4}, jt in {4, 8, 16}); gate (nullable,
-// device): the launch does nothing unless *gate != 0
-static void launch_mac_exact(const Context& c, int bt, int jt, const double* dn, const double* key, double* acc, int B,
-                             int D, int gsize, int G, int E, hipStream_t s, const int* gate = nullptr) {
-    RCK_REQUIRE((bt == 1 || bt == 2 || bt == 4) && (jt == 4 || jt == 8 || jt == 16), "bad exact keyswitch tile");
-    if (bt == 4)
-        mac_exact_j<4>(c, jt, dn, key, acc, B, D, gsize, G, E, s, gate);
-    else if (bt == 2)
-        mac_exact_j<2>(c, jt, dn, key, acc, B, D, gsize, G, E, s, gate);
-    else
-        mac_exact_j<1>(c, jt, dn, key, acc, B, D, gsize, G, E, s, gate);
+// Inverse row stages of slot-major accumulators: in [G][2][N][Zc] (Zc = B cols) -> rows
+// ((g Zc + z) 2 + s) N, the layout of the fused inverse column pass and recombination.
+template <int LOGR>
+__global__ __launch_bounds__(256) void k_inv_rows_sm(const double* __restrict__ in, double* __restrict__ out,
+                                                     int logn, int Zc, int sbase, const DevPrime* __restrict__ primes,
+                                                     const int* __restrict__ gate) {
+    constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    __shared__ double lds[S * row_pitch<R>()];
+    if (gate != nullptr && *gate == 0) return;
+    const int n = 1 << logn, N1 = n >> LOGR;
+    const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
+    const int z0 = blockIdx.x * S, rb = blockIdx.y, s = blockIdx.z & 1, g = blockIdx.z >> 1;
+    const DevPrime P = primes[sbase + s];
+    const double q = P.q, qinv = P.qinv;
+    const int zz = tid % S, pos0 = tid / S;
+    const double* src =
+        in + ((long long)(g * 2 + s) * n + (long long)rb * R) * Zc + min(z0 + zz, Zc - 1);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int pos = pos0 + TPS * e;
+        lds[row_lds<R>(zz, pos)] = src[(long long)pos * Zc];
+    }
+    __syncthreads();
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, e * TPS + tau)];  // position e TPS + tau: x = 16 tau + e
+    inv_phase_b<LOGR>(v, P.iv, N1 + rb, tau, q, qinv);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, 16 * tau + e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, tau + TPS * e)];
+    inv_phase_a(v, P.iv, N1 + rb, q, qinv);
+    const int z = z0 + row;
+    if (z >= Zc) return;
+    double* o = out + (((long long)g * Zc + z) * 2 + s) * n + (long long)rb * R;
+#pragma unroll
+    for (int e = 0; e < 16; e++) o[tau + TPS * e] = v[e];
+}
+
+template <int LOGR>
+static void rows_sm_t(const Context& c, bool fwd, const double* din, uint64_t* dout, const double* ain, double* aout,
+                      int Z, int D, int Dp, int B, int G, hipStream_t s, const int* gate) {
+    constexpr int S = 256 / ((1 << LOGR) / 16);
+    const int n1 = 1 << (c.logn - LOGR);
+    const int sbase = (int)(c.L + c.K);
+    if (fwd) {
+        ProfScope ps(s, "ntt_fwd_rows");
+        k_fwd_rows_sm<LOGR><<<dim3((Z + S - 1) / S, n1, 2), 256, 0, s>>>(din, dout, (int)c.logn, Z, D, Dp, B, sbase,
+                                                                         c.d_primes);
+    } else {
+        ProfScope ps(s, gate ? "ntt_inv_rows_gated" : "ntt_inv_rows");
+        k_inv_rows_sm<LOGR><<<dim3((Z + S - 1) / S, n1, 2 * G), 256, 0, s>>>(ain, aout, (int)c.logn, Z, sbase,
+                                                                             c.d_primes, gate);
+    }
+    RCK_CHECK_HIP(hipGetLastError());
+}
+
+static void launch_rows_sm(const Context& c, bool fwd, const double* din, uint64_t* dout, const double* ain,
+                           double* aout, int Z, int D, int Dp, int B, int G, hipStream_t s,
+                           const int* gate = nullptr) {
+    RCK_REQUIRE(Z >= 1 && 2 * G <= 65535, "bad exact keyswitch rows");
+    int n1, n2;
+    split_logn((int)c.logn, &n1, &n2);
+    switch (n2) {
+        case 6: rows_sm_t<6>(c, fwd, din, dout, ain, aout, Z, D, Dp, B, G, s, gate); break;
+        case 7: rows_sm_t<7>(c, fwd, din, dout, ain, aout, Z, D, Dp, B, G, s, gate); break;
+        case 8: rows_sm_t<8>(c, fwd, din, dout, ain, aout, Z, D, Dp, B, G, s, gate); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// The key's matrix-core form: per slot (s, x), units of 16 bytes in the order
+//   N-tile nt (16 key columns; the last one w = cols - 16 nt wide), K-block kb, lane group gq, column j,
+// unit = key digits 8 kb + 2 gq and 8 kb + 2 gq + 1 of column 16 nt + j, each a signed-byte word with
+// its bytes reversed.  Digits past the key's are zero.  An operand load of a wave is then 4 w
+// consecutive units.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ long long key_unit_word(int n, int cols, int kbk, int s, long long x, int col, int d) {
+    const int nt = col >> 4, j = col & 15, w = min(16, cols - 16 * nt);
+    const int kb = d >> 3, gq = (d & 7) >> 1, h = d & 1;
+    const long long units = 4LL * kbk * cols;
+    const long long u = 16LL * nt * kbk * 4 + (long long)(kb * 4 + gq) * w + j;
+    return (((long long)s * n + x) * units + u) * 2 + h;
+}
+
+// digit d of the key: stage [cols][2][N] (the NTT forms under s1, s2, reduced) -> its words in key
+__global__ void k_key_digit_to_mfma(const double* __restrict__ stage, uint64_t* __restrict__ key, int n, int cols,
+                                    int kbk, int d) {
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int col = blockIdx.y, s = blockIdx.z;
+    const double v = stage[((long long)col * 2 + s) * n + x];
+    key[key_unit_word(n, cols, kbk, s, x, col, d)] = __builtin_bswap64(sbyte_word(f64_to_i64(v)));
+}
+
+// the inverse: digit d's words -> stage [cols][2][N] (centred doubles)
+__global__ void k_key_digit_from_mfma(const uint64_t* __restrict__ key, double* __restrict__ stage, int n, int cols,
+                                      int kbk, int d) {
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int col = blockIdx.y, s = blockIdx.z;
+    stage[((long long)col * 2 + s) * n + x] =
+        (double)sbyte_value(__builtin_bswap64(key[key_unit_word(n, cols, kbk, s, x, col, d)]));
 }
 
 // Inverse column pass of the accumulators fused with the recombination: a workgroup transforms the
@@ -648,20 +702,24 @@ __global__ __launch_bounds__(1024) void k_exact_gate(const double* __restrict__ 
 
 bool exact_keyswitch_available(const Context& c, uint32_t aux) { return c.S == 2 && aux > 0 && c.logn >= 12; }
 
-// Converts a key from its NTT form mod the eval primes (data) to the exact-convolution form (sdata);
-// data is released unless keep_data.
+// Converts a key from its NTT form mod the eval primes (data) to the exact-convolution form (mdata, the
+// matrix-core words of its centred limbs' NTT forms under s1 and s2); data is released unless keep_data.
 void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
     require_fp64(c);
     RCK_REQUIRE(k && k->data, "null key");
     RCK_REQUIRE(exact_keyswitch_available(c, k->aux), "exact-convolution keys need hybrid keys at N >= 2^12");
+    RCK_REQUIRE(k->digits <= 64, "exact-convolution keys hold at most 64 digits");
     const long long n = c.n;
-    const int E = (int)k->limbs, nq = (int)k->level + 1;
+    const int E = (int)k->limbs, nq = (int)k->level + 1, cols = 2 * E;
+    const int kbk = ((int)k->digits + 7) / 8;
     const long long P = (long long)E * n;
-    double* sd = nullptr;
+    const size_t bytes = (size_t)2 * n * 4 * kbk * cols * 16;
+    uint64_t* md = nullptr;
     uint64_t* coeff = nullptr;
     double* stage = nullptr;  // 2 P doubles (inverse NTT input), then 4 P (the centred limbs mod s1, s2)
     try {
-        RCK_CHECK_HIP(hipMalloc(&sd, (size_t)k->digits * 2 * P * 2 * 8));
+        RCK_CHECK_HIP(hipMalloc(&md, bytes));
+        RCK_CHECK_HIP(hipMemset(md, 0, bytes));  // digits past the key's stay zero
         RCK_CHECK_HIP(hipMalloc(&coeff, (size_t)2 * P * 8));
         RCK_CHECK_HIP(hipMalloc(&stage, (size_t)4 * P * 8));
         const PrimeMap pm{nq, (int)c.L, 0};
@@ -672,11 +730,13 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
             k_key_to_special<<<dim3((c.n + 255) / 256, E, 2), 256>>>(coeff, stage, (int)c.n, E, nq, (int)c.L,
                                                                     (int)(c.L + c.K), c.d_primes);
             RCK_CHECK_HIP(hipGetLastError());
-            launch_fwd(c, kInF64, stage, n, 2 * n, sd + (size_t)d * 2 * P * 2, n, 2 * n, 2, 2 * E, ps, nullptr);
+            launch_fwd(c, kInF64, stage, n, 2 * n, stage, n, 2 * n, 2, cols, ps, nullptr);
+            k_key_digit_to_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(stage, md, (int)c.n, cols, kbk, (int)d);
+            RCK_CHECK_HIP(hipGetLastError());
         }
         RCK_CHECK_HIP(hipDeviceSynchronize());
     } catch (...) {
-        if (sd) (void)hipFree(sd);
+        if (md) (void)hipFree(md);
         if (coeff) (void)hipFree(coeff);
         if (stage) (void)hipFree(stage);
         throw;
@@ -687,21 +747,24 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
         RCK_CHECK_HIP(hipFree(k->data));
         k->data = nullptr;
     }
-    k->sdata = sd;
+    k->mdata = md;
 }
 
 // The crate's coefficient form of digit d of an exact-convolution key: out [2][E][N] (device).
 void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out) {
     const long long n = c.n;
-    const int E = (int)k->limbs, nq = (int)k->level + 1;
+    const int E = (int)k->limbs, nq = (int)k->level + 1, cols = 2 * E;
+    const int kbk = ((int)k->digits + 7) / 8;
     const long long P = (long long)E * n;
     double* stage = nullptr;
     uint64_t* coeff = nullptr;
     try {
         RCK_CHECK_HIP(hipMalloc(&stage, (size_t)2 * P * 2 * 8));
         RCK_CHECK_HIP(hipMalloc(&coeff, (size_t)2 * P * 2 * 8));
-        RCK_CHECK_HIP(hipMemcpy(stage, k->sdata + (size_t)d * 2 * P * 2, 2 * P * 2 * 8, hipMemcpyDeviceToDevice));
-        launch_inv(c, stage, n, 2 * n, kOutU64, coeff, n, 2 * n, 2, 2 * E, PrimeMap{0, (int)(c.L + c.K), 0}, nullptr);
+        k_key_digit_from_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(k->mdata, stage, (int)c.n, cols, kbk,
+                                                                         (int)d);
+        RCK_CHECK_HIP(hipGetLastError());
+        launch_inv(c, stage, n, 2 * n, kOutU64, coeff, n, 2 * n, 2, cols, PrimeMap{0, (int)(c.L + c.K), 0}, nullptr);
         k_key_from_special<<<dim3((c.n + 255) / 256, E, 2), 256>>>(coeff, out, (int)c.n, E, nq, (int)c.L,
                                                                   spec_const(c), c.d_spmod, c.d_primes);
         RCK_CHECK_HIP(hipGetLastError());
@@ -716,21 +779,28 @@ void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* o
 }
 
 // Keyswitch of B polynomials (stride in_ps) at `limbs` Q limbs with an exact-convolution key, `levels`
-// digits; adds add0/add1 and writes outs[b] = [c0][c1] (keyswitch_batch's contract).
+// digits; adds add0/add1 and writes outs[b] = [c0][c1] (keyswitch_batch's contract).  spw: slots per
+// wave of the multiply-accumulate.
 void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, unsigned long long ginv, int bt, int jt) {
+                     hipStream_t s, unsigned long long ginv, int spw) {
     require_fp64(c);
     const long long n = c.n;
-    const int E = limbs + (int)key->aux;
+    const int E = limbs + (int)key->aux, cols = 2 * E;
+    const int kbk = ((int)key->digits + 7) / 8;
+    const int Dp = 8 * ((levels + 7) / 8);
+    RCK_REQUIRE(levels <= (int)key->digits && cols == 2 * (int)key->limbs, "exact keyswitch key does not fit");
     const ExactPlan pl = exact_plan(c, limbs, levels);
     const int G = pl.G, Gw = pl.Gw;
+    const int Zc = B * cols;
     int32_t* digits = static_cast<int32_t*>(c.workspace(5, (size_t)B * levels * n * 4));
-    // workspace slots 30-34 (slots 20-26 hold the BSGS buffers of a transform whose rotations come here);
+    // workspace slots 30-36 (slots 20-26 hold the BSGS buffers of a transform whose rotations come here);
     // the accumulators are sized for the worst-case groups, which the recomputation reuses
     double* dn = static_cast<double*>(c.workspace(30, (size_t)B * levels * 2 * n * 8));
-    double* acc = static_cast<double*>(c.workspace(31, (size_t)Gw * B * 2 * E * 2 * n * 8));
-    double* accq = static_cast<double*>(c.workspace(32, (size_t)Gw * B * 2 * E * n * 8));
+    uint64_t* dsb = static_cast<uint64_t*>(c.workspace(35, (size_t)2 * n * B * Dp * 8));
+    double* accm = static_cast<double*>(c.workspace(36, (size_t)Gw * 2 * n * Zc * 8));
+    double* acc = static_cast<double*>(c.workspace(31, (size_t)Gw * Zc * 2 * n * 8));
+    double* accq = static_cast<double*>(c.workspace(32, (size_t)Gw * Zc * n * 8));
     double* l1 = nullptr;
     int* gate = nullptr;
     if (pl.gated) {
@@ -744,20 +814,22 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
         RCK_CHECK_HIP(hipGetLastError());
     }
     const PrimeMap ps{0, (int)(c.L + c.K), 0};
-    // every digit under s1 and s2: limb stride 0 on the input (the same digit row for both)
-    launch_fwd(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
-    launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, pl.gsize, G, E, s);
-    // inverse rows under s1, s2; inverse columns fused with the recombination (one value per
-    // accumulator and coefficient: the group's exact sum mod p_j); then the group sum and mod-down
-    // (skipped when the gate is set)
-    launch_inv_rows(c, acc, n, 2 * n, 2, G * B * 2 * E, ps, s);
-    launch_inv_cols_crt(c, acc, accq, limbs, G * B * 2 * E, s);
+    // every digit under s1 and s2: column pass (limb stride 0 on the input: the same digit row for both),
+    // then the row stages written slot-major as signed-byte words
+    launch_fwd_cols(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
+    launch_rows_sm(c, true, dn, dsb, nullptr, nullptr, B * levels, levels, Dp, B, 1, s);
+    launch_mac_mfma(c, dsb, key->mdata, accm, B, levels, Dp, kbk, cols, pl.gsize, G, spw, s);
+    // inverse rows under s1, s2 (back to limb rows); inverse columns fused with the recombination (one
+    // value per accumulator and coefficient: the group's exact sum mod p_j); then the group sum and
+    // mod-down (skipped when the gate is set)
+    launch_rows_sm(c, false, nullptr, nullptr, accm, acc, Zc, 0, 0, B, G, s);
+    launch_inv_cols_crt(c, acc, accq, limbs, G * Zc, s);
     launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s, gate, 0);
     if (pl.gated) {
         // some ciphertext's digits exceed the one-group bound: the batch again with the worst-case groups
-        launch_mac_exact(c, bt, jt, dn, key->sdata, acc, B, levels, pl.gsw, Gw, E, s, gate);
-        launch_inv_rows(c, acc, n, 2 * n, 2, Gw * B * 2 * E, ps, s, 1, 0, nullptr, 0, 0, 0, gate);
-        launch_inv_cols_crt(c, acc, accq, limbs, Gw * B * 2 * E, s, gate);
+        launch_mac_mfma(c, dsb, key->mdata, accm, B, levels, Dp, kbk, cols, pl.gsw, Gw, spw, s, gate);
+        launch_rows_sm(c, false, nullptr, nullptr, accm, acc, Zc, 0, 0, B, Gw, s, gate);
+        launch_inv_cols_crt(c, acc, accq, limbs, Gw * Zc, s, gate);
         launch_crt_moddown(c, accq, limbs, B, Gw, add0, add1, add_ps, ginv, outs, s, gate, 1);
     }
 }

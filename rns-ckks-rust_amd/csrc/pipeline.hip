@@ -261,10 +261,9 @@ int ks_tuned_import(const std::string& text) {
         RCK_REQUIRE(v[1] >= 1 && v[2] >= v[1] && v[3] >= 1 && v[4] >= 1, "tuning table entry out of range: " + line);
         const bool w16 = v[9] == 16 && (v[7] == 4 || v[7] == 16) && v[8] >= 0 && (v[8] <= 4 || ((v[8] == 5 || v[8] == 6) && v[7] == 4));
         const bool w8 = v[9] == 8 && (v[7] == 8 || v[7] == 16) && v[8] == 0;
-        // exact-convolution entries: load schedule kExactShape, rows = ciphertexts per thread, width = key
-        // columns per thread
+        // exact-convolution entries: load schedule kExactShape, rows = slots per wave of the multiply-accumulate
         const bool ex = v[8] == kExactShape && v[5] == v[2] && v[6] == 1 && (v[7] == 1 || v[7] == 2 || v[7] == 4) &&
-                        (v[9] == 4 || v[9] == 8 || v[9] == 16);
+                        v[9] == 16;
         RCK_REQUIRE(v[5] >= 1 && v[5] <= v[2] && v[6] >= 1 && v[6] <= 8 && (w16 || w8 || ex),
                     "tuning table launch shape out of range: " + line);
         rows.push_back({std::make_tuple(v[0], v[1], v[2], v[3], v[4], ex ? 1 : 0),
@@ -276,7 +275,7 @@ int ks_tuned_import(const std::string& text) {
 }
 
 bool ks_use_exact(const Context& c, const KeyDev* key, int limbs, int E, int D, int B, const KsShape& shape) {
-    if (!key->sdata) return false;
+    if (!key->mdata) return false;
     if (!key->data) return true;
     if (shape.prefetch == kExactShape) return true;
     if (shape.prefetch >= 0 || shape.group > 0 || shape.rows > 0) return false;  // an explicit digit-NTT shape
@@ -290,12 +289,12 @@ bool ks_use_exact(const Context& c, const KeyDev* key, int limbs, int E, int D, 
 KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B, bool exact) {
     KsShape r;
     if (ks_tuned_find(c, limbs, E, D, B, &r, nullptr, exact)) return r;
-    if (exact) {  // four ciphertexts by eight key columns per thread from three ciphertexts up
+    if (exact) {  // two slots per wave of the matrix-core multiply-accumulate
         r.group = E;
         r.dsplit = 1;
-        r.rows = B >= 3 ? 4 : B;
+        r.rows = 2;
         r.prefetch = kExactShape;
-        r.width = 8;
+        r.width = 16;
         return r;
     }
     r.group = c.tune_group > 0 ? c.tune_group : E;
@@ -316,10 +315,7 @@ KsShape ks_auto_shape(const Context& c, int limbs, int E, int D, int B, bool exa
 std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B, bool exact) {
     std::vector<KsShape> out;
     if (exact) {
-        for (int bt : {1, 2, 4}) {
-            if (bt > 1 && bt >= 2 * B) continue;
-            for (int jt : {4, 8, 16}) out.push_back({E, 1, bt, kExactShape, jt});
-        }
+        for (int spw : {1, 2, 4}) out.push_back({E, 1, spw, kExactShape, 16});
         return out;
     }
     if (c.logn <= 11) {
@@ -373,11 +369,11 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
                            add1 ? add1 + b * add_ps : nullptr, outs[b], s);
         return;
     }
-    RCK_REQUIRE(key->data != nullptr || key->sdata != nullptr,
+    RCK_REQUIRE(key->data != nullptr || key->mdata != nullptr,
                 "keyswitch key was created for another arithmetic of this context");
     if (addn == nullptr && ks_use_exact(c, key, limbs, limbs + naux, levels, B, shape)) {
         const KsShape sh = shape.prefetch == kExactShape ? shape : ks_auto_shape(c, limbs, limbs + naux, levels, B, true);
-        keyswitch_exact(c, poly, in_ps, B, limbs, key, levels, add0, add1, add_ps, outs, s, ginv, sh.rows, sh.width);
+        keyswitch_exact(c, poly, in_ps, B, limbs, key, levels, add0, add1, add_ps, outs, s, ginv, sh.rows);
         return;
     }
     RCK_REQUIRE(key->data != nullptr, "folded addends need a key with its digit-NTT form");

@@ -8,9 +8,10 @@ namespace rck {
 // over level+1 Q-limbs followed by aux limbs (hybrid) or none (Q-only).
 struct KeyDev {
     double* data = nullptr;
-    // exact-convolution form instead (hybrid keys, N >= 2^12): sdata[digit][comp][limb][s][N], the NTT
-    // forms under the two auxiliary primes s of the key's centred limbs (data is then null)
-    double* sdata = nullptr;
+    // exact-convolution form (hybrid keys, N >= 2^12): the NTT forms under the two auxiliary primes of the
+    // key's centred limbs, as the matrix-core operand words of ks_exact.hip (data is null when the key
+    // holds this form only)
+    uint64_t* mdata = nullptr;
     // 64-bit integer contexts instead: wdata[digit][comp][limb][N], NTT form times R = 2^64 (wide.hpp)
     uint64_t* wdata = nullptr;
     uint32_t digits = 0, level = 0, aux = 0, limbs = 0;
@@ -29,7 +30,7 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data = false);
 void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out);
 void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, unsigned long long ginv, int bt, int jt);
+                     hipStream_t s, unsigned long long ginv, int spw);
 
 // Launch shape of the digit MAC: eval limbs per group and digit splits (0 = automatic).
 struct KsShape {
@@ -39,8 +40,8 @@ struct KsShape {
     int prefetch = -1; // load schedule: 0 late keys, 1 early keys, 2 next digit's rows prefetched, 3 keys in groups
     int width = 16;    // elements per thread in the row pass (16, or 8 with rows 8 / 16)
 };
-// load schedule value of an exact-convolution shape: rows = ciphertexts and width = key columns per
-// thread of its multiply-accumulate
+// load schedule value of an exact-convolution shape: rows = slots per wave of its multiply-accumulate
+// (1, 2 or 4; width is unused and 16)
 constexpr int kExactShape = 7;
 
 inline void check_ptr(const void* p, const char* what) {

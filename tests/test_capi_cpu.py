@@ -64,14 +64,14 @@ def test_tune_table_round_trip_and_validation():
     G = capi
     before = G.tune_table_export()
     rows = ("9 5 7 9 3 7 2 16 1 16 1.250000\n9 5 7 9 2 4 1 4 0 16 0.500000\n9 5 7 9 4 7 1 8 0 8 0.750000\n"
-            "9 5 7 9 3 7 1 4 7 8 0.125000\n")  # the last: an exact-convolution tile (load schedule 7)
+            "9 5 7 9 3 7 1 4 7 16 0.125000\n")  # the last: an exact-convolution shape (load schedule 7)
     assert G.tune_table_import(rows) == 4
     table = G.tune_table_export()
     for line in rows.splitlines():
         assert line in table.splitlines()
     for bad in ("9 5 7 9 3 7 2 8 1 16 1.0\n", "9 5 7 9 3 8 2 16 1 16 1.0\n", "9 5 7 9\n",
                 "9 5 7 9 3 7 2 4 0 8 1.0\n", "9 5 7 9 3 7 2 16 1 1.0\n", "9 5 7 9 3 7 2 16 5 16 1.0\n", "9 5 7 9 3 7 2 16 6 16 1.0\n", "9 5 7 9 3 7 2 4 7 16 1.0\n",
-                "9 5 7 9 3 7 1 3 7 8 1.0\n", "9 5 7 9 3 6 1 4 7 8 1.0\n", "9 5 7 9 3 7 1 4 7 32 1.0\n"):
+                "9 5 7 9 3 7 1 3 7 16 1.0\n", "9 5 7 9 3 6 1 4 7 16 1.0\n", "9 5 7 9 3 7 1 4 7 8 1.0\n"):
         with pytest.raises(G.RckError):
             G.tune_table_import(bad)
         assert G.tune_table_export() == table

@@ -207,7 +207,7 @@ def test_headline_workload_batched_matches_per_op_and_oracle():
     assert np.array_equal(want[0], O.relinearize(qb, eb, O.multiply_ciphertexts(qb, lhs_h[0], rhs_h[0]), key))
     for pf in (5, 6, 7):  # the streamed digit-NTT MACs, then the exact-convolution method
         key_used = ke if pf == 7 else kc
-        G.tune_table_import(f"16 {L} {L + K} {D} {B} {L + K} 1 4 {pf} {8 if pf == 7 else 16} 1.0\n")
+        G.tune_table_import(f"16 {L} {L + K} {D} {B} {L + K} 1 {2 if pf == 7 else 4} {pf} 16 1.0\n")
         out = G.DeviceBuffer((B, 2, L, n))
         bt = G.Batch(ctx)
         for b in range(B):

@@ -89,10 +89,11 @@ def test_mac_streamed_batch_bit_identical(logn, limbs):
             assert np.array_equal(got[b], want[b]), (ds, pf, b)
 
 
-@pytest.mark.parametrize("logn,limbs,B", [(12, 4, 1), (13, 6, 5), (16, 5, 3), (14, 5, 9)])
-def test_exact_mac_tiles_bit_identical(logn, limbs, B):
-    """Every multiply-accumulate tile of the exact-convolution keyswitch (ciphertexts 1 / 2 / 4 by key
-    columns 4 / 8 / 16 per thread), forced through the tuning table, gives the digit-NTT result."""
+@pytest.mark.parametrize("logn,limbs,B", [(12, 4, 1), (13, 6, 5), (16, 5, 3), (14, 5, 9), (13, 9, 17)])
+def test_exact_mac_shapes_bit_identical(logn, limbs, B):
+    """Every launch shape of the exact-convolution keyswitch's matrix-core multiply-accumulate (1 / 2 / 4
+    slots per wave), forced through the tuning table, gives the digit-NTT result: one or two tiles of 16
+    ciphertexts, key columns filling 12 / 14 / 16 / 22 of the 16-column tiles, 1 to 2 digit blocks."""
     n = 1 << logn
     p = PR.config_batched(n=1 << 16, limbs=limbs, aux=2)
     ctx = G.Context(poly_degree=n, primes=p.primes, roots=p.roots, aux_primes=p.aux_primes, aux_roots=p.aux_roots)
@@ -115,17 +116,16 @@ def test_exact_mac_tiles_bit_identical(logn, limbs, B):
         ctx.mult(lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), limbs, 40, 40, quad)
         ctx.relinearize(quad, limbs, 80, kc, one)
         want.append(one.numpy())
-    for bt in (1, 2, 4):
-        for jt in (4, 8, 16):
-            G.tune_table_import(f"{logn} {limbs} {E} {D} {B} {E} 1 {bt} 7 {jt} 1.0\n")
-            out = G.DeviceBuffer((B, 2, limbs, n))
-            batch = G.Batch(ctx)
-            for b in range(B):
-                batch.add(G.OP_MULT_RELIN, limbs, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ke,
-                          out.offset(b * 2 * P))
-            batch.compile(tune=True)
-            grp = batch.report()["groups"][0]
-            assert (grp["mac_rows"], grp["load_schedule"]) == (bt, 7)
-            got = out.numpy()
-            for b in range(B):
-                assert np.array_equal(got[b], want[b]), (bt, jt, b)
+    for spw in (1, 2, 4):
+        G.tune_table_import(f"{logn} {limbs} {E} {D} {B} {E} 1 {spw} 7 16 1.0\n")
+        out = G.DeviceBuffer((B, 2, limbs, n))
+        batch = G.Batch(ctx)
+        for b in range(B):
+            batch.add(G.OP_MULT_RELIN, limbs, lhs.offset(b * 2 * P), rhs.offset(b * 2 * P), ke,
+                      out.offset(b * 2 * P))
+        batch.compile(tune=True)
+        grp = batch.report()["groups"][0]
+        assert (grp["mac_rows"], grp["load_schedule"]) == (spw, 7)
+        got = out.numpy()
+        for b in range(B):
+            assert np.array_equal(got[b], want[b]), (spw, b)
