@@ -98,13 +98,14 @@ __global__ void k_key_from_special(const uint64_t* __restrict__ in, uint64_t* __
 //   C_p[b][jc] = sum_i sum_{u+v=p} dn_u[b][i] key_v[i][jc]      (|C_p| <= 8 D 2^14)
 // are products of int8 matrices, and acc = sum_p C_p 256^p exactly.  One v_mfma_i32_16x16x64_i8 takes
 // 16 ciphertexts by 16 key columns over 8 digits x 8 byte pairs: lane l holds row (A) or column (B)
-// l % 16 and the K-items 16 (l / 16) + t, t < 16, here two digits x 8 bytes; output row 4 (l / 16) + r,
+// l % 16 and the K-items 16 (l / 16) + t, t < 16, here two digits x 8 bytes in the word order
+// [digit 0 bytes 0-3, digit 1 bytes 0-3, digit 0 bytes 4-7, digit 1 bytes 4-7]; output row 4 (l / 16) + r,
 // column l % 16 in register r.  The key's bytes are stored reversed (item t of a digit = key byte
 // 7 - t) and the digit's operand for position p is its word shifted by p - 7 bytes (item t = digit
 // byte p - 7 + t, zero outside 0 .. 6), so item t pairs digit byte u = p - 7 + t with key byte
-// v = 7 - t: u + v = p.  The 13 sums are recombined in FP64: H_k = sum_{t<4} C_{4k+t} 2^{8t}
-// (|H| < 2^47.1, exact), V = ((H_3 c + H_2) c + H_1) c + H_0 mod s with c = 2^32 mod s (three exact
-// modular products), then reduced: the value the crate's per-prime products sum to, mod s.
+// v = 7 - t: u + v = p.  The 13 sums are recombined in FP64 from three exact chunks (positions 0-3,
+// 4-7 and 8-12, each below 2^47.1; see mfma_tile) with two exact modular products by 2^32 mod s, then
+// reduced: the value the crate's per-prime products sum to, mod s.
 // ---------------------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned long long v2u __attribute__((ext_vector_type(2)));
@@ -123,134 +124,223 @@ struct MfmaArgs {
     const uint64_t* dsb;  // [2][N][B][Dp] digit words (slot-major)
     const uint64_t* key;  // [2][N][units] key operand units of 16 bytes (two u64)
     double* acc;          // [G][2][N][B cols] accumulators (slot-major, reduced)
+    double* trash;        // 256 doubles: the store target of lanes without an output
     int n, B, Dp, kbk, cols, D, gsize;
     double q[2], qinv[2], c32[2];  // the auxiliary primes, 1/s, 2^32 mod s (centred)
 };
 
-// Digit operand for position P: both digit words of the lane shifted by P - 7 bytes.
-template <int P>
-__device__ __forceinline__ v4i digit_window(const uint64_t (&d)[2]) {
-    uint64_t w0, w1;
-    if constexpr (P <= 7) {
-        w0 = d[0] << (8 * (7 - P));
-        w1 = d[1] << (8 * (7 - P));
-    } else {
-        w0 = d[0] >> (8 * (P - 7));
-        w1 = d[1] >> (8 * (P - 7));
-    }
-    v4i r;
-    r.x = (int)(uint32_t)w0;
-    r.y = (int)(uint32_t)(w0 >> 32);
-    r.z = (int)(uint32_t)w1;
-    r.w = (int)(uint32_t)(w1 >> 32);
-    return r;
-}
-
-// Four consecutive positions P0 .. P0+3 (four independent accumulators) folded into H[r] exactly.
-template <int P0, int KB>
-__device__ __forceinline__ void mfma_positions(const uint64_t (&a)[KB][2], const v4i (&bk)[KB], double (&H)[4]) {
-    v4i c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+// One step: the 16 x 16 output tile (M-tile of ciphertexts, N-tile of key columns) of one slot, all 13
+// positions accumulated K-block by K-block.  refill(kb) runs right after K-block kb's last product: it
+// loads the next step's operands into the registers that block no longer needs, so every load has about
+// one step to arrive and no second buffer is held.  MASK: digits outside [i0, i1) (a digit group of the
+// gated recomputation) are zeroed as they are used (lane constants in0 / in1 per K-block).
+// Recombination: the chunks H_low (positions 0-3), H_mid (4-7) and H_top (8-12) are exact doubles.  Byte
+// 6 of a residue below 2^50.1 (centred) is at most 3 in size, so |C_12| <= 9 D and every chunk stays
+// below D 2^41.03 <= 2^47.03 (D <= 64); V = (H_top c + H_mid) c + H_low mod s, c = 2^32 mod s.
+template <int KB, bool MASK, class Refill>
+__device__ __forceinline__ void mfma_tile(v2u (&a)[KB], v4i (&bk)[KB], double (&V)[4], double q, double qinv,
+                                          double c32, int kb0, int gq, int i0, int i1, Refill refill) {
+    v4i c[13];
+#pragma unroll
+    for (int p = 0; p < 13; p++) c[p] = v4i{0, 0, 0, 0};
 #pragma unroll
     for (int kb = 0; kb < KB; kb++) {
-        c0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0>(a[kb]), bk[kb], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0 + 1>(a[kb]), bk[kb], c1, 0, 0, 0);
-        c2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0 + 2>(a[kb]), bk[kb], c2, 0, 0, 0);
-        c3 = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<P0 + 3>(a[kb]), bk[kb], c3, 0, 0, 0);
+        uint64_t d0 = a[kb].x, d1 = a[kb].y;
+        // opaque here: the shifted operands are formed per use, not held across N-tiles (registers)
+        asm volatile("" : "+v"(d0), "+v"(d1));
+        if constexpr (MASK) {
+            const int e0 = 8 * (kb0 + kb) + 2 * gq;
+            d0 = e0 >= i0 && e0 < i1 ? d0 : 0ull;
+            d1 = e0 + 1 >= i0 && e0 + 1 < i1 ? d1 : 0ull;
+        }
+        // The operand of position p = 4k + r is bits [8p, 8p + 64) of X = d 2^56 for both digits, as the
+        // words [d0 lo, d1 lo, d0 hi, d1 hi] (the lane's item order).  With Y[2m + h] = word m of digit h's
+        // X (words 0, 4, 5 are zero) and Z_r[2m + h] = alignbyte(Y[2m + 2 + h], Y[2m + h], r), that operand
+        // is Y[2k .. 2k+3] (r = 0) or Z_r[2k .. 2k+3]: 15 shifts per digit for all 13 operands.
+        const uint32_t l0 = (uint32_t)d0, h0 = (uint32_t)(d0 >> 32), l1 = (uint32_t)d1, h1 = (uint32_t)(d1 >> 32);
+        const uint32_t Y[12] = {0u,      0u,      l0 << 24, l1 << 24, __builtin_amdgcn_alignbyte(h0, l0, 1u),
+                                __builtin_amdgcn_alignbyte(h1, l1, 1u), h0 >> 8, h1 >> 8, 0u, 0u, 0u, 0u};
+#define RCK_MM4(W, K) v4i{(int)W[2 * K], (int)W[2 * K + 1], (int)W[2 * K + 2], (int)W[2 * K + 3]}
+        c[0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 0), c[0], 0, 0, 0);
+        c[4] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 1), c[4], 0, 0, 0);
+        c[8] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 2), c[8], 0, 0, 0);
+        c[12] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 3), c[12], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 1; r < 4; r++) {
+            uint32_t Z[8];
+#pragma unroll
+            for (int m = 0; m < 8; m++) Z[m] = __builtin_amdgcn_alignbyte(Y[m + 2], Y[m], (uint32_t)r);
+            c[r] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Z, 0), c[r], 0, 0, 0);
+            c[4 + r] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Z, 1), c[4 + r], 0, 0, 0);
+            c[8 + r] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Z, 2), c[8 + r], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#undef RCK_MM4
+        refill(kb);
+        // keeps the refill loads here (the compiler would otherwise sink them to their use); the barrier
+        // reads no register, so it adds no wait
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-        H[r] = __builtin_fma((double)c3[r], 16777216.0,
-                             __builtin_fma((double)c2[r], 65536.0, __builtin_fma((double)c1[r], 256.0, (double)c0[r])));
+    for (int r = 0; r < 4; r++) {
+        double top = (double)c[12][r];
+#pragma unroll
+        for (int p = 11; p >= 8; p--) top = __builtin_fma(top, 256.0, (double)c[p][r]);
+        double mid = (double)c[7][r];
+#pragma unroll
+        for (int p = 6; p >= 4; p--) mid = __builtin_fma(mid, 256.0, (double)c[p][r]);
+        double low = (double)c[3][r];
+#pragma unroll
+        for (int p = 2; p >= 0; p--) low = __builtin_fma(low, 256.0, (double)c[p][r]);
+        const double v = mulmod(top, c32, q, qinv) + mid;
+        V[r] = reduce(mulmod(v, c32, q, qinv) + low, q, qinv);
+        __builtin_amdgcn_sched_barrier(0);  // one output register at a time: 13 sums read, not 52
+    }
 }
 
-// One slot: every M-tile of 16 ciphertexts by every N-tile of 16 key columns over the K-blocks
-// kb0 .. kb0+KB-1 (8 digits each), digits outside [i0, i1) masked to zero.
-template <int KB>
-__device__ __forceinline__ void mfma_slot(const MfmaArgs& A, int s, long long x, int g, int kb0, int i0, int i1,
-                                          int lane) {
+// A wave's slots xbeg .. xbeg+nslots-1 (xbeg wave-uniform): for each (slot, M-tile) its N-tiles in order.
+// The products are D = key^T x digits, so lane l holds ciphertext 16 mt + l % 16 and the four key columns
+// 16 nt + 4 (l / 16) + r: 32 contiguous bytes of the [b][col] output, two 16-byte stores.
+// No memory operation of the loops is conditional, so the in-order memory counter lets every wait cover
+// exactly the loads it needs:
+//  * each step's refill loads the next N-tile's key operands; the last N-tile's refill also loads the
+//    next (slot, M-tile)'s digit words (the digit words of a slot are loaded once, not per N-tile); the
+//    last step reloads its own operands (valid addresses, never used);
+//  * key columns past the last tile's width read its last column (those products are not stored), and
+//    K-blocks past the key's (a group's span in the gated recomputation) read its last block, against
+//    digits that MASK zeroes;
+//  * a step's stores are issued during the next step, after its first refill, and lanes with no output
+//    (padding columns, rows past the batch) store to a scratch line instead.
+// Each address is a wave-uniform base plus a lane offset fixed for the launch.
+template <int KB, bool MASK>
+__device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xbeg, int nslots, int g, int kb0, int i0,
+                                          int i1, int lane) {
     const int j = lane & 15, gq = lane >> 4;
-    const long long slot = (long long)s * A.n + x;
-    const uint64_t* drow = A.dsb + slot * A.B * A.Dp;
-    const long long units = 4LL * A.kbk * A.cols;
-    const uint64_t* krow = A.key + slot * units * 2;
-    double* arow = A.acc + ((long long)g * 2 * A.n + slot) * ((long long)A.B * A.cols);
+    const int NT = (A.cols + 15) >> 4, MT = (A.B + 15) >> 4;
+    const int wl = A.cols - 16 * (NT - 1);         // width of the last N-tile
+    const long long units = 4LL * A.kbk * A.cols;  // 16-byte key units per slot
+    // lane byte offsets (32-bit) from wave-uniform bases: the loads take the scalar-base + vector-offset form
+    const uint32_t lk_full = (uint32_t)(gq * 16 + j) * 16u, lk_last = (uint32_t)(gq * wl + min(j, wl - 1)) * 16u;
     const double q = A.q[s], qinv = A.qinv[s], c32 = A.c32[s];
-    for (int m0 = 0; m0 < A.B; m0 += 16) {
-        const int b = min(m0 + j, A.B - 1);  // rows past the batch repeat the last one; they are not stored
-        uint64_t a[KB][2];
+    const long long zc = (long long)A.B * A.cols;
+    typedef const __attribute__((address_space(1))) char gchar;
+    auto uniform_ptr = [](const char* p) -> gchar* {
+        const uint64_t v = (uint64_t)p;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+        return (gchar*)(((uint64_t)hi << 32) | lo);
+    };
+    auto load_k = [&](long long sl, int nt, int kb) -> v4i {
+        const bool last = nt == NT - 1;
+        const int w = last ? wl : 16;
+        const int kbc = min(kb0 + kb, A.kbk - 1);
+        const long long base = sl * units + 16LL * nt * A.kbk * 4 + (long long)kbc * 4 * w;
+        gchar* kp = uniform_ptr(reinterpret_cast<const char*>(A.key) + base * 16);
+        return *(const __attribute__((address_space(1))) v4i*)(kp + (uint64_t)(last ? lk_last : lk_full));
+    };
+    // digit words of ciphertext 16 mt + j (rows past the batch repeat the last one; they are not stored)
+    auto load_a = [&](long long sl, int mt, int kb) -> v2u {
+        const int u0 = min(8 * (kb0 + kb), A.Dp - 8);
+        gchar* dp = uniform_ptr(reinterpret_cast<const char*>(A.dsb) + ((sl * A.B + 16LL * mt) * A.Dp + u0) * 8);
+        const uint32_t la = (uint32_t)(min(j, A.B - 1 - 16 * mt) * A.Dp + 2 * gq) * 8u;
+        return *(const __attribute__((address_space(1))) v2u*)(dp + (uint64_t)la);
+    };
+    // output addresses of a step: ciphertext 16 mt + j, columns 16 nt + 4 gq + 2 pr + {0, 1} (cols is even,
+    // so a pair is valid or not as a whole); lanes without an output write the scratch line
+    double* const accg = A.acc + (long long)g * 2 * A.n * zc;
+    auto out_ptr = [&](long long sl, int mt, int nt, int pr) -> double* {
+        const int bb = 16 * mt + j, col = 16 * nt + 4 * gq + 2 * pr;
+        const bool ok = bb < A.B && col < A.cols;
+        // both addresses computed, then selected (no branch in the loop)
+        double* const p = accg + sl * zc + (long long)(ok ? bb : 0) * A.cols + (ok ? col : 0);
+        return ok ? p : A.trash + lane * 4 + 2 * pr;
+    };
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    const int groups = nslots * MT;
+    long long sl = (long long)s * A.n + xbeg;
+    int mt = 0;
+    v2u a[KB];
+    v4i bk[KB];
+    // first step's operands in K-block order, as the refills issue them
 #pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            const int d0 = 8 * (kb0 + kb) + 2 * gq;
-            const bool in0 = d0 >= i0 && d0 < i1, in1 = d0 + 1 >= i0 && d0 + 1 < i1;
-            v2u v = {0ull, 0ull};
-            if (in0 || in1) v = *reinterpret_cast<const v2u*>(drow + (long long)b * A.Dp + d0);
-            a[kb][0] = in0 ? v.x : 0ull;
-            a[kb][1] = in1 ? v.y : 0ull;
-        }
-        for (int nt = 0; 16 * nt < A.cols; nt++) {
-            // the digit words are opaque to the compiler here, so the 13 KB shifted operands are formed in
-            // each N-tile (a few shifts) rather than all held in registers across the loop
-#pragma unroll
-            for (int kb = 0; kb < KB; kb++) asm volatile("" : "+v"(a[kb][0]), "+v"(a[kb][1]));
-            const int w = min(16, A.cols - 16 * nt);
-            const uint64_t* kt = krow + 2LL * 16 * nt * A.kbk * 4;
-            v4i bk[KB];
-#pragma unroll
-            for (int kb = 0; kb < KB; kb++) {
-                const bool ok = j < w && kb0 + kb < A.kbk;
-                bk[kb] = ok ? *reinterpret_cast<const v4i*>(kt + 2LL * (((kb0 + kb) * 4 + gq) * w + j))
-                            : v4i{0, 0, 0, 0};
-            }
-            double V[4], H[4];
-            {
-                v4i c = {0, 0, 0, 0};
-#pragma unroll
-                for (int kb = 0; kb < KB; kb++)
-                    c = __builtin_amdgcn_mfma_i32_16x16x64_i8(digit_window<12>(a[kb]), bk[kb], c, 0, 0, 0);
-#pragma unroll
-                for (int r = 0; r < 4; r++) V[r] = (double)c[r];
-            }
-            mfma_positions<8, KB>(a, bk, H);
-#pragma unroll
-            for (int r = 0; r < 4; r++) V[r] = mulmod(V[r], c32, q, qinv) + H[r];
-            mfma_positions<4, KB>(a, bk, H);
-#pragma unroll
-            for (int r = 0; r < 4; r++) V[r] = mulmod(V[r], c32, q, qinv) + H[r];
-            mfma_positions<0, KB>(a, bk, H);
-#pragma unroll
-            for (int r = 0; r < 4; r++) V[r] = reduce(mulmod(V[r], c32, q, qinv) + H[r], q, qinv);
-            if (j < w) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int bb = m0 + 4 * gq + r;
-                    if (bb < A.B) arow[(long long)bb * A.cols + 16 * nt + j] = V[r];
-                }
-            }
-        }
+    for (int kb = 0; kb < KB; kb++) {
+        bk[kb] = load_k(sl, 0, kb);
+        a[kb] = load_a(sl, 0, kb);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
     }
+    v2d Vp[2];     // the previous step's outputs, stored during this step (the first step stores zeros to
+    double* op[2];  // the scratch line)
+#pragma unroll
+    for (int pr = 0; pr < 2; pr++) {
+        Vp[pr] = v2d{0.0, 0.0};
+        op[pr] = A.trash + lane * 4 + 2 * pr;
+    }
+    auto store_prev = [&]() {
+#pragma unroll
+        for (int pr = 0; pr < 2; pr++) *reinterpret_cast<v2d*>(op[pr]) = Vp[pr];
+    };
+    auto keep = [&](const double (&V)[4], long long sl0, int mt0, int nt0) {
+#pragma unroll
+        for (int pr = 0; pr < 2; pr++) {
+            Vp[pr] = v2d{V[2 * pr], V[2 * pr + 1]};
+            op[pr] = out_ptr(sl0, mt0, nt0, pr);
+        }
+    };
+    for (int u = 0; u < groups; u++) {
+        int mt1 = mt + 1;
+        long long sl1 = sl;
+        if (mt1 == MT) {
+            mt1 = 0;
+            sl1++;
+        }
+        if (u + 1 == groups) {  // the last step reloads its own operands (a valid address; never used)
+            mt1 = mt;
+            sl1 = sl;
+        }
+        double V[4];
+        for (int nt = 0; nt < NT - 1; nt++) {
+            mfma_tile<KB, MASK>(a, bk, V, q, qinv, c32, kb0, gq, i0, i1, [&](int kb) {
+                bk[kb] = load_k(sl, nt + 1, kb);
+                if (kb == 0) store_prev();
+            });
+            keep(V, sl, mt, nt);
+        }
+        mfma_tile<KB, MASK>(a, bk, V, q, qinv, c32, kb0, gq, i0, i1, [&](int kb) {
+            bk[kb] = load_k(sl1, 0, kb);
+            a[kb] = load_a(sl1, mt1, kb);
+            if (kb == 0) store_prev();
+        });
+        keep(V, sl, mt, NT - 1);
+        sl = sl1;
+        mt = mt1;
+    }
+    store_prev();
 }
 
 // Work item (digit group g, prime s, block of 4 spw slots); each of the four waves takes spw consecutive
-// slots.  gate (nullable, device): the launch does nothing unless *gate != 0.
-template <int KB>
+// slots.  gate (nullable, device): the launch does nothing unless *gate != 0.  MASK: digit groups that do
+// not cover all digits.
+template <int KB, bool MASK>
 __global__ __launch_bounds__(256) void k_ks_mac_mfma(MfmaArgs A, int spw, int nxb, const int* __restrict__ gate) {
     if (gate != nullptr && *gate == 0) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     int r = blockIdx.x;
     const int xb = r % nxb;
     r /= nxb;
     const int s = r & 1, g = r >> 1;
     const int i0 = g * A.gsize, i1 = min(A.D, i0 + A.gsize);
-    const long long x0 = ((long long)xb * 4 + wave) * spw;
-    for (int t = 0; t < spw; t++) mfma_slot<KB>(A, s, x0 + t, g, i0 >> 3, i0, i1, lane);
+    mfma_wave<KB, MASK>(A, s, ((long long)xb * 4 + wave) * spw, spw, g, i0 >> 3, i0, i1, lane);
 }
 
-static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, int B, int D,
-                          int Dp, int kbk, int cols, int gsize) {
+static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, double* trash,
+                          int B, int D, int Dp, int kbk, int cols, int gsize) {
     MfmaArgs A{};
     A.dsb = dsb;
     A.key = key;
     A.acc = acc;
+    A.trash = trash;
     A.n = (int)c.n;
     A.B = B;
     A.Dp = Dp;
@@ -269,8 +359,9 @@ static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint64_t*
 }
 
 // spw slots per wave (1, 2 or 4); digits in groups of gsize (G groups); gate as in k_ks_mac_mfma
-static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, int B, int D,
-                            int Dp, int kbk, int cols, int gsize, int G, int spw, hipStream_t s,
+// acc: G groups of [2][N][B cols]; trash: 256 doubles of scratch
+static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, double* trash,
+                            int B, int D, int Dp, int kbk, int cols, int gsize, int G, int spw, hipStream_t s,
                             const int* gate = nullptr) {
     RCK_REQUIRE(spw == 1 || spw == 2 || spw == 4, "bad exact keyswitch shape");
     RCK_REQUIRE(c.n % (4 * spw) == 0 && B >= 1 && 8 * kbk >= D && Dp >= 8 * ((D + 7) / 8), "bad exact keyswitch size");
@@ -280,14 +371,17 @@ static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint64_
         const int i0 = g * gsize, i1 = std::min(D, i0 + gsize);
         if (i1 > i0) span = std::max(span, (i1 - 1) / 8 - i0 / 8 + 1);
     }
-    const MfmaArgs A = mfma_args(c, dsb, key, acc, B, D, Dp, kbk, cols, gsize);
+    const MfmaArgs A = mfma_args(c, dsb, key, acc, trash, B, D, Dp, kbk, cols, gsize);
     const int nxb = (int)(c.n / (4 * spw));
     const long long blocks = (long long)G * 2 * nxb;
     RCK_REQUIRE(blocks < (1LL << 31), "exact keyswitch launch too large");
+    // one group over every digit needs no mask: the digit rows are zero past D, and so is the key past its digits
+    const bool mask = G > 1;
     ProfScope ps(s, gate ? "ks_mac_gated" : "ks_mac");
-    switch (span) {
-#define RCK_MM(K)                                                                   \
-    case K: k_ks_mac_mfma<K><<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate); break;
+    switch (span * 2 + (mask ? 1 : 0)) {
+#define RCK_MM(K)                                                                                      \
+    case 2 * K: k_ks_mac_mfma<K, false><<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate); break; \
+    case 2 * K + 1: k_ks_mac_mfma<K, true><<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate); break;
         RCK_MM(1)
         RCK_MM(2)
         RCK_MM(3)
@@ -308,20 +402,24 @@ static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint64_
 // once more so that each store (or load) instruction covers S consecutive z of a few slots.
 // ---------------------------------------------------------------------------------------
 // Forward row stages of the digit transforms: in rows (z, s) at in + (z 2 + s) N after the column pass,
-// z = b D + i (Z = B D rows) -> out[(s N + x) B Dp + b Dp + i] as signed-byte words (x the NTT-domain
-// position).
+// z = b D + i -> out[(s N + x) B Dp + b Dp + i] as signed-byte words (x the NTT-domain position), over
+// the B Dp rows zp = b Dp + i: the padding digits D <= i < Dp are written as zeros, so the
+// multiply-accumulate reads whole K-blocks without masks.
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_fwd_rows_sm(const double* __restrict__ in, uint64_t* __restrict__ out,
-                                                     int logn, int Z, int D, int Dp, int B, int sbase,
+                                                     int logn, int D, int Dp, int B, int sbase,
                                                      const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[S * row_pitch<R>()];
-    const int n = 1 << logn, N1 = n >> LOGR;
+    const int n = 1 << logn, N1 = n >> LOGR, Zp = B * Dp;
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
     const int z0 = blockIdx.x * S, rb = blockIdx.y, s = blockIdx.z;
     const DevPrime P = primes[sbase + s];
     const double q = P.q, qinv = P.qinv;
-    const int z = min(z0 + row, Z - 1);  // rows past Z transform the last one; nothing of theirs is stored
+    // rows past Zp or on a padding digit transform digit row 0 (a valid row); their results are replaced
+    const int zp = min(z0 + row, Zp - 1), bi = zp / Dp, di = zp % Dp;
+    const bool real = z0 + row < Zp && di < D;
+    const int z = real ? bi * D + di : 0;
     const double* p = in + ((long long)z * 2 + s) * n + (long long)rb * R;
     double v[16];
 #pragma unroll
@@ -335,16 +433,15 @@ __global__ __launch_bounds__(256) void k_fwd_rows_sm(const double* __restrict__ 
     fwd_phase_b<LOGR>(v, P.fw, N1 + rb, tau, q, qinv);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, e * TPS + tau)] = v[e];  // NTT-domain position e TPS + tau
+    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, e * TPS + tau)] = real ? v[e] : 0.0;  // position e TPS + tau
     __syncthreads();
     const int zz = tid % S, pos0 = tid / S, zo = z0 + zz;
-    if (zo >= Z) return;
-    const long long pitch = (long long)B * Dp;
-    uint64_t* o = out + ((long long)s * n + (long long)rb * R) * pitch + (long long)(zo / D) * Dp + zo % D;
+    if (zo >= Zp) return;
+    uint64_t* o = out + ((long long)s * n + (long long)rb * R) * Zp + zo;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int pos = pos0 + TPS * e;
-        o[(long long)pos * pitch] = sbyte_word(f64_to_i64(lds[row_lds<R>(zz, pos)]));
+        o[(long long)pos * Zp] = sbyte_word(f64_to_i64(lds[row_lds<R>(zz, pos)]));
     }
 }
 
@@ -397,8 +494,9 @@ static void rows_sm_t(const Context& c, bool fwd, const double* din, uint64_t* d
     const int sbase = (int)(c.L + c.K);
     if (fwd) {
         ProfScope ps(s, "ntt_fwd_rows");
-        k_fwd_rows_sm<LOGR><<<dim3((Z + S - 1) / S, n1, 2), 256, 0, s>>>(din, dout, (int)c.logn, Z, D, Dp, B, sbase,
-                                                                         c.d_primes);
+        const int Zp = B * Dp;
+        k_fwd_rows_sm<LOGR><<<dim3((Zp + S - 1) / S, n1, 2), 256, 0, s>>>(din, dout, (int)c.logn, D, Dp, B, sbase,
+                                                                          c.d_primes);
     } else {
         ProfScope ps(s, gate ? "ntt_inv_rows_gated" : "ntt_inv_rows");
         k_inv_rows_sm<LOGR><<<dim3((Z + S - 1) / S, n1, 2 * G), 256, 0, s>>>(ain, aout, (int)c.logn, Z, sbase,
@@ -428,12 +526,14 @@ static void launch_rows_sm(const Context& c, bool fwd, const double* din, uint64
 // its bytes reversed.  Digits past the key's are zero.  An operand load of a wave is then 4 w
 // consecutive units.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ long long key_unit_word(int n, int cols, int kbk, int s, long long x, int col, int d) {
+// 32-bit word index of the low half of digit d's (byte-reversed) key word in its unit; the high half is
+// two words further (unit word order [digit 2 gq lo, digit 2 gq + 1 lo, digit 2 gq hi, digit 2 gq + 1 hi])
+__device__ __forceinline__ long long key_unit_dword(int n, int cols, int kbk, int s, long long x, int col, int d) {
     const int nt = col >> 4, j = col & 15, w = min(16, cols - 16 * nt);
     const int kb = d >> 3, gq = (d & 7) >> 1, h = d & 1;
     const long long units = 4LL * kbk * cols;
     const long long u = 16LL * nt * kbk * 4 + (long long)(kb * 4 + gq) * w + j;
-    return (((long long)s * n + x) * units + u) * 2 + h;
+    return (((long long)s * n + x) * units + u) * 4 + h;
 }
 
 // digit d of the key: stage [cols][2][N] (the NTT forms under s1, s2, reduced) -> its words in key
@@ -443,7 +543,11 @@ __global__ void k_key_digit_to_mfma(const double* __restrict__ stage, uint64_t* 
     if (x >= n) return;
     const int col = blockIdx.y, s = blockIdx.z;
     const double v = stage[((long long)col * 2 + s) * n + x];
-    key[key_unit_word(n, cols, kbk, s, x, col, d)] = __builtin_bswap64(sbyte_word(f64_to_i64(v)));
+    const uint64_t w = __builtin_bswap64(sbyte_word(f64_to_i64(v)));
+    uint32_t* k32 = reinterpret_cast<uint32_t*>(key);
+    const long long i = key_unit_dword(n, cols, kbk, s, x, col, d);
+    k32[i] = (uint32_t)w;
+    k32[i + 2] = (uint32_t)(w >> 32);
 }
 
 // the inverse: digit d's words -> stage [cols][2][N] (centred doubles)
@@ -452,8 +556,10 @@ __global__ void k_key_digit_from_mfma(const uint64_t* __restrict__ key, double* 
     const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
     const int col = blockIdx.y, s = blockIdx.z;
-    stage[((long long)col * 2 + s) * n + x] =
-        (double)sbyte_value(__builtin_bswap64(key[key_unit_word(n, cols, kbk, s, x, col, d)]));
+    const uint32_t* k32 = reinterpret_cast<const uint32_t*>(key);
+    const long long i = key_unit_dword(n, cols, kbk, s, x, col, d);
+    const uint64_t w = (uint64_t)k32[i] | ((uint64_t)k32[i + 2] << 32);
+    stage[((long long)col * 2 + s) * n + x] = (double)sbyte_value(__builtin_bswap64(w));
 }
 
 // Inverse column pass of the accumulators fused with the recombination: a workgroup transforms the
@@ -798,7 +904,7 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     // the accumulators are sized for the worst-case groups, which the recomputation reuses
     double* dn = static_cast<double*>(c.workspace(30, (size_t)B * levels * 2 * n * 8));
     uint64_t* dsb = static_cast<uint64_t*>(c.workspace(35, (size_t)2 * n * B * Dp * 8));
-    double* accm = static_cast<double*>(c.workspace(36, (size_t)Gw * 2 * n * Zc * 8));
+    double* accm = static_cast<double*>(c.workspace(36, ((size_t)Gw * 2 * n * Zc + 256) * 8));
     double* acc = static_cast<double*>(c.workspace(31, (size_t)Gw * Zc * 2 * n * 8));
     double* accq = static_cast<double*>(c.workspace(32, (size_t)Gw * Zc * n * 8));
     double* l1 = nullptr;
@@ -818,7 +924,8 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     // then the row stages written slot-major as signed-byte words
     launch_fwd_cols(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
     launch_rows_sm(c, true, dn, dsb, nullptr, nullptr, B * levels, levels, Dp, B, 1, s);
-    launch_mac_mfma(c, dsb, key->mdata, accm, B, levels, Dp, kbk, cols, pl.gsize, G, spw, s);
+    double* trash = accm + (size_t)Gw * 2 * n * Zc;
+    launch_mac_mfma(c, dsb, key->mdata, accm, trash, B, levels, Dp, kbk, cols, pl.gsize, G, spw, s);
     // inverse rows under s1, s2 (back to limb rows); inverse columns fused with the recombination (one
     // value per accumulator and coefficient: the group's exact sum mod p_j); then the group sum and
     // mod-down (skipped when the gate is set)
@@ -827,7 +934,7 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s, gate, 0);
     if (pl.gated) {
         // some ciphertext's digits exceed the one-group bound: the batch again with the worst-case groups
-        launch_mac_mfma(c, dsb, key->mdata, accm, B, levels, Dp, kbk, cols, pl.gsw, Gw, spw, s, gate);
+        launch_mac_mfma(c, dsb, key->mdata, accm, trash, B, levels, Dp, kbk, cols, pl.gsw, Gw, spw, s, gate);
         launch_rows_sm(c, false, nullptr, nullptr, accm, acc, Zc, 0, 0, B, Gw, s, gate);
         launch_inv_cols_crt(c, acc, accq, limbs, Gw * Zc, s, gate);
         launch_crt_moddown(c, accq, limbs, B, Gw, add0, add1, add_ps, ginv, outs, s, gate, 1);
