@@ -93,18 +93,13 @@ __global__ void k_key_from_special(const uint64_t* __restrict__ in, uint64_t* __
 // At NTT slot x under auxiliary prime s the accumulators of a batch are one matrix product mod s:
 //   acc[b][jc] = sum_i dn[b][i] * key[i][jc]   (B ciphertexts x D digits, by D digits x 2E key columns),
 // every operand a residue mod s < 2^50.1, centred.  Each operand is written in balanced signed bytes,
-// x = sum_{u<7} x_u 256^u with x_u in [-128, 128) and byte 7 zero.  For output byte position
-// p = u + v (0 .. 12) the integer sums
-//   C_p[b][jc] = sum_i sum_{u+v=p} dn_u[b][i] key_v[i][jc]      (|C_p| <= 8 D 2^14)
-// are products of int8 matrices, and acc = sum_p C_p 256^p exactly.  One v_mfma_i32_16x16x64_i8 takes
-// 16 ciphertexts by 16 key columns over 8 digits x 8 byte pairs: lane l holds row (A) or column (B)
-// l % 16 and the K-items 16 (l / 16) + t, t < 16, here two digits x 8 bytes in the word order
-// [digit 0 bytes 0-3, digit 1 bytes 0-3, digit 0 bytes 4-7, digit 1 bytes 4-7]; output row 4 (l / 16) + r,
-// column l % 16 in register r.  The key's bytes are stored reversed (item t of a digit = key byte
-// 7 - t) and the digit's operand for position p is its word shifted by p - 7 bytes (item t = digit
-// byte p - 7 + t, zero outside 0 .. 6), so item t pairs digit byte u = p - 7 + t with key byte
-// v = 7 - t: u + v = p.  The 13 sums are recombined in FP64 from three exact chunks (positions 0-3,
-// 4-7 and 8-12, each below 2^47.1; see mfma_tile) with two exact modular products by 2^32 mod s, then
+// x = sum_{u<7} x_u 256^u with x_u in [-128, 128).  With the byte planes dn_u[b][i] and key_v[i][jc],
+//   acc = sum_p 256^p C_p,   C_p = sum_{u+v=p} key_v^T dn_u      (p = 0 .. 12, |C_p| <= 7 D 2^14 < 2^23),
+// 49 products of int8 matrices.  One v_mfma_i32_16x16x64_i8 multiplies a 16-column key tile by 16
+// ciphertexts over K = 64 digits (D <= 64): lane l holds key column / ciphertext l % 16 and the digits
+// 16 (l / 16) + t, t < 16, of one byte plane (16 bytes); output key column 4 (l / 16) + r, ciphertext
+// l % 16 in register r.  The 13 sums are recombined in FP64 from three exact chunks (positions 0-3, 4-7
+// and 8-12, each below 2^47.1; see mfma_tile) with two exact modular products by 2^32 mod s, then
 // reduced: the value the crate's per-prime products sum to, mod s.
 // ---------------------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -120,66 +115,78 @@ __device__ __forceinline__ int64_t f64_to_i64(double v) {
 __device__ __forceinline__ uint64_t sbyte_word(int64_t x) { return ((uint64_t)x + kByteBias) ^ kByteBias; }
 __device__ __forceinline__ int64_t sbyte_value(uint64_t w) { return (int64_t)((w ^ kByteBias) - kByteBias); }
 
+// 4 x 4 byte transpose (v_perm_b32 picks bytes of {src0 : src1}, src1 the low word): y_u byte i = x_i byte u
+__device__ __forceinline__ void byte_transpose4(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t (&y)[4]) {
+    const uint32_t t0 = __builtin_amdgcn_perm(x1, x0, 0x05010400u);  // x0.b0 x1.b0 x0.b1 x1.b1
+    const uint32_t t1 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);  // x0.b2 x1.b2 x0.b3 x1.b3
+    const uint32_t t2 = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
+    const uint32_t t3 = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+    y[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+    y[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+    y[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+    y[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
+
 struct MfmaArgs {
-    const uint64_t* dsb;  // [2][N][B][Dp] digit words (slot-major)
-    const uint64_t* key;  // [2][N][units] key operand units of 16 bytes (two u64)
-    double* acc;          // [G][2][N][B cols] accumulators (slot-major, reduced)
-    double* trash;        // 256 doubles: the store target of lanes without an output
-    int n, B, Dp, kbk, cols, D, gsize;
+    const uint64_t* dsb;   // [2][N][B][Dp] digit words (slot-major)
+    const uint8_t* key;    // [2][N][7 cols Dk] key byte planes (Dk = digits rounded up to 8), then 64 bytes of padding
+    double* acc;           // [G][2][N][B cols] accumulators (slot-major, reduced)
+    double* trash;         // 256 doubles: the store target of lanes without an output
+    int n, B, Dp, Dk, cols, D, gsize;
     double q[2], qinv[2], c32[2];  // the auxiliary primes, 1/s, 2^32 mod s (centred)
 };
 
-// One step: the 16 x 16 output tile (M-tile of ciphertexts, N-tile of key columns) of one slot, all 13
-// positions accumulated K-block by K-block.  refill(kb) runs right after K-block kb's last product: it
-// loads the next step's operands into the registers that block no longer needs, so every load has about
-// one step to arrive and no second buffer is held.  MASK: digits outside [i0, i1) (a digit group of the
-// gated recomputation) are zeroed as they are used (lane constants in0 / in1 per K-block).
+// The digit planes of the lane's 16 digits from their 16 words w (two per v2u): plane u, word k holds
+// byte u of digits 4k .. 4k + 3.  Digits outside [i0, i1) (index d0 + t) are zeroed first: those past the
+// ciphertext's digits (the key's bytes there are other data) and, with several groups, the other groups'.
+__device__ __forceinline__ void digit_planes(const v2u (&w)[8], v4i (&dp)[7], int d0, int i0, int i1) {
+    uint32_t lo[16], hi[16];
+#pragma unroll
+    for (int h = 0; h < 8; h++) {
+        uint64_t a = w[h].x, b = w[h].y;
+        const int e = d0 + 2 * h;
+        a = e >= i0 && e < i1 ? a : 0ull;
+        b = e + 1 >= i0 && e + 1 < i1 ? b : 0ull;
+        lo[2 * h] = (uint32_t)a;
+        hi[2 * h] = (uint32_t)(a >> 32);
+        lo[2 * h + 1] = (uint32_t)b;
+        hi[2 * h + 1] = (uint32_t)(b >> 32);
+    }
+    uint32_t P[8][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t y[4];
+        byte_transpose4(lo[4 * k], lo[4 * k + 1], lo[4 * k + 2], lo[4 * k + 3], y);
+#pragma unroll
+        for (int u = 0; u < 4; u++) P[u][k] = y[u];
+        byte_transpose4(hi[4 * k], hi[4 * k + 1], hi[4 * k + 2], hi[4 * k + 3], y);
+#pragma unroll
+        for (int u = 0; u < 4; u++) P[4 + u][k] = y[u];  // plane 7 is zero and unused
+    }
+#pragma unroll
+    for (int u = 0; u < 7; u++) dp[u] = v4i{(int)P[u][0], (int)P[u][1], (int)P[u][2], (int)P[u][3]};
+}
+
+// One step: the 16 x 16 output tile (N-tile of key columns, M-tile of ciphertexts) of one slot: the 49
+// plane products accumulated into 13 positions, key plane by key plane.  refill(v) runs right after key
+// plane v's last product: it loads the next step's plane into the registers this one no longer needs,
+// so every load has about one step to arrive and no second buffer is held.
 // Recombination: the chunks H_low (positions 0-3), H_mid (4-7) and H_top (8-12) are exact doubles.  Byte
 // 6 of a residue below 2^50.1 (centred) is at most 3 in size, so |C_12| <= 9 D and every chunk stays
-// below D 2^41.03 <= 2^47.03 (D <= 64); V = (H_top c + H_mid) c + H_low mod s, c = 2^32 mod s.
-template <int KB, bool MASK, class Refill>
-__device__ __forceinline__ void mfma_tile(v2u (&a)[KB], v4i (&bk)[KB], double (&V)[4], double q, double qinv,
-                                          double c32, int kb0, int gq, int i0, int i1, Refill refill) {
+// below D 2^40.9 <= 2^46.9 (D <= 64); V = (H_top c + H_mid) c + H_low mod s, c = 2^32 mod s.
+template <class Refill>
+__device__ __forceinline__ void mfma_tile(const v4i (&dp)[7], v4i (&kp)[7], double (&V)[4], double q, double qinv,
+                                          double c32, Refill refill) {
     v4i c[13];
 #pragma unroll
     for (int p = 0; p < 13; p++) c[p] = v4i{0, 0, 0, 0};
 #pragma unroll
-    for (int kb = 0; kb < KB; kb++) {
-        uint64_t d0 = a[kb].x, d1 = a[kb].y;
-        // opaque here: the shifted operands are formed per use, not held across N-tiles (registers)
-        asm volatile("" : "+v"(d0), "+v"(d1));
-        if constexpr (MASK) {
-            const int e0 = 8 * (kb0 + kb) + 2 * gq;
-            d0 = e0 >= i0 && e0 < i1 ? d0 : 0ull;
-            d1 = e0 + 1 >= i0 && e0 + 1 < i1 ? d1 : 0ull;
-        }
-        // The operand of position p = 4k + r is bits [8p, 8p + 64) of X = d 2^56 for both digits, as the
-        // words [d0 lo, d1 lo, d0 hi, d1 hi] (the lane's item order).  With Y[2m + h] = word m of digit h's
-        // X (words 0, 4, 5 are zero) and Z_r[2m + h] = alignbyte(Y[2m + 2 + h], Y[2m + h], r), that operand
-        // is Y[2k .. 2k+3] (r = 0) or Z_r[2k .. 2k+3]: 15 shifts per digit for all 13 operands.
-        const uint32_t l0 = (uint32_t)d0, h0 = (uint32_t)(d0 >> 32), l1 = (uint32_t)d1, h1 = (uint32_t)(d1 >> 32);
-        const uint32_t Y[12] = {0u,      0u,      l0 << 24, l1 << 24, __builtin_amdgcn_alignbyte(h0, l0, 1u),
-                                __builtin_amdgcn_alignbyte(h1, l1, 1u), h0 >> 8, h1 >> 8, 0u, 0u, 0u, 0u};
-#define RCK_MM4(W, K) v4i{(int)W[2 * K], (int)W[2 * K + 1], (int)W[2 * K + 2], (int)W[2 * K + 3]}
-        c[0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 0), c[0], 0, 0, 0);
-        c[4] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 1), c[4], 0, 0, 0);
-        c[8] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 2), c[8], 0, 0, 0);
-        c[12] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Y, 3), c[12], 0, 0, 0);
-        __builtin_amdgcn_sched_barrier(0);
+    for (int v = 0; v < 7; v++) {
 #pragma unroll
-        for (int r = 1; r < 4; r++) {
-            uint32_t Z[8];
-#pragma unroll
-            for (int m = 0; m < 8; m++) Z[m] = __builtin_amdgcn_alignbyte(Y[m + 2], Y[m], (uint32_t)r);
-            c[r] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Z, 0), c[r], 0, 0, 0);
-            c[4 + r] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Z, 1), c[4 + r], 0, 0, 0);
-            c[8 + r] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bk[kb], RCK_MM4(Z, 2), c[8 + r], 0, 0, 0);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-#undef RCK_MM4
-        refill(kb);
-        // keeps the refill loads here (the compiler would otherwise sink them to their use); the barrier
-        // reads no register, so it adds no wait
+        for (int u = 0; u < 7; u++) c[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(kp[v], dp[u], c[u + v], 0, 0, 0);
+        refill(v);
+        // keeps the refill load here (the compiler would otherwise sink it to its use); the barrier reads
+        // no register, so it adds no wait
         asm volatile("" ::: "memory");
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -194,8 +201,8 @@ __device__ __forceinline__ void mfma_tile(v2u (&a)[KB], v4i (&bk)[KB], double (&
         double low = (double)c[3][r];
 #pragma unroll
         for (int p = 2; p >= 0; p--) low = __builtin_fma(low, 256.0, (double)c[p][r]);
-        const double v = mulmod(top, c32, q, qinv) + mid;
-        V[r] = reduce(mulmod(v, c32, q, qinv) + low, q, qinv);
+        const double t = mulmod(top, c32, q, qinv) + mid;
+        V[r] = reduce(mulmod(t, c32, q, qinv) + low, q, qinv);
         __builtin_amdgcn_sched_barrier(0);  // one output register at a time: 13 sums read, not 52
     }
 }
@@ -205,45 +212,47 @@ __device__ __forceinline__ void mfma_tile(v2u (&a)[KB], v4i (&bk)[KB], double (&
 // 16 nt + 4 (l / 16) + r: 32 contiguous bytes of the [b][col] output, two 16-byte stores.
 // No memory operation of the loops is conditional, so the in-order memory counter lets every wait cover
 // exactly the loads it needs:
-//  * each step's refill loads the next N-tile's key operands; the last N-tile's refill also loads the
-//    next (slot, M-tile)'s digit words (the digit words of a slot are loaded once, not per N-tile); the
-//    last step reloads its own operands (valid addresses, never used);
-//  * key columns past the last tile's width read its last column (those products are not stored), and
-//    K-blocks past the key's (a group's span in the gated recomputation) read its last block, against
-//    digits that MASK zeroes;
+//  * each step's refills load the next N-tile's key planes; the last N-tile's also load the next
+//    (slot, M-tile)'s digit words (a slot's digit words are loaded once); the last step reloads its own
+//    operands (valid addresses, never used);
+//  * key columns past the last tile's width read its last column (those products are not stored);
+//    digit words past D read the row's last words, and the digits outside the group (past D included)
+//    are zeroed when the planes are formed, so the key bytes they meet (other data or padding) add nothing;
 //  * a step's stores are issued during the next step, after its first refill, and lanes with no output
 //    (padding columns, rows past the batch) store to a scratch line instead.
 // Each address is a wave-uniform base plus a lane offset fixed for the launch.
-template <int KB, bool MASK>
-__device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xbeg, int nslots, int g, int kb0, int i0,
-                                          int i1, int lane) {
+__device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xbeg, int nslots, int g, int i0, int i1,
+                                          int lane) {
     const int j = lane & 15, gq = lane >> 4;
     const int NT = (A.cols + 15) >> 4, MT = (A.B + 15) >> 4;
-    const int wl = A.cols - 16 * (NT - 1);         // width of the last N-tile
-    const long long units = 4LL * A.kbk * A.cols;  // 16-byte key units per slot
-    // lane byte offsets (32-bit) from wave-uniform bases: the loads take the scalar-base + vector-offset form
-    const uint32_t lk_full = (uint32_t)(gq * 16 + j) * 16u, lk_last = (uint32_t)(gq * wl + min(j, wl - 1)) * 16u;
+    const int wl = A.cols - 16 * (NT - 1);           // width of the last N-tile
+    const long long kslot = 7LL * A.cols * A.Dk;    // key bytes per slot
+    // lane byte offsets (32-bit) within a plane of an N-tile: column j (the last tile's columns past its
+    // width read its last column; those products are not stored), digits 16 gq .. 16 gq + 15 (past the key's
+    // digits the bytes are other data or padding, multiplied by zeroed digits)
+    const uint32_t lk_full = (uint32_t)(j * A.Dk + 16 * gq), lk_last = (uint32_t)(min(j, wl - 1) * A.Dk + 16 * gq);
     const double q = A.q[s], qinv = A.qinv[s], c32 = A.c32[s];
     const long long zc = (long long)A.B * A.cols;
+    const int d0 = 16 * gq;  // the lane's first digit
     typedef const __attribute__((address_space(1))) char gchar;
-    auto uniform_ptr = [](const char* p) -> gchar* {
+    auto uniform_ptr = [](const void* p) -> gchar* {
         const uint64_t v = (uint64_t)p;
         const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
         return (gchar*)(((uint64_t)hi << 32) | lo);
     };
-    auto load_k = [&](long long sl, int nt, int kb) -> v4i {
+    auto load_k = [&](long long sl, int nt, int v) -> v4i {
         const bool last = nt == NT - 1;
         const int w = last ? wl : 16;
-        const int kbc = min(kb0 + kb, A.kbk - 1);
-        const long long base = sl * units + 16LL * nt * A.kbk * 4 + (long long)kbc * 4 * w;
-        gchar* kp = uniform_ptr(reinterpret_cast<const char*>(A.key) + base * 16);
+        const long long base = sl * kslot + (7LL * 16 * nt + (long long)v * w) * A.Dk;
+        gchar* kp = uniform_ptr(A.key + base);
         return *(const __attribute__((address_space(1))) v4i*)(kp + (uint64_t)(last ? lk_last : lk_full));
     };
-    // digit words of ciphertext 16 mt + j (rows past the batch repeat the last one; they are not stored)
-    auto load_a = [&](long long sl, int mt, int kb) -> v2u {
-        const int u0 = min(8 * (kb0 + kb), A.Dp - 8);
-        gchar* dp = uniform_ptr(reinterpret_cast<const char*>(A.dsb) + ((sl * A.B + 16LL * mt) * A.Dp + u0) * 8);
-        const uint32_t la = (uint32_t)(min(j, A.B - 1 - 16 * mt) * A.Dp + 2 * gq) * 8u;
+    // digit words 2h, 2h + 1 of the lane's 16 (ciphertext 16 mt + j; rows past the batch repeat the last one,
+    // they are not stored)
+    auto load_w = [&](long long sl, int mt, int h) -> v2u {
+        gchar* dp = uniform_ptr(A.dsb + (sl * A.B + 16LL * mt) * A.Dp);
+        const int e = min(d0 + 2 * h, A.Dp - 2);
+        const uint32_t la = (uint32_t)(min(j, A.B - 1 - 16 * mt) * A.Dp + e) * 8u;
         return *(const __attribute__((address_space(1))) v2u*)(dp + (uint64_t)la);
     };
     // output addresses of a step: ciphertext 16 mt + j, columns 16 nt + 4 gq + 2 pr + {0, 1} (cols is even,
@@ -260,17 +269,18 @@ __device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xb
     const int groups = nslots * MT;
     long long sl = (long long)s * A.n + xbeg;
     int mt = 0;
-    v2u a[KB];
-    v4i bk[KB];
-    // first step's operands in K-block order, as the refills issue them
+    v2u wb[8];  // digit words of the next (slot, M-tile)
+    v4i kp[7], dp[7];
+    // first step's operands, in the order the refills issue them
 #pragma unroll
-    for (int kb = 0; kb < KB; kb++) {
-        bk[kb] = load_k(sl, 0, kb);
-        a[kb] = load_a(sl, 0, kb);
+    for (int v = 0; v < 7; v++) {
+        kp[v] = load_k(sl, 0, v);
         asm volatile("" ::: "memory");
         __builtin_amdgcn_sched_barrier(0);
     }
-    v2d Vp[2];     // the previous step's outputs, stored during this step (the first step stores zeros to
+#pragma unroll
+    for (int h = 0; h < 8; h++) wb[h] = load_w(sl, 0, h);
+    v2d Vp[2];      // the previous step's outputs, stored during this step (the first step stores zeros to
     double* op[2];  // the scratch line)
 #pragma unroll
     for (int pr = 0; pr < 2; pr++) {
@@ -299,18 +309,22 @@ __device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xb
             mt1 = mt;
             sl1 = sl;
         }
+        digit_planes(wb, dp, d0, i0, i1);
         double V[4];
         for (int nt = 0; nt < NT - 1; nt++) {
-            mfma_tile<KB, MASK>(a, bk, V, q, qinv, c32, kb0, gq, i0, i1, [&](int kb) {
-                bk[kb] = load_k(sl, nt + 1, kb);
-                if (kb == 0) store_prev();
+            mfma_tile(dp, kp, V, q, qinv, c32, [&](int v) {
+                kp[v] = load_k(sl, nt + 1, v);
+                if (v == 0) store_prev();
             });
             keep(V, sl, mt, nt);
         }
-        mfma_tile<KB, MASK>(a, bk, V, q, qinv, c32, kb0, gq, i0, i1, [&](int kb) {
-            bk[kb] = load_k(sl1, 0, kb);
-            a[kb] = load_a(sl1, mt1, kb);
-            if (kb == 0) store_prev();
+        mfma_tile(dp, kp, V, q, qinv, c32, [&](int v) {
+            kp[v] = load_k(sl1, 0, v);
+            if (v == 0) {
+                store_prev();
+#pragma unroll
+                for (int h = 0; h < 8; h++) wb[h] = load_w(sl1, mt1, h);
+            }
         });
         keep(V, sl, mt, NT - 1);
         sl = sl1;
@@ -320,9 +334,7 @@ __device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xb
 }
 
 // Work item (digit group g, prime s, block of 4 spw slots); each of the four waves takes spw consecutive
-// slots.  gate (nullable, device): the launch does nothing unless *gate != 0.  MASK: digit groups that do
-// not cover all digits.
-template <int KB, bool MASK>
+// slots.  gate (nullable, device): the launch does nothing unless *gate != 0.
 __global__ __launch_bounds__(256) void k_ks_mac_mfma(MfmaArgs A, int spw, int nxb, const int* __restrict__ gate) {
     if (gate != nullptr && *gate == 0) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -331,11 +343,11 @@ __global__ __launch_bounds__(256) void k_ks_mac_mfma(MfmaArgs A, int spw, int nx
     r /= nxb;
     const int s = r & 1, g = r >> 1;
     const int i0 = g * A.gsize, i1 = min(A.D, i0 + A.gsize);
-    mfma_wave<KB, MASK>(A, s, ((long long)xb * 4 + wave) * spw, spw, g, i0 >> 3, i0, i1, lane);
+    mfma_wave(A, s, ((long long)xb * 4 + wave) * spw, spw, g, i0, i1, lane);
 }
 
-static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, double* trash,
-                          int B, int D, int Dp, int kbk, int cols, int gsize) {
+static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint8_t* key, double* acc, double* trash,
+                          int B, int D, int Dp, int Dk, int cols, int gsize) {
     MfmaArgs A{};
     A.dsb = dsb;
     A.key = key;
@@ -344,7 +356,7 @@ static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint64_t*
     A.n = (int)c.n;
     A.B = B;
     A.Dp = Dp;
-    A.kbk = kbk;
+    A.Dk = Dk;
     A.cols = cols;
     A.D = D;
     A.gsize = gsize;
@@ -358,41 +370,20 @@ static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint64_t*
     return A;
 }
 
+// acc: G groups of [2][N][B cols]; trash: 256 doubles of scratch; kdig: the key's digits (its K-groups);
 // spw slots per wave (1, 2 or 4); digits in groups of gsize (G groups); gate as in k_ks_mac_mfma
-// acc: G groups of [2][N][B cols]; trash: 256 doubles of scratch
-static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint64_t* key, double* acc, double* trash,
-                            int B, int D, int Dp, int kbk, int cols, int gsize, int G, int spw, hipStream_t s,
+static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint8_t* key, int kdig, double* acc,
+                            double* trash, int B, int D, int Dp, int cols, int gsize, int G, int spw, hipStream_t s,
                             const int* gate = nullptr) {
     RCK_REQUIRE(spw == 1 || spw == 2 || spw == 4, "bad exact keyswitch shape");
-    RCK_REQUIRE(c.n % (4 * spw) == 0 && B >= 1 && 8 * kbk >= D && Dp >= 8 * ((D + 7) / 8), "bad exact keyswitch size");
-    // K-blocks one group spans (uniform over the groups' starts)
-    int span = 1;
-    for (int g = 0; g < G; g++) {
-        const int i0 = g * gsize, i1 = std::min(D, i0 + gsize);
-        if (i1 > i0) span = std::max(span, (i1 - 1) / 8 - i0 / 8 + 1);
-    }
-    const MfmaArgs A = mfma_args(c, dsb, key, acc, trash, B, D, Dp, kbk, cols, gsize);
+    RCK_REQUIRE(c.n % (4 * spw) == 0 && B >= 1 && D <= kdig && kdig <= 64 && Dp >= 2 && Dp % 2 == 0 && cols % 2 == 0,
+                "bad exact keyswitch size");
+    const MfmaArgs A = mfma_args(c, dsb, key, acc, trash, B, D, Dp, 8 * ((kdig + 7) / 8), cols, gsize);
     const int nxb = (int)(c.n / (4 * spw));
     const long long blocks = (long long)G * 2 * nxb;
     RCK_REQUIRE(blocks < (1LL << 31), "exact keyswitch launch too large");
-    // one group over every digit needs no mask: the digit rows are zero past D, and so is the key past its digits
-    const bool mask = G > 1;
     ProfScope ps(s, gate ? "ks_mac_gated" : "ks_mac");
-    switch (span * 2 + (mask ? 1 : 0)) {
-#define RCK_MM(K)                                                                                      \
-    case 2 * K: k_ks_mac_mfma<K, false><<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate); break; \
-    case 2 * K + 1: k_ks_mac_mfma<K, true><<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate); break;
-        RCK_MM(1)
-        RCK_MM(2)
-        RCK_MM(3)
-        RCK_MM(4)
-        RCK_MM(5)
-        RCK_MM(6)
-        RCK_MM(7)
-        RCK_MM(8)
-#undef RCK_MM
-        default: RCK_REQUIRE(false, "exact keyswitch supports at most 64 digits per group");
-    }
+    k_ks_mac_mfma<<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate);
     RCK_CHECK_HIP(hipGetLastError());
 }
 
@@ -520,46 +511,43 @@ static void launch_rows_sm(const Context& c, bool fwd, const double* din, uint64
 }
 
 // ---------------------------------------------------------------------------------------
-// The key's matrix-core form: per slot (s, x), units of 16 bytes in the order
-//   N-tile nt (16 key columns; the last one w = cols - 16 nt wide), K-block kb, lane group gq, column j,
-// unit = key digits 8 kb + 2 gq and 8 kb + 2 gq + 1 of column 16 nt + j, each a signed-byte word with
-// its bytes reversed.  Digits past the key's are zero.  An operand load of a wave is then 4 w
-// consecutive units.
+// The key's matrix-core form: byte planes.  Per slot (s, x), records of Dk bytes (the key's digits
+// rounded up to 8; a lane's 16-byte operand load is 8-byte aligned, which the hardware serves) in the order N-tile nt (16 key columns; the last one w = cols - 16 nt wide), byte
+// plane v (7), column j: record = byte v of every key digit of column 16 nt + j (balanced signed bytes;
+// the rounding digits are zero).  A lane's operand (16 digits of one plane) is 16 consecutive bytes of a
+// record, so a wave's load covers the w records of one plane.  64 bytes of padding follow the last slot
+// (a lane's 16 bytes may run past the last record).
 // ---------------------------------------------------------------------------------------
-// 32-bit word index of the low half of digit d's (byte-reversed) key word in its unit; the high half is
-// two words further (unit word order [digit 2 gq lo, digit 2 gq + 1 lo, digit 2 gq hi, digit 2 gq + 1 hi])
-__device__ __forceinline__ long long key_unit_dword(int n, int cols, int kbk, int s, long long x, int col, int d) {
+size_t key_mfma_bytes(const Context& c, int digits, int cols) {
+    return (size_t)2 * c.n * 7 * cols * (8 * ((digits + 7) / 8)) + 64;
+}
+
+__device__ __forceinline__ long long key_plane_byte(int n, int cols, int Dk, int s, long long x, int col, int d, int v) {
     const int nt = col >> 4, j = col & 15, w = min(16, cols - 16 * nt);
-    const int kb = d >> 3, gq = (d & 7) >> 1, h = d & 1;
-    const long long units = 4LL * kbk * cols;
-    const long long u = 16LL * nt * kbk * 4 + (long long)(kb * 4 + gq) * w + j;
-    return (((long long)s * n + x) * units + u) * 4 + h;
+    return ((long long)s * n + x) * 7LL * cols * Dk + (7LL * 16 * nt + (long long)v * w + j) * Dk + d;
 }
 
-// digit d of the key: stage [cols][2][N] (the NTT forms under s1, s2, reduced) -> its words in key
-__global__ void k_key_digit_to_mfma(const double* __restrict__ stage, uint64_t* __restrict__ key, int n, int cols,
-                                    int kbk, int d) {
+// digit d of the key: stage [cols][2][N] (the NTT forms under s1, s2, reduced) -> its bytes in the planes
+__global__ void k_key_digit_to_mfma(const double* __restrict__ stage, uint8_t* __restrict__ key, int n, int cols,
+                                    int Dk, int d) {
     const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
     const int col = blockIdx.y, s = blockIdx.z;
-    const double v = stage[((long long)col * 2 + s) * n + x];
-    const uint64_t w = __builtin_bswap64(sbyte_word(f64_to_i64(v)));
-    uint32_t* k32 = reinterpret_cast<uint32_t*>(key);
-    const long long i = key_unit_dword(n, cols, kbk, s, x, col, d);
-    k32[i] = (uint32_t)w;
-    k32[i + 2] = (uint32_t)(w >> 32);
+    const uint64_t w = sbyte_word(f64_to_i64(stage[((long long)col * 2 + s) * n + x]));
+#pragma unroll
+    for (int v = 0; v < 7; v++) key[key_plane_byte(n, cols, Dk, s, x, col, d, v)] = (uint8_t)(w >> (8 * v));
 }
 
-// the inverse: digit d's words -> stage [cols][2][N] (centred doubles)
-__global__ void k_key_digit_from_mfma(const uint64_t* __restrict__ key, double* __restrict__ stage, int n, int cols,
-                                      int kbk, int d) {
+// the inverse: digit d's bytes -> stage [cols][2][N] (centred doubles)
+__global__ void k_key_digit_from_mfma(const uint8_t* __restrict__ key, double* __restrict__ stage, int n, int cols,
+                                      int Dk, int d) {
     const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
     const int col = blockIdx.y, s = blockIdx.z;
-    const uint32_t* k32 = reinterpret_cast<const uint32_t*>(key);
-    const long long i = key_unit_dword(n, cols, kbk, s, x, col, d);
-    const uint64_t w = (uint64_t)k32[i] | ((uint64_t)k32[i + 2] << 32);
-    stage[((long long)col * 2 + s) * n + x] = (double)sbyte_value(__builtin_bswap64(w));
+    uint64_t w = 0;
+#pragma unroll
+    for (int v = 0; v < 7; v++) w |= (uint64_t)key[key_plane_byte(n, cols, Dk, s, x, col, d, v)] << (8 * v);
+    stage[((long long)col * 2 + s) * n + x] = (double)sbyte_value(w);
 }
 
 // Inverse column pass of the accumulators fused with the recombination: a workgroup transforms the
@@ -809,7 +797,7 @@ __global__ __launch_bounds__(1024) void k_exact_gate(const double* __restrict__ 
 bool exact_keyswitch_available(const Context& c, uint32_t aux) { return c.S == 2 && aux > 0 && c.logn >= 12; }
 
 // Converts a key from its NTT form mod the eval primes (data) to the exact-convolution form (mdata, the
-// matrix-core words of its centred limbs' NTT forms under s1 and s2); data is released unless keep_data.
+// matrix-core byte planes of its centred limbs' NTT forms under s1 and s2); data is released unless keep_data.
 void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
     require_fp64(c);
     RCK_REQUIRE(k && k->data, "null key");
@@ -817,15 +805,15 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
     RCK_REQUIRE(k->digits <= 64, "exact-convolution keys hold at most 64 digits");
     const long long n = c.n;
     const int E = (int)k->limbs, nq = (int)k->level + 1, cols = 2 * E;
-    const int kbk = ((int)k->digits + 7) / 8;
+    const int Dk = 8 * (((int)k->digits + 7) / 8);
     const long long P = (long long)E * n;
-    const size_t bytes = (size_t)2 * n * 4 * kbk * cols * 16;
-    uint64_t* md = nullptr;
+    const size_t bytes = key_mfma_bytes(c, (int)k->digits, cols);
+    uint8_t* md = nullptr;
     uint64_t* coeff = nullptr;
     double* stage = nullptr;  // 2 P doubles (inverse NTT input), then 4 P (the centred limbs mod s1, s2)
     try {
         RCK_CHECK_HIP(hipMalloc(&md, bytes));
-        RCK_CHECK_HIP(hipMemset(md, 0, bytes));  // digits past the key's stay zero
+        RCK_CHECK_HIP(hipMemset(md, 0, bytes));  // the rounding digits and the padding stay zero
         RCK_CHECK_HIP(hipMalloc(&coeff, (size_t)2 * P * 8));
         RCK_CHECK_HIP(hipMalloc(&stage, (size_t)4 * P * 8));
         const PrimeMap pm{nq, (int)c.L, 0};
@@ -837,7 +825,7 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
                                                                     (int)(c.L + c.K), c.d_primes);
             RCK_CHECK_HIP(hipGetLastError());
             launch_fwd(c, kInF64, stage, n, 2 * n, stage, n, 2 * n, 2, cols, ps, nullptr);
-            k_key_digit_to_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(stage, md, (int)c.n, cols, kbk, (int)d);
+            k_key_digit_to_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(stage, md, (int)c.n, cols, Dk, (int)d);
             RCK_CHECK_HIP(hipGetLastError());
         }
         RCK_CHECK_HIP(hipDeviceSynchronize());
@@ -860,14 +848,14 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
 void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out) {
     const long long n = c.n;
     const int E = (int)k->limbs, nq = (int)k->level + 1, cols = 2 * E;
-    const int kbk = ((int)k->digits + 7) / 8;
+    const int Dk = 8 * (((int)k->digits + 7) / 8);
     const long long P = (long long)E * n;
     double* stage = nullptr;
     uint64_t* coeff = nullptr;
     try {
         RCK_CHECK_HIP(hipMalloc(&stage, (size_t)2 * P * 2 * 8));
         RCK_CHECK_HIP(hipMalloc(&coeff, (size_t)2 * P * 2 * 8));
-        k_key_digit_from_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(k->mdata, stage, (int)c.n, cols, kbk,
+        k_key_digit_from_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(k->mdata, stage, (int)c.n, cols, Dk,
                                                                          (int)d);
         RCK_CHECK_HIP(hipGetLastError());
         launch_inv(c, stage, n, 2 * n, kOutU64, coeff, n, 2 * n, 2, cols, PrimeMap{0, (int)(c.L + c.K), 0}, nullptr);
@@ -893,7 +881,6 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     require_fp64(c);
     const long long n = c.n;
     const int E = limbs + (int)key->aux, cols = 2 * E;
-    const int kbk = ((int)key->digits + 7) / 8;
     const int Dp = 8 * ((levels + 7) / 8);
     RCK_REQUIRE(levels <= (int)key->digits && cols == 2 * (int)key->limbs, "exact keyswitch key does not fit");
     const ExactPlan pl = exact_plan(c, limbs, levels);
@@ -925,7 +912,7 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     launch_fwd_cols(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
     launch_rows_sm(c, true, dn, dsb, nullptr, nullptr, B * levels, levels, Dp, B, 1, s);
     double* trash = accm + (size_t)Gw * 2 * n * Zc;
-    launch_mac_mfma(c, dsb, key->mdata, accm, trash, B, levels, Dp, kbk, cols, pl.gsize, G, spw, s);
+    launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, trash, B, levels, Dp, cols, pl.gsize, G, spw, s);
     // inverse rows under s1, s2 (back to limb rows); inverse columns fused with the recombination (one
     // value per accumulator and coefficient: the group's exact sum mod p_j); then the group sum and
     // mod-down (skipped when the gate is set)
@@ -934,7 +921,8 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s, gate, 0);
     if (pl.gated) {
         // some ciphertext's digits exceed the one-group bound: the batch again with the worst-case groups
-        launch_mac_mfma(c, dsb, key->mdata, accm, trash, B, levels, Dp, kbk, cols, pl.gsw, Gw, spw, s, gate);
+        launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, trash, B, levels, Dp, cols, pl.gsw, Gw, spw, s,
+                        gate);
         launch_rows_sm(c, false, nullptr, nullptr, accm, acc, Zc, 0, 0, B, Gw, s, gate);
         launch_inv_cols_crt(c, acc, accq, limbs, Gw * Zc, s, gate);
         launch_crt_moddown(c, accq, limbs, B, Gw, add0, add1, add_ps, ginv, outs, s, gate, 1);

@@ -9,9 +9,9 @@ namespace rck {
 struct KeyDev {
     double* data = nullptr;
     // exact-convolution form (hybrid keys, N >= 2^12): the NTT forms under the two auxiliary primes of the
-    // key's centred limbs, as the matrix-core operand words of ks_exact.hip (data is null when the key
+    // key's centred limbs, as the matrix-core byte planes of ks_exact.hip (data is null when the key
     // holds this form only)
-    uint64_t* mdata = nullptr;
+    uint8_t* mdata = nullptr;
     // 64-bit integer contexts instead: wdata[digit][comp][limb][N], NTT form times R = 2^64 (wide.hpp)
     uint64_t* wdata = nullptr;
     uint32_t digits = 0, level = 0, aux = 0, limbs = 0;
