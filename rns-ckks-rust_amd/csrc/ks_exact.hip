@@ -555,12 +555,16 @@ __global__ void k_key_digit_from_mfma(const uint8_t* __restrict__ key, double* _
 // (each exactly as k_inv_cols, with the N^{-1} scaling) and writes V mod p_j, V the group's exact sum
 // (crt_mod_q, |V mod p_j| <= p_j/2 + 1 as a double).  acc: [Z][2][N] after the inverse row pass,
 // Z = G B 2E; out: [Z][N].  Half the bytes of writing both residues for the recombination to read back.
+// addr (nullable): for group 0 and the Q limbs j < nq, the addend d_comp limb j of ciphertext b (after the
+// inverse row stages: addr + b addr_ps + (comp nq + j) N) gets the same inverse column pass under p_j and is
+// added times P (pmodq[j] = P mod p_j): mod-down by P then returns keyswitch + d_comp.
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_inv_cols_crt(const double* __restrict__ acc, double* __restrict__ out,
                                                       int logn, int E, int nq, int L, int sbase, SpecConst SK,
                                                       const double* __restrict__ spmod,
                                                       const DevPrime* __restrict__ primes,
-                                                      const int* __restrict__ gate) {
+                                                      const int* __restrict__ gate, const double* __restrict__ addr,
+                                                      long long addr_ps, int B, const double* __restrict__ pmodq) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     __shared__ double lds[R * S];
     if (gate != nullptr && *gate == 0) return;
@@ -568,52 +572,70 @@ __global__ __launch_bounds__(256) void k_inv_cols_crt(const double* __restrict__
     const int tid = threadIdx.x, col = tid % S, tau = tid / S;
     const int r = blockIdx.x * S + col;
     const long long z = blockIdx.y;
-    const int j = (int)(z % (2 * E)) % E;
+    const int jc = (int)(z % (2 * E)), comp = jc / E, j = jc % E;
     const int pj = prime_of(j, nq, L);
-    // both primes' columns are loaded up front (the second set is in flight during the first transform)
-    double a1[16], a2[16];
+    // both primes' columns, and the addend's, are loaded up front (in flight during the first transform)
+    const int g = (int)(z / (2LL * E * B));
+    const bool fold = addr != nullptr && g == 0 && j < nq;
+    double a1[16], a2[16], a3[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) a1[e] = acc[(z * 2) * n + r + (long long)N2 * (16 * tau + e)];
 #pragma unroll
     for (int e = 0; e < 16; e++) a2[e] = acc[(z * 2 + 1) * n + r + (long long)N2 * (16 * tau + e)];
-    auto column = [&](int s, double (&v)[16]) {
-        const DevPrime Ps = primes[sbase + s];
+    if (fold) {
+        const int b = (int)((z / (2 * E)) % B);
+        const double* dsrc = addr + (long long)b * addr_ps + ((long long)comp * nq + j) * n;
+#pragma unroll
+        for (int e = 0; e < 16; e++) a3[e] = dsrc[r + (long long)N2 * (16 * tau + e)];
+    }
+    auto column = [&](const DevPrime& Ps, double (&v)[16]) {
         inv_phase_b<LOGR>(v, Ps.iv, 1, tau, Ps.q, Ps.qinv);
-        // LDS-only barriers: the second prime's loads stay in flight across the first exchange
-        if (s) lds_barrier();  // the first transform's reads of lds are done
+        // LDS-only barriers: the next column's loads stay in flight across this exchange
+        lds_barrier();  // the previous transform's reads of lds are done
 #pragma unroll
         for (int e = 0; e < 16; e++) lds[(16 * tau + e) * S + col] = v[e];
         lds_barrier();
 #pragma unroll
         for (int e = 0; e < 16; e++) v[e] = lds[(tau + TPS * e) * S + col];
         inv_phase_a<kTwScalar>(v, Ps.iv, 1, Ps.q, Ps.qinv);
-#pragma unroll
-        for (int e = 0; e < 16; e++) v[e] = canon(mulmod(v[e], Ps.ninv, Ps.q, Ps.qinv), Ps.q, Ps.qinv);
     };
-    column(0, a1);
-    column(1, a2);
+    const DevPrime P1 = primes[sbase], P2 = primes[sbase + 1];
+    column(P1, a1);
+#pragma unroll
+    for (int e = 0; e < 16; e++) a1[e] = canon(mulmod(a1[e], P1.ninv, P1.q, P1.qinv), P1.q, P1.qinv);
+    column(P2, a2);
+#pragma unroll
+    for (int e = 0; e < 16; e++) a2[e] = canon(mulmod(a2[e], P2.ninv, P2.q, P2.qinv), P2.q, P2.qinv);
     const DevPrime Pj = primes[pj];
+#pragma unroll
+    for (int e = 0; e < 16; e++) a1[e] = crt_mod_q(a1[e], a2[e], SK, Pj.q, Pj.qinv, spmod[2 * pj], spmod[2 * pj + 1]);
+    if (fold) {
+        column(Pj, a3);
+        const double pm = reduce(mulmod(Pj.ninv, pmodq[j], Pj.q, Pj.qinv), Pj.q, Pj.qinv);  // N^{-1} P mod p_j
+#pragma unroll
+        for (int e = 0; e < 16; e++) a1[e] = reduce(a1[e] + mulmod(a3[e], pm, Pj.q, Pj.qinv), Pj.q, Pj.qinv);
+    }
     double* o = out + z * n;
 #pragma unroll
-    for (int e = 0; e < 16; e++)
-        o[r + (long long)N2 * (tau + TPS * e)] =
-            crt_mod_q(a1[e], a2[e], SK, Pj.q, Pj.qinv, spmod[2 * pj], spmod[2 * pj + 1]);
+    for (int e = 0; e < 16; e++) o[r + (long long)N2 * (tau + TPS * e)] = a1[e];
 }
 
-static void launch_inv_cols_crt(const Context& c, const double* acc, double* out, int nq, int Z, hipStream_t s,
-                                const int* gate = nullptr) {
+static void launch_inv_cols_crt(const Context& c, const double* acc, double* out, int nq, int B, int G, hipStream_t s,
+                                const int* gate = nullptr, const double* addr = nullptr, long long addr_ps = 0) {
     const int E = nq + (int)c.K;
     const SpecConst SK = spec_const(c);
     int n1, n2;
     split_logn((int)c.logn, &n1, &n2);
     const int S = 256 / ((1 << n1) / 16);
+    const long long Z = (long long)G * B * 2 * E;
+    RCK_REQUIRE(Z <= 65535, "exact keyswitch batch too large for one recombination launch");
     const dim3 grid((unsigned)((1 << n2) / S), (unsigned)Z);
     ProfScope prof(s, gate ? "inv_cols_crt_gated" : "inv_cols_crt");
     switch (n1) {
 #define RCK_IC(LR)                                                                                           \
     case LR:                                                                                                 \
         k_inv_cols_crt<LR><<<grid, 256, 0, s>>>(acc, out, (int)c.logn, E, nq, (int)c.L, (int)(c.L + c.K), SK, \
-                                                c.d_spmod, c.d_primes, gate);                                \
+                                                c.d_spmod, c.d_primes, gate, addr, addr_ps, B, c.d_pmodq);   \
         break;
         RCK_IC(6)
         RCK_IC(7)
@@ -873,11 +895,11 @@ void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* o
 }
 
 // Keyswitch of B polynomials (stride in_ps) at `limbs` Q limbs with an exact-convolution key, `levels`
-// digits; adds add0/add1 and writes outs[b] = [c0][c1] (keyswitch_batch's contract).  spw: slots per
-// wave of the multiply-accumulate.
+// digits; adds add0/add1 (or the row-form addends addr) and writes outs[b] = [c0][c1] (keyswitch_batch's
+// contract).  spw: slots per wave of the multiply-accumulate.
 void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, unsigned long long ginv, int spw) {
+                     hipStream_t s, unsigned long long ginv, int spw, const double* addr, long long addr_ps) {
     require_fp64(c);
     const long long n = c.n;
     const int E = limbs + (int)key->aux, cols = 2 * E;
@@ -917,14 +939,14 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     // value per accumulator and coefficient: the group's exact sum mod p_j); then the group sum and
     // mod-down (skipped when the gate is set)
     launch_rows_sm(c, false, nullptr, nullptr, accm, acc, Zc, 0, 0, B, G, s);
-    launch_inv_cols_crt(c, acc, accq, limbs, G * Zc, s);
+    launch_inv_cols_crt(c, acc, accq, limbs, B, G, s, nullptr, addr, addr_ps);
     launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s, gate, 0);
     if (pl.gated) {
         // some ciphertext's digits exceed the one-group bound: the batch again with the worst-case groups
         launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, trash, B, levels, Dp, cols, pl.gsw, Gw, spw, s,
                         gate);
         launch_rows_sm(c, false, nullptr, nullptr, accm, acc, Zc, 0, 0, B, Gw, s, gate);
-        launch_inv_cols_crt(c, acc, accq, limbs, Gw * Zc, s, gate);
+        launch_inv_cols_crt(c, acc, accq, limbs, B, Gw, s, gate, addr, addr_ps);
         launch_crt_moddown(c, accq, limbs, B, Gw, add0, add1, add_ps, ginv, outs, s, gate, 1);
     }
 }

@@ -39,14 +39,15 @@ void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, 
 }
 
 // B ciphertext products lhs[b] x rhs[b] (ciphertext strides in_ps) -> quadratic out[b] (stride out_ps).
-// fold_ntt (two-pass rings): out[b] receives only d2 (coefficient form); d0, d1 stay in NTT form in the
-// returned buffer ([B][3][limbs][N] doubles, d0 at +0 and d1 at +limbs*N of each ciphertext) for a
-// keyswitch that folds them into its accumulators.
+// mode kMultFoldNtt / kMultFoldRows (two-pass rings): out[b] receives only d2 (coefficient form); d0, d1 stay
+// in the returned buffer ([B][3][limbs][N] doubles, d0 at +0 and d1 at +limbs*N of each ciphertext), in NTT
+// form or after the inverse row stages, for a keyswitch that folds them in.
 double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
-                   uint64_t* out, long long out_ps, hipStream_t s, bool fold_ntt) {
+                   uint64_t* out, long long out_ps, hipStream_t s, int mode) {
+    const bool fold_ntt = mode == kMultFoldNtt, d2_only = mode != kMultFull;
     const long long n = c.n, P = (long long)limbs * n;
     if (c.wide) {
-        RCK_REQUIRE(!fold_ntt, "the folded product serves the FP64 arithmetic only");
+        RCK_REQUIRE(!d2_only, "the folded product serves the FP64 arithmetic only");
         for (int b = 0; b < B; b++) wide_mult(c, lhs + b * in_ps, rhs + b * in_ps, limbs, out + b * out_ps, s);
         return nullptr;
     }
@@ -54,7 +55,7 @@ double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long lo
     double* x = static_cast<double*>(c.workspace(0, (size_t)B * 4 * P * 8));
     double* d = static_cast<double*>(c.workspace(1, (size_t)B * 3 * P * 8));
     if (c.logn <= 11) {
-        RCK_REQUIRE(!fold_ntt, "the folded product needs the two-pass NTT");
+        RCK_REQUIRE(!d2_only, "the folded product needs the two-pass NTT");
         for (int b = 0; b < B; b++) {
             launch_fwd(c, kInU64, lhs + b * in_ps, n, P, x, n, P, limbs, 2, pm, s);
             launch_fwd(c, kInU64, rhs + b * in_ps, n, P, x + 2 * P, n, P, limbs, 2, pm, s);
@@ -76,7 +77,7 @@ double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long lo
         }
     }
     launch_mult_rows(c, xl, xr, d, limbs, B, pm, s, fold_ntt);
-    if (fold_ntt) {
+    if (d2_only) {
         // inverse column pass of d2 only: d[b][2] -> out[b]
         launch_inv_cols(c, d + 2 * P, n, 3 * P, kOutU64, out, n, out_ps, limbs, B, pm, s);
         return d;
@@ -145,8 +146,16 @@ void mult_relin_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long
         // relinearize = (d0, d1) + keyswitch(d2): d0, d1 are added to the Q limbs of the keyswitch
         // accumulators in NTT form, times P, so mod-down by P returns keyswitch + (d0, d1) exactly
         uint64_t* c2 = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * P * 8));
-        const double* dn = mult_batch(c, lhs, rhs, in_ps, B, limbs, c2, P, s, true);
+        const double* dn = mult_batch(c, lhs, rhs, in_ps, B, limbs, c2, P, s, kMultFoldNtt);
         keyswitch_batch(c, c2, P, B, limbs, key, nullptr, nullptr, 0, outs, s, shape, dn, 3 * P);
+        return;
+    }
+    if (c.logn >= 12 && key && key->aux > 0 && key->mdata != nullptr && !c.wide) {
+        // the exact convolution: d0, d1 leave the product after the inverse row stages, and their inverse
+        // column pass runs inside the keyswitch's recombination, which adds them times P before mod-down
+        uint64_t* c2 = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * P * 8));
+        const double* dr = mult_batch(c, lhs, rhs, in_ps, B, limbs, c2, P, s, kMultFoldRows);
+        keyswitch_batch(c, c2, P, B, limbs, key, nullptr, nullptr, 0, outs, s, shape, nullptr, 0, 0, dr, 3 * P);
         return;
     }
     uint64_t* quad = static_cast<uint64_t*>(c.workspace(kSlotQuad, (size_t)B * 3 * P * 8));
@@ -344,7 +353,8 @@ std::vector<KsShape> ks_candidates(const Context& c, int E, int D, int B, bool e
 // keyswitch_rns_hybrid (:159-194).
 void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, KsShape shape, const double* addn, long long addn_ps, unsigned long long ginv) {
+                     hipStream_t s, KsShape shape, const double* addn, long long addn_ps, unsigned long long ginv,
+                     const double* addr, long long addr_ps) {
     RCK_REQUIRE((int)outs.size() == B, "one output per ciphertext");
     check_ptr(key, "key");
     const long long n = c.n, P = (long long)limbs * n;
@@ -371,6 +381,14 @@ void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, i
     }
     RCK_REQUIRE(key->data != nullptr || key->mdata != nullptr,
                 "keyswitch key was created for another arithmetic of this context");
+    if (addr != nullptr) {
+        RCK_REQUIRE(add0 == nullptr && add1 == nullptr && addn == nullptr && ginv == 0 && key->mdata != nullptr &&
+                        naux > 0 && c.logn >= 12,
+                    "row-form addends serve the exact-convolution keyswitch only");
+        const KsShape sh = shape.prefetch == kExactShape ? shape : ks_auto_shape(c, limbs, limbs + naux, levels, B, true);
+        keyswitch_exact(c, poly, in_ps, B, limbs, key, levels, nullptr, nullptr, 0, outs, s, 0, sh.rows, addr, addr_ps);
+        return;
+    }
     if (addn == nullptr && ks_use_exact(c, key, limbs, limbs + naux, levels, B, shape)) {
         const KsShape sh = shape.prefetch == kExactShape ? shape : ks_auto_shape(c, limbs, limbs + naux, levels, B, true);
         keyswitch_exact(c, poly, in_ps, B, limbs, key, levels, add0, add1, add_ps, outs, s, ginv, sh.rows);

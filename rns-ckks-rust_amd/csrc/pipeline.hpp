@@ -28,9 +28,12 @@ void exact_grouping(const Context& c, int nq, int levels, int* gsize, int* G);
 // keep_data: the key keeps its digit-NTT form as well (both methods available)
 void key_make_exact(Context& c, KeyDev* k, bool keep_data = false);
 void key_export_exact_digit(Context& c, const KeyDev* k, uint32_t d, uint64_t* out);
+// addr (nullable): addends d0, d1 after the inverse row stages (kMultFoldRows; ciphertext b's d0 at
+// addr + b * addr_ps, d1 at + limbs*N), folded in times P before mod-down (add0, add1 then null)
 void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key, int levels,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
-                     hipStream_t s, unsigned long long ginv, int spw);
+                     hipStream_t s, unsigned long long ginv, int spw, const double* addr = nullptr,
+                     long long addr_ps = 0);
 
 // Launch shape of the digit MAC: eval limbs per group and digit splits (0 = automatic).
 struct KsShape {
@@ -53,20 +56,25 @@ inline void check_ptr(const void* p, const char* what) {
 size_t poly_mul_scratch_bytes(const Context& c, long long b_ps, int limbs, int polys);
 void poly_mul(Context& c, const uint64_t* a, long long a_ps, const uint64_t* b, long long b_ps, uint64_t* out,
               long long out_ps, int limbs, int polys, PrimeMap pm, hipStream_t s, double* scratch = nullptr);
-// B products lhs[b] x rhs[b] (ciphertext stride in_ps) -> quadratic out[b] (stride out_ps).  fold_ntt:
-// out[b] gets d2 only and d0, d1 stay in NTT form in the returned buffer ([B][3][limbs][N] doubles)
+// What mult_batch leaves of d0 and d1 (two-pass rings): with kMultD2Only out[b] gets d2 only, and d0, d1 stay
+// in the returned buffer ([B][3][limbs][N] doubles, d0 at +0 and d1 at +limbs*N of each ciphertext), in NTT
+// form (kMultFoldNtt) or after the inverse row stages, before the inverse column pass (kMultFoldRows).
+enum MultOut { kMultFull = 0, kMultFoldNtt = 1, kMultFoldRows = 2 };
+// B products lhs[b] x rhs[b] (ciphertext stride in_ps) -> quadratic out[b] (stride out_ps), or d2 only (mode)
 double* mult_batch(Context& c, const uint64_t* lhs, const uint64_t* rhs, long long in_ps, int B, int limbs,
-                   uint64_t* out, long long out_ps, hipStream_t s, bool fold_ntt = false);
+                   uint64_t* out, long long out_ps, hipStream_t s, int mode = kMultFull);
 void mult_impl(Context& c, const uint64_t* lhs, const uint64_t* rhs, int limbs, uint64_t* out, hipStream_t s);
 void rescale_impl(Context& c, const uint64_t* in, int limbs, int polys, uint64_t* out, hipStream_t s);
 // keyswitch of B polynomials (stride in_ps) with one key, adding add0/add1 (stride add_ps, nullable)
 // into outs[b] = [c0][c1].  addn (hybrid keys, two-pass rings; nullable): NTT-form addends
 // addn[b * addn_ps + {0, limbs*N}] folded into the accumulators (times P) instead of add0/add1.
 // ginv != 0 (hybrid keys, two-pass rings): poly and add0 are read through the automorphism X -> X^g.
+// addr (exact-convolution keyswitch only; nullable): d0, d1 addends after the inverse row stages, as in
+// keyswitch_exact.
 void keyswitch_batch(Context& c, const uint64_t* poly, long long in_ps, int B, int limbs, const KeyDev* key,
                      const uint64_t* add0, const uint64_t* add1, long long add_ps, const std::vector<uint64_t*>& outs,
                      hipStream_t s, KsShape shape = {}, const double* addn = nullptr, long long addn_ps = 0,
-                     unsigned long long ginv = 0);
+                     unsigned long long ginv = 0, const double* addr = nullptr, long long addr_ps = 0);
 // rotate (RnsCkksContext::rotate, context/rotation.rs:104-127) of B ciphertexts at `limbs` limbs with one
 // rotation key
 void rotate_batch(Context& c, const std::vector<const uint64_t*>& as, int limbs, const KeyDev* key,
