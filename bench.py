@@ -10,6 +10,7 @@ relinearization key resident in NTT form.  Prints one JSON line.
 import argparse
 import hashlib
 import json
+import math
 import os
 import re
 import subprocess
@@ -82,6 +83,114 @@ def modmul_per_op(n, L, K, D):
             "decompose": n * (L * (L - 1) // 2 + L * D // 2 + D * (L + K)),
             "mod_down": 2 * n * L * (K + 2), "tensor": 4 * L * n}
 
+
+
+MFMA_I8_PEAK = 5.0e15  # MI355X dense int8 matrix-core peak (ops/s, 2 per multiply-accumulate)
+
+
+def exact_mac_io(n, B, D, cols, G_):
+    """HBM bytes of one exact-convolution MAC launch (matrix cores): the key's byte planes read once
+    (7 planes x cols x D rounded up to 8 bytes per slot, two auxiliary primes), the digit words (B x D rounded
+    up to 8, 8 bytes each, two primes) and the accumulators written (G_ groups x B x cols doubles, two primes)."""
+    d8 = 8 * ((D + 7) // 8)
+    return 2 * n * (7 * cols * d8 + B * d8 * 8 + G_ * B * cols * 8)
+
+
+def exact_mac_mfma(n, B, D, cols, G_):
+    """int8 matrix-core work of one exact-convolution MAC launch: every 16 x 16 (ciphertexts x key columns)
+    tile of every slot takes 49 v_mfma_i32_16x16x64_i8 (one per key plane x digit plane); the useful part
+    is the 49 byte products of each digit x key pair."""
+    mfma = G_ * 2 * n * (-(-B // 16)) * (-(-cols // 16)) * 49
+    return {"mfma_per_launch": mfma, "int8_ops_issued": mfma * 16 * 16 * 64 * 2,
+            "int8_ops_useful": 2 * n * B * D * cols * 49 * 2}
+
+
+def ingest_counters(dirs, mode, workload, batch, steps):
+    """profiles/counters.json record of `mode` from rocprofv3 --pmc passes (one directory each) over
+    `bench.py --mode MODE --counter-run --steps STEPS`: the dispatches after the 1 MiB marker copy,
+    FETCH_SIZE / WRITE_SIZE (KiB) times the copy calibration (profiles/copy_calibration.md), FP64 and all
+    VALU wave instructions, matrix-core instructions and busy cycles; per step and per launch of each kernel.
+    A kernel's launches that do no work (the gated recomputation when its flag is clear) are left out of
+    its per-launch figures; the dominant kernel is the keyswitch multiply-accumulate."""
+    import csv
+    import glob
+    cal = {"fetch": 1.9913, "write": 0.9666}
+    disp = {}  # dispatch id -> [kernel, {counter: value}]
+    for d in dirs:
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if len(files) != 1:
+            raise SystemExit(f"ingest: expected one counter_collection.csv under {d}, found {len(files)}")
+        pass_disp = {}
+        with open(files[0]) as f:
+            for r in csv.DictReader(f):
+                pass_disp.setdefault(int(r["Dispatch_Id"]), [r["Kernel_Name"], {}])[1][r["Counter_Name"]] = \
+                    float(r["Counter_Value"])
+        ids = sorted(pass_disp)
+        marks = [i for i in ids if pass_disp[i][0].split("(")[0].split("::")[-1].strip() == "k_copy"]
+        if not marks:
+            raise SystemExit(f"ingest: no marker copy in {d}")
+        after = [pass_disp[i] for i in ids if i > marks[-1]]
+        for k, (name, cs) in enumerate(after):
+            e = disp.setdefault(k, [name, {}])
+            if e[0] != name:
+                raise SystemExit("ingest: the passes dispatched different kernels")
+            e[1].update(cs)
+
+    def conv(cs):
+        out = {}
+        if "FETCH_SIZE" in cs:
+            out["fetch_bytes"] = cs["FETCH_SIZE"] * 1024 * cal["fetch"]
+        if "WRITE_SIZE" in cs:
+            out["write_bytes"] = cs["WRITE_SIZE"] * 1024 * cal["write"]
+        f64 = [cs[k] for k in ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+                               "SQ_INSTS_VALU_TRANS_F64") if k in cs]
+        if f64:
+            out["valu_f64"] = sum(f64)
+        for src, dst, scale in (("SQ_INSTS_VALU", "valu", 1), ("SQ_INSTS_VALU_MFMA_MOPS_I8", "mfma_i8_ops", 512),
+                                ("SQ_VALU_MFMA_BUSY_CYCLES", "mfma_busy_cycles", 1),
+                                ("GRBM_GUI_ACTIVE", "gui_active_cycles", 1), ("MfmaUtil", "mfma_util_pct", 1)):
+            if src in cs:
+                out[dst] = cs[src] * scale
+        return out
+
+    per_step, kernels = {}, {}
+    by_name = {}
+    for k in sorted(disp):
+        name, cs = disp[k]
+        v = conv(cs)
+        for key, x in v.items():
+            per_step[key] = per_step.get(key, 0.0) + x / steps
+        by_name.setdefault(name, []).append(v)
+    for name, vs in by_name.items():
+        top = max(v.get("fetch_bytes", 0.0) + v.get("write_bytes", 0.0) for v in vs)
+        work = [v for v in vs if v.get("fetch_bytes", 0.0) + v.get("write_bytes", 0.0) > 0.01 * top] or vs
+        rec = {"dispatches_per_step": len(work) / steps}
+        for key in work[0]:
+            rec[key] = sum(v.get(key, 0.0) for v in work) / len(work)
+        kernels[name] = rec
+    dom = [k for k in kernels if "k_ks_mac" in k or "k_digit_mac" in k]
+    if not dom:
+        raise SystemExit("ingest: no keyswitch multiply-accumulate among the dispatches")
+    dname = max(dom, key=lambda k: kernels[k].get("fetch_bytes", 0.0))
+    rnd = lambda r: {k: (round(v, 3) if k in ("dispatches_per_step", "mfma_util_pct") else int(round(v)))
+                     for k, v in r.items()}
+    record = {"workload": workload, "source_sha": source_sha(), "steps_profiled": steps, "batch": batch,
+              "calibration": cal, "dominant": dict(rnd(kernels[dname]), kernel=dname),
+              "per_step": rnd(per_step), "kernels": {k: rnd(v) for k, v in kernels.items()}}
+    path = os.path.join(ROOT, "profiles", "counters.json")
+    allrec = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            allrec = json.load(f)
+    allrec["_note"] = ("rocprofv3 --pmc runs of bench.py --counter-run (dispatches after the marker copy), "
+                       "aggregated by bench.py --ingest-counters: HBM bytes = FETCH_SIZE / WRITE_SIZE KiB x copy "
+                       "calibration; FP64 = SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64 wave instructions; per launch of "
+                       "each kernel (launches that do no work left out) and per step (one batch); int8 matrix-core ops = "
+                       "SQ_INSTS_VALU_MFMA_MOPS_I8 x 512; mfma_util_pct = the MfmaUtil derived counter (its own pass)")
+    allrec[mode] = record
+    with open(path, "w") as f:
+        json.dump(allrec, f, indent=1)
+    return record
 
 # the sources the measured mult+relinearize step is compiled from (kernels, their launchers and the
 # batched executor): counter records taken on other versions of these files are not reused
@@ -228,6 +337,9 @@ def main():
     ap.add_argument("--counter-run", action="store_true",
                     help="only the measured work for rocprofv3 --pmc: after a 1 MiB marker copy, exactly --steps "
                          "batches (or per-op steps) and nothing else")
+    ap.add_argument("--ingest-counters", nargs="+", default=None, metavar="DIR",
+                    help="write the profiles/counters.json record of --mode (batched / per-op at --batch, --steps) "
+                         "from rocprofv3 --pmc output directories of its --counter-run, and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (5 times): the known-volume reference "
                          "for rocprofv3 FETCH_SIZE/WRITE_SIZE calibration")
@@ -242,6 +354,19 @@ def main():
 
     from rnsckks import capi as G
     from rnsckks import params as PR
+
+    if args.ingest_counters:
+        p = PR.config_per_op_gpu()
+        n, L, K = p.poly_degree, len(p.primes), len(p.aux_primes)
+        B = args.batch if args.mode == "batched" else 1
+        if args.mode not in ("batched", "per-op"):
+            raise SystemExit("--ingest-counters: --mode batched or per-op")
+        # decomposition digits: 30-bit words of Q_L (the library's rule; no device needed)
+        D = (math.prod(p.primes[:L]).bit_length() + 29) // 30
+        rec = ingest_counters(args.ingest_counters, args.mode, f"N=2^{n.bit_length() - 1} L={L} K={K} D={D} B={B}",
+                              B, args.steps)
+        print(json.dumps({"ingested": args.mode, "dominant": rec["dominant"], "per_step": rec["per_step"]}))
+        return
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20
@@ -473,10 +598,11 @@ def main():
     alg_op = alg_bytes_per_op(n, L, K, D, B)
     if name in ("digit_mac", "ks_mac"):
         if name == "ks_mac":
-            # exact-convolution MAC: key read once (2 D E limbs under two auxiliary primes), digit NTTs read
-            # (B D under two primes), accumulators written (G groups x B x 2 E under two primes)
+            # exact-convolution MAC on the matrix cores: key byte planes read once, digit words read,
+            # accumulators written (each group once)
             groups = ctx.exact_group_size(L)[1]
-            io = int((2 * D * (L + K) * 2 + B * D * 2 + groups * B * 2 * (L + K) * 2) * n * 8 / launches)
+            io = int(exact_mac_io(n, B, D, 2 * (L + K), groups) / launches)
+            mf = exact_mac_mfma(n, B, D, 2 * (L + K), groups)
         else:
             G_ = -(-(L + K) // launches)  # eval limbs per launch
             ds = report["groups"][0]["digit_splits"] if report else tune1["digit_splits"]
@@ -493,18 +619,34 @@ def main():
                                        "ops per launch",
                     "kernel_io_bytes_per_launch": io,
                     "kernel_io_gbs": round(io / avg_s / 1e9, 1)}
+        if name == "ks_mac":
+            ops_launch = mf["int8_ops_issued"] / launches
+            roofline["mfma"] = {
+                "instr": "v_mfma_i32_16x16x64_i8", "instr_per_launch": int(mf["mfma_per_launch"] / launches),
+                "int8_ops_issued_per_launch": int(ops_launch),
+                "int8_ops_useful_per_launch": int(mf["int8_ops_useful"] / launches),
+                "achieved_tops": round(ops_launch / avg_s / 1e12, 1), "peak_tops": MFMA_I8_PEAK / 1e12,
+                "frac": round(ops_launch / avg_s / MFMA_I8_PEAK, 4),
+                "basis": "49 byte-plane products per 16 x 16 tile (ciphertexts x key columns) per NTT slot, "
+                         "K = 64 digits; issued = the tiles' full size, useful = B D 2(L+K) byte products"}
         if rec:
             dom = rec["dominant"]
             roofline["traffic"] = int(dom["fetch_bytes"] + dom["write_bytes"])
             roofline["traffic_vs_alg"] = round(roofline["traffic"] / alg, 3)
-            f64 = dom["valu_f64"]
-            roofline["fp64_issue"] = {
-                "wave_instr_per_launch": int(f64),
-                "frac_at_nominal_2.4GHz": round(f64 / (avg_s * CUS * NOMINAL_GHZ * 1e9), 4),
-                "frac_at_sustained_clock": round(f64 / (avg_s * CUS * sclk * 1e9), 4) if sclk else None,
-                "sustained_clock_ghz": sclk,
-                "valu_wave_instr_per_launch": int(dom["valu"]),
-                "source": "profiles/counters.json (SQ_INSTS_VALU_{FMA,ADD,MUL}_F64, SQ_INSTS_VALU)"}
+            roofline["traffic_vs_kernel_io"] = round(roofline["traffic"] / io, 3)
+            roofline["traffic_gbs"] = round(roofline["traffic"] / avg_s / 1e9, 1)
+            if "mfma_i8_ops" in dom and "mfma" in roofline:
+                roofline["mfma"]["int8_ops_counted_per_launch"] = int(dom["mfma_i8_ops"])
+                roofline["mfma"]["busy_pct_counted"] = dom.get("mfma_util_pct")
+            f64 = dom.get("valu_f64")
+            if f64 is not None:
+                roofline["fp64_issue"] = {
+                    "wave_instr_per_launch": int(f64),
+                    "frac_at_nominal_2.4GHz": round(f64 / (avg_s * CUS * NOMINAL_GHZ * 1e9), 4),
+                    "frac_at_sustained_clock": round(f64 / (avg_s * CUS * sclk * 1e9), 4) if sclk else None,
+                    "sustained_clock_ghz": sclk,
+                    "valu_wave_instr_per_launch": int(dom["valu"]),
+                    "source": "profiles/counters.json (SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64, SQ_INSTS_VALU)"}
     step = {"alg_bytes": int(alg_op * ops_per_step), "ms": round(ms_per_step, 4),
             "achieved": round(alg_op * ops_per_step / ms_per_step / 1e6, 1),
             "frac": round(alg_op * ops_per_step / ms_per_step / 1e6 / HBM_PEAK_GBS, 4)}
