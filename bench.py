@@ -159,7 +159,8 @@ def ingest_counters(dirs, mode, workload, batch, steps):
         name, cs = disp[k]
         v = conv(cs)
         for key, x in v.items():
-            per_step[key] = per_step.get(key, 0.0) + x / steps
+            if key != "mfma_util_pct":  # a ratio: per kernel only
+                per_step[key] = per_step.get(key, 0.0) + x / steps
         by_name.setdefault(name, []).append(v)
     for name, vs in by_name.items():
         top = max(v.get("fetch_bytes", 0.0) + v.get("write_bytes", 0.0) for v in vs)
