@@ -104,6 +104,11 @@ __global__ void k_key_from_special(const uint64_t* __restrict__ in, uint64_t* __
 // ---------------------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned long long v2u __attribute__((ext_vector_type(2)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// word 3 of a raw buffer resource on gfx9-family GPUs (untyped dword access, range-checked against the
+// record count: an out-of-range store is dropped)
+constexpr int kBufferRsrcWord3 = 0x00020000;
 
 constexpr uint64_t kByteBias = 0x0080808080808080ull;  // 128 in bytes 0 .. 6
 
@@ -131,7 +136,6 @@ struct MfmaArgs {
     const uint64_t* dsb;   // [2][N][B][Dp] digit words (slot-major)
     const uint8_t* key;    // [2][N][7 cols Dk] key byte planes (Dk = digits rounded up to 8), then 64 bytes of padding
     double* acc;           // [G][2][N][B cols] accumulators (slot-major, reduced)
-    double* trash;         // 256 doubles: the store target of lanes without an output
     int n, B, Dp, Dk, cols, D, gsize;
     double q[2], qinv[2], c32[2];  // the auxiliary primes, 1/s, 2^32 mod s (centred)
 };
@@ -218,8 +222,9 @@ __device__ __forceinline__ void mfma_tile(const v4i (&dp)[7], v4i (&kp)[7], doub
 //  * key columns past the last tile's width read its last column (those products are not stored);
 //    digit words past D read the row's last words, and the digits outside the group (past D included)
 //    are zeroed when the planes are formed, so the key bytes they meet (other data or padding) add nothing;
-//  * a step's stores are issued during the next step, after its first refill, and lanes with no output
-//    (padding columns, rows past the batch) store to a scratch line instead.
+//  * a step's stores are issued during the next step, after its first refill, as buffer stores over the
+//    slot's accumulator record; lanes with no output (padding columns, rows past the batch) give an offset
+//    past the record, and the buffer range check drops their stores.
 // Each address is a wave-uniform base plus a lane offset fixed for the launch.
 __device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xbeg, int nslots, int g, int i0, int i1,
                                           int lane) {
@@ -255,15 +260,14 @@ __device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xb
         const uint32_t la = (uint32_t)(min(j, A.B - 1 - 16 * mt) * A.Dp + e) * 8u;
         return *(const __attribute__((address_space(1))) v2u*)(dp + (uint64_t)la);
     };
-    // output addresses of a step: ciphertext 16 mt + j, columns 16 nt + 4 gq + 2 pr + {0, 1} (cols is even,
-    // so a pair is valid or not as a whole); lanes without an output write the scratch line
+    // output of a step: ciphertext 16 mt + j, columns 16 nt + 4 gq + 2 pr + {0, 1} (cols is even, so a pair
+    // is valid or not as a whole), as a byte offset in the slot's record of zc doubles; kDrop for lanes
+    // without an output
+    constexpr uint32_t kDrop = 0x80000000u;
     double* const accg = A.acc + (long long)g * 2 * A.n * zc;
-    auto out_ptr = [&](long long sl, int mt, int nt, int pr) -> double* {
+    auto out_off = [&](int mt, int nt, int pr) -> uint32_t {
         const int bb = 16 * mt + j, col = 16 * nt + 4 * gq + 2 * pr;
-        const bool ok = bb < A.B && col < A.cols;
-        // both addresses computed, then selected (no branch in the loop)
-        double* const p = accg + sl * zc + (long long)(ok ? bb : 0) * A.cols + (ok ? col : 0);
-        return ok ? p : A.trash + lane * 4 + 2 * pr;
+        return bb < A.B && col < A.cols ? (uint32_t)(bb * A.cols + col) * 8u : kDrop;
     };
     typedef double v2d __attribute__((ext_vector_type(2)));
     const int groups = nslots * MT;
@@ -280,22 +284,28 @@ __device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xb
     }
 #pragma unroll
     for (int h = 0; h < 8; h++) wb[h] = load_w(sl, 0, h);
-    v2d Vp[2];      // the previous step's outputs, stored during this step (the first step stores zeros to
-    double* op[2];  // the scratch line)
+    // the previous step's outputs, stored during this step (the first step's stores are all dropped)
+    v2d Vp[2];
+    uint32_t oo[2];
+    long long osl = sl;  // the slot of those outputs (wave-uniform)
 #pragma unroll
     for (int pr = 0; pr < 2; pr++) {
         Vp[pr] = v2d{0.0, 0.0};
-        op[pr] = A.trash + lane * 4 + 2 * pr;
+        oo[pr] = kDrop;
     }
     auto store_prev = [&]() {
+        const __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)uniform_ptr(accg + osl * zc), (short)0, (int)(zc * 8), kBufferRsrcWord3);
 #pragma unroll
-        for (int pr = 0; pr < 2; pr++) *reinterpret_cast<v2d*>(op[pr]) = Vp[pr];
+        for (int pr = 0; pr < 2; pr++)
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, Vp[pr]), rec, (int)oo[pr], 0, 0);
     };
     auto keep = [&](const double (&V)[4], long long sl0, int mt0, int nt0) {
+        osl = sl0;
 #pragma unroll
         for (int pr = 0; pr < 2; pr++) {
             Vp[pr] = v2d{V[2 * pr], V[2 * pr + 1]};
-            op[pr] = out_ptr(sl0, mt0, nt0, pr);
+            oo[pr] = out_off(mt0, nt0, pr);
         }
     };
     for (int u = 0; u < groups; u++) {
@@ -346,13 +356,12 @@ __global__ __launch_bounds__(256) void k_ks_mac_mfma(MfmaArgs A, int spw, int nx
     mfma_wave(A, s, ((long long)xb * 4 + wave) * spw, spw, g, i0, i1, lane);
 }
 
-static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint8_t* key, double* acc, double* trash,
+static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint8_t* key, double* acc,
                           int B, int D, int Dp, int Dk, int cols, int gsize) {
     MfmaArgs A{};
     A.dsb = dsb;
     A.key = key;
     A.acc = acc;
-    A.trash = trash;
     A.n = (int)c.n;
     A.B = B;
     A.Dp = Dp;
@@ -370,18 +379,20 @@ static MfmaArgs mfma_args(const Context& c, const uint64_t* dsb, const uint8_t* 
     return A;
 }
 
-// acc: G groups of [2][N][B cols]; trash: 256 doubles of scratch; kdig: the key's digits (its K-groups);
-// spw slots per wave (1, 2 or 4); digits in groups of gsize (G groups); gate as in k_ks_mac_mfma
+// acc: G groups of [2][N][B cols]; kdig: the key's digits (its K-groups); spw slots per wave (1, 2 or 4);
+// digits in groups of gsize (G groups); gate as in k_ks_mac_mfma
 static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint8_t* key, int kdig, double* acc,
-                            double* trash, int B, int D, int Dp, int cols, int gsize, int G, int spw, hipStream_t s,
+                            int B, int D, int Dp, int cols, int gsize, int G, int spw, hipStream_t s,
                             const int* gate = nullptr) {
     RCK_REQUIRE(spw == 1 || spw == 2 || spw == 4, "bad exact keyswitch shape");
     RCK_REQUIRE(c.n % (4 * spw) == 0 && B >= 1 && D <= kdig && kdig <= 64 && Dp >= 2 && Dp % 2 == 0 && cols % 2 == 0,
                 "bad exact keyswitch size");
-    const MfmaArgs A = mfma_args(c, dsb, key, acc, trash, B, D, Dp, 8 * ((kdig + 7) / 8), cols, gsize);
+    const MfmaArgs A = mfma_args(c, dsb, key, acc, B, D, Dp, 8 * ((kdig + 7) / 8), cols, gsize);
     const int nxb = (int)(c.n / (4 * spw));
     const long long blocks = (long long)G * 2 * nxb;
     RCK_REQUIRE(blocks < (1LL << 31), "exact keyswitch launch too large");
+    // a slot's accumulator record is one buffer range; its byte offsets stay below the dropped offset 2^31
+    RCK_REQUIRE((long long)B * cols * 8 < (1LL << 31), "exact keyswitch batch too large");
     ProfScope ps(s, gate ? "ks_mac_gated" : "ks_mac");
     k_ks_mac_mfma<<<dim3((unsigned)blocks), 256, 0, s>>>(A, spw, nxb, gate);
     RCK_CHECK_HIP(hipGetLastError());
@@ -913,7 +924,7 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     // the accumulators are sized for the worst-case groups, which the recomputation reuses
     double* dn = static_cast<double*>(c.workspace(30, (size_t)B * levels * 2 * n * 8));
     uint64_t* dsb = static_cast<uint64_t*>(c.workspace(35, (size_t)2 * n * B * Dp * 8));
-    double* accm = static_cast<double*>(c.workspace(36, ((size_t)Gw * 2 * n * Zc + 256) * 8));
+    double* accm = static_cast<double*>(c.workspace(36, (size_t)Gw * 2 * n * Zc * 8));
     double* acc = static_cast<double*>(c.workspace(31, (size_t)Gw * Zc * 2 * n * 8));
     double* accq = static_cast<double*>(c.workspace(32, (size_t)Gw * Zc * n * 8));
     double* l1 = nullptr;
@@ -933,8 +944,7 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     // then the row stages written slot-major as signed-byte words
     launch_fwd_cols(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
     launch_rows_sm(c, true, dn, dsb, nullptr, nullptr, B * levels, levels, Dp, B, 1, s);
-    double* trash = accm + (size_t)Gw * 2 * n * Zc;
-    launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, trash, B, levels, Dp, cols, pl.gsize, G, spw, s);
+    launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, B, levels, Dp, cols, pl.gsize, G, spw, s);
     // inverse rows under s1, s2 (back to limb rows); inverse columns fused with the recombination (one
     // value per accumulator and coefficient: the group's exact sum mod p_j); then the group sum and
     // mod-down (skipped when the gate is set)
@@ -943,7 +953,7 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
     launch_crt_moddown(c, accq, limbs, B, G, add0, add1, add_ps, ginv, outs, s, gate, 0);
     if (pl.gated) {
         // some ciphertext's digits exceed the one-group bound: the batch again with the worst-case groups
-        launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, trash, B, levels, Dp, cols, pl.gsw, Gw, spw, s,
+        launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, B, levels, Dp, cols, pl.gsw, Gw, spw, s,
                         gate);
         launch_rows_sm(c, false, nullptr, nullptr, accm, acc, Zc, 0, 0, B, Gw, s, gate);
         launch_inv_cols_crt(c, acc, accq, limbs, B, Gw, s, gate, addr, addr_ps);
