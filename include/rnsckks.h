@@ -73,12 +73,12 @@ int rck_context_destroy(rck_context* ctx);
 int rck_num_decomp_digits(const rck_context* ctx, uint32_t limbs, uint32_t* digits);
 /* Keyswitch method of the keys created (or generated) from now on in this context; no counterpart in
  * the crate, whose single method both produce bit for bit (keyswitching.rs:159-194).
- * RCK_KS_EXACT_CONVOLUTION (default): hybrid keys at N >= 2^12 are kept as the NTT forms, under two
+ * RCK_KS_EXACT_CONVOLUTION: hybrid keys at N >= 2^12 are kept as the NTT forms, under two
  * auxiliary primes, of their centred limbs; a digit times a key limb is then an exact integer
- * convolution modulo the two primes' product (every digit transformed twice instead of once per eval
- * prime; the key takes twice the memory).  RCK_KS_DIGIT_NTT: keys kept in NTT form mod every eval
- * prime, every digit transformed under every eval prime.  Q-only keys and N <= 2^11 always use the
- * latter.  RCK_KS_AUTO (the default): such keys keep both forms (three times the memory of the
+ * convolution modulo the two primes' product, summed on the int8 matrix cores (every digit transformed
+ * twice instead of once per eval prime; the key, 7 signed bytes per value, takes 1.75 times the
+ * memory).  RCK_KS_DIGIT_NTT: keys kept in NTT form mod every eval prime, every digit transformed
+ * under every eval prime.  Q-only keys and N <= 2^11 always use the latter.  RCK_KS_AUTO (the default): such keys keep both forms (2.75 times the memory of the
  * digit-NTT form) and every keyswitch takes the faster method for its batch size (the tuned times of
  * both when the tuning table has them, else the exact convolution from four ciphertexts up).
  * rck_ksk_method reports a key's method (RCK_KS_AUTO when it holds both forms). */
@@ -369,7 +369,8 @@ int rck_batch_destroy(rck_batch* batch);
 int rck_tune_keyswitch(rck_context* ctx, uint32_t limbs, const rck_ksk* key, uint32_t batch, char* buf, size_t len);
 /* The process-wide tuning table that per-op calls and rck_batch_compile consult (entries keyed by
  * ring degree, limbs, eval limbs, digits and batch size), as text: one line per entry,
- * "log2N limbs eval_limbs digits batch group digit_splits mac_rows load_schedule mac_width ms".  Export writes
+ * "log2N limbs eval_limbs digits batch group digit_splits mac_rows load_schedule mac_width ms" (an
+ * exact-convolution entry has load_schedule 7, mac_rows = slots per wave of its MAC, mac_width 16).  Export writes
  * the table into buf (needed = bytes including the terminating NUL; buf may be NULL to query);
  * import validates every line, then merges the entries (a tuned group is not re-timed).  No
  * counterpart in the crate (it has no GPU launch shapes); lets a process reuse earlier tuning. */

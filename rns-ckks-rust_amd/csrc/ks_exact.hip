@@ -538,15 +538,26 @@ __device__ __forceinline__ long long key_plane_byte(int n, int cols, int Dk, int
     return ((long long)s * n + x) * 7LL * cols * Dk + (7LL * 16 * nt + (long long)v * w + j) * Dk + d;
 }
 
-// digit d of the key: stage [cols][2][N] (the NTT forms under s1, s2, reduced) -> its bytes in the planes
-__global__ void k_key_digit_to_mfma(const double* __restrict__ stage, uint8_t* __restrict__ key, int n, int cols,
-                                    int Dk, int d) {
+// digits d0 .. d0 + nd - 1 of the key (nd <= 8, d0 a multiple of 8): stage [nd][cols][2][N] (the NTT forms
+// under s1, s2, reduced) -> their bytes in the planes, one 8-byte word per (slot, column, plane); the bytes
+// of digits past nd are written as zeros
+__global__ void k_key_chunk_to_mfma(const double* __restrict__ stage, uint8_t* __restrict__ key, int n, int cols,
+                                    int Dk, int d0, int nd) {
     const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= n) return;
     const int col = blockIdx.y, s = blockIdx.z;
-    const uint64_t w = sbyte_word(f64_to_i64(stage[((long long)col * 2 + s) * n + x]));
+    const long long dstride = (long long)cols * 2 * n;
+    uint64_t w[8];
 #pragma unroll
-    for (int v = 0; v < 7; v++) key[key_plane_byte(n, cols, Dk, s, x, col, d, v)] = (uint8_t)(w >> (8 * v));
+    for (int t = 0; t < 8; t++)
+        w[t] = t < nd ? sbyte_word(f64_to_i64(stage[t * dstride + ((long long)col * 2 + s) * n + x])) : 0ull;
+#pragma unroll
+    for (int v = 0; v < 7; v++) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++) o |= ((w[t] >> (8 * v)) & 0xffull) << (8 * t);
+        *reinterpret_cast<uint64_t*>(key + key_plane_byte(n, cols, Dk, s, x, col, d0, v)) = o;
+    }
 }
 
 // the inverse: digit d's bytes -> stage [cols][2][N] (centred doubles)
@@ -843,22 +854,29 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
     const size_t bytes = key_mfma_bytes(c, (int)k->digits, cols);
     uint8_t* md = nullptr;
     uint64_t* coeff = nullptr;
-    double* stage = nullptr;  // 2 P doubles (inverse NTT input), then 4 P (the centred limbs mod s1, s2)
+    double* stage = nullptr;   // 2 P doubles (inverse NTT input), then 4 P (the centred limbs mod s1, s2)
+    double* stage8 = nullptr;  // the NTT forms of 8 digits, [8][cols][2][N]
     try {
         RCK_CHECK_HIP(hipMalloc(&md, bytes));
-        RCK_CHECK_HIP(hipMemset(md, 0, bytes));  // the rounding digits and the padding stay zero
+        RCK_CHECK_HIP(hipMemset(md, 0, bytes));  // the padding stays zero
         RCK_CHECK_HIP(hipMalloc(&coeff, (size_t)2 * P * 8));
         RCK_CHECK_HIP(hipMalloc(&stage, (size_t)4 * P * 8));
+        RCK_CHECK_HIP(hipMalloc(&stage8, (size_t)8 * 4 * P * 8));
         const PrimeMap pm{nq, (int)c.L, 0};
         const PrimeMap ps{0, (int)(c.L + c.K), 0};
-        for (uint32_t d = 0; d < k->digits; d++) {
-            RCK_CHECK_HIP(hipMemcpy(stage, k->data + (size_t)d * 2 * P, 2 * P * 8, hipMemcpyDeviceToDevice));
-            launch_inv(c, stage, n, P, kOutU64, coeff, n, P, E, 2, pm, nullptr);
-            k_key_to_special<<<dim3((c.n + 255) / 256, E, 2), 256>>>(coeff, stage, (int)c.n, E, nq, (int)c.L,
-                                                                    (int)(c.L + c.K), c.d_primes);
-            RCK_CHECK_HIP(hipGetLastError());
-            launch_fwd(c, kInF64, stage, n, 2 * n, stage, n, 2 * n, 2, cols, ps, nullptr);
-            k_key_digit_to_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(stage, md, (int)c.n, cols, Dk, (int)d);
+        for (uint32_t d0 = 0; d0 < k->digits; d0 += 8) {
+            const int nd = std::min(8, (int)k->digits - (int)d0);
+            for (int t = 0; t < nd; t++) {
+                RCK_CHECK_HIP(hipMemcpy(stage, k->data + (size_t)(d0 + t) * 2 * P, 2 * P * 8,
+                                        hipMemcpyDeviceToDevice));
+                launch_inv(c, stage, n, P, kOutU64, coeff, n, P, E, 2, pm, nullptr);
+                k_key_to_special<<<dim3((c.n + 255) / 256, E, 2), 256>>>(coeff, stage, (int)c.n, E, nq, (int)c.L,
+                                                                        (int)(c.L + c.K), c.d_primes);
+                RCK_CHECK_HIP(hipGetLastError());
+                launch_fwd(c, kInF64, stage, n, 2 * n, stage8 + (size_t)t * 4 * P, n, 2 * n, 2, cols, ps, nullptr);
+            }
+            k_key_chunk_to_mfma<<<dim3((c.n + 255) / 256, cols, 2), 256>>>(stage8, md, (int)c.n, cols, Dk, (int)d0,
+                                                                         nd);
             RCK_CHECK_HIP(hipGetLastError());
         }
         RCK_CHECK_HIP(hipDeviceSynchronize());
@@ -866,10 +884,12 @@ void key_make_exact(Context& c, KeyDev* k, bool keep_data) {
         if (md) (void)hipFree(md);
         if (coeff) (void)hipFree(coeff);
         if (stage) (void)hipFree(stage);
+        if (stage8) (void)hipFree(stage8);
         throw;
     }
     (void)hipFree(coeff);
     (void)hipFree(stage);
+    (void)hipFree(stage8);
     if (!keep_data) {
         RCK_CHECK_HIP(hipFree(k->data));
         k->data = nullptr;
