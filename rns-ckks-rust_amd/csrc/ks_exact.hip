@@ -401,8 +401,16 @@ static void launch_mac_mfma(const Context& c, const uint64_t* dsb, const uint8_t
 // ---------------------------------------------------------------------------------------
 // Row passes between the standard limb rows and the slot-major operands of the matrix-core MAC.
 // A workgroup takes ring row rb of S consecutive rows z under one prime; the tile goes through LDS
-// once more so that each store (or load) instruction covers S consecutive z of a few slots.
+// once more so that each store (or load) instruction covers S consecutive z of a few slots.  That
+// exchange is stored position-major, tile_lds(z, x) = x (S + 1) + z: the S rows of one position are
+// adjacent (the slot-major side reads or writes S consecutive z of one position) and a position's
+// block is S + 1 doubles, so the row side's 16 consecutive positions of one z fall on distinct banks.
 // ---------------------------------------------------------------------------------------
+template <int S>
+__device__ __forceinline__ int tile_lds(int z, int x) {
+    return x * (S + 1) + z;  // R (S + 1) <= S row_pitch<R>() doubles
+}
+
 // Forward row stages of the digit transforms: in rows (z, s) at in + (z 2 + s) N after the column pass,
 // z = b D + i -> out[(s N + x) B Dp + b Dp + i] as signed-byte words (x the NTT-domain position), over
 // the B Dp rows zp = b Dp + i: the padding digits D <= i < Dp are written as zeros, so the
@@ -412,6 +420,7 @@ __global__ __launch_bounds__(256) void k_fwd_rows_sm(const double* __restrict__ 
                                                      int logn, int D, int Dp, int B, int sbase,
                                                      const DevPrime* __restrict__ primes) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    static_assert(R * (S + 1) <= S * row_pitch<R>(), "the exchange tile must fit the row buffer");
     __shared__ double lds[S * row_pitch<R>()];
     const int n = 1 << logn, N1 = n >> LOGR, Zp = B * Dp;
     const int tid = threadIdx.x, tau = tid % TPS, row = tid / TPS;
@@ -435,7 +444,7 @@ __global__ __launch_bounds__(256) void k_fwd_rows_sm(const double* __restrict__ 
     fwd_phase_b<LOGR>(v, P.fw, N1 + rb, tau, q, qinv);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; e++) lds[row_lds<R>(row, e * TPS + tau)] = real ? v[e] : 0.0;  // position e TPS + tau
+    for (int e = 0; e < 16; e++) lds[tile_lds<S>(row, e * TPS + tau)] = real ? v[e] : 0.0;  // position e TPS + tau
     __syncthreads();
     const int zz = tid % S, pos0 = tid / S, zo = z0 + zz;
     if (zo >= Zp) return;
@@ -443,7 +452,7 @@ __global__ __launch_bounds__(256) void k_fwd_rows_sm(const double* __restrict__ 
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int pos = pos0 + TPS * e;
-        o[(long long)pos * Zp] = sbyte_word(f64_to_i64(lds[row_lds<R>(zz, pos)]));
+        o[(long long)pos * Zp] = sbyte_word(f64_to_i64(lds[tile_lds<S>(zz, pos)]));
     }
 }
 
@@ -454,6 +463,7 @@ __global__ __launch_bounds__(256) void k_inv_rows_sm(const double* __restrict__ 
                                                      int logn, int Zc, int sbase, const DevPrime* __restrict__ primes,
                                                      const int* __restrict__ gate) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
+    static_assert(R * (S + 1) <= S * row_pitch<R>(), "the exchange tile must fit the row buffer");
     __shared__ double lds[S * row_pitch<R>()];
     if (gate != nullptr && *gate == 0) return;
     const int n = 1 << logn, N1 = n >> LOGR;
@@ -467,12 +477,12 @@ __global__ __launch_bounds__(256) void k_inv_rows_sm(const double* __restrict__ 
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int pos = pos0 + TPS * e;
-        lds[row_lds<R>(zz, pos)] = src[(long long)pos * Zc];
+        lds[tile_lds<S>(zz, pos)] = src[(long long)pos * Zc];
     }
     __syncthreads();
     double v[16];
 #pragma unroll
-    for (int e = 0; e < 16; e++) v[e] = lds[row_lds<R>(row, e * TPS + tau)];  // position e TPS + tau: x = 16 tau + e
+    for (int e = 0; e < 16; e++) v[e] = lds[tile_lds<S>(row, e * TPS + tau)];  // position e TPS + tau: x = 16 tau + e
     inv_phase_b<LOGR>(v, P.iv, N1 + rb, tau, q, qinv);
     __syncthreads();
 #pragma unroll
