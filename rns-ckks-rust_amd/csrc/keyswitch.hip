@@ -345,6 +345,18 @@ static void digit_cols_t(const Context& c, const int32_t* digits, double* out, i
     RCK_CHECK_HIP(hipGetLastError());
 }
 
+void launch_digit_cols(const Context& c, const int32_t* digits, double* out, int g, int polys, PrimeMap pm,
+                       hipStream_t s) {
+    int n1, n2;
+    split_logn((int)c.logn, &n1, &n2);
+    switch (n1) {
+        case 6: digit_cols_t<6>(c, digits, out, g, polys, pm, s); break;
+        case 7: digit_cols_t<7>(c, digits, out, g, polys, pm, s); break;
+        case 8: digit_cols_t<8>(c, digits, out, g, polys, pm, s); break;
+        default: RCK_REQUIRE(false, "unsupported ring degree");
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Digit MAC: the row pass of the forward NTT of every digit, fused with the multiply-
 // accumulate against the NTT-form key.  The column pass ran before into

@@ -970,9 +970,9 @@ void keyswitch_exact(Context& c, const uint64_t* poly, long long in_ps, int B, i
         RCK_CHECK_HIP(hipGetLastError());
     }
     const PrimeMap ps{0, (int)(c.L + c.K), 0};
-    // every digit under s1 and s2: column pass (limb stride 0 on the input: the same digit row for both),
-    // then the row stages written slot-major as signed-byte words
-    launch_fwd_cols(c, kInI32, digits, 0, n, dn, n, 2 * n, 2, B * levels, ps, s);
+    // every digit under s1 and s2: column pass (each digit row read once for both primes), then the row
+    // stages written slot-major as signed-byte words
+    launch_digit_cols(c, digits, dn, 2, B * levels, ps, s);
     launch_rows_sm(c, true, dn, dsb, nullptr, nullptr, B * levels, levels, Dp, B, 1, s);
     launch_mac_mfma(c, dsb, key->mdata, (int)key->digits, accm, B, levels, Dp, cols, pl.gsize, G, spw, s);
     // inverse rows under s1, s2 (back to limb rows); inverse columns fused with the recombination (one

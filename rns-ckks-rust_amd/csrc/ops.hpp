@@ -29,6 +29,10 @@ void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // l1 (nullable): [B][N / 64] partial sums of |digit| (each over 64 coefficients and every digit).
 void launch_decompose(const Context& c, const uint64_t* poly, int nq, int levels, int32_t* digits, hipStream_t s,
                       int B = 1, long long ps = 0, unsigned long long ginv = 0, double* l1 = nullptr);
+// Forward column pass of `polys` digit rows (int32 [polys][N]) under the g primes pm(0 .. g-1), each row read
+// once for all of them: out[z][jl][N] (reduced doubles, the row pass's input).
+void launch_digit_cols(const Context& c, const int32_t* digits, double* out, int g, int polys, PrimeMap pm,
+                       hipStream_t s);
 // For B ciphertexts sharing one key: acc_{0,1}[j] = sum_i NTT_j(digit_i) * key[i][{0,1}][j] for every eval
 // prime j (Q-prefix nq, then naux aux primes), NTT form.  digits: [B][D][N]; key: [>=D][2][nq+naux][N]
 // NTT-form doubles; part: [dsplit][B][2][nq+naux][N] partial sums over digit ranges.  Eval limbs
