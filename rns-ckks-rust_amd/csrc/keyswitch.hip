@@ -293,7 +293,12 @@ __global__ __launch_bounds__(256) void k_digit_cols(const int32_t* __restrict__ 
     __shared__ double lds[R * S];
     const int n = 1 << logn, N2 = n >> LOGR;
     const int tid = threadIdx.x, col = tid % S, tau = tid / S;
-    const int r = blockIdx.x * S + col;
+    // S = 16 int32 columns are half a 128-byte line: column blocks 2k and 2k + 1 are given to workgroups
+    // 8 apart in dispatch order, which run on the same XCD (workgroups go round-robin over the 8), so
+    // each line is fetched into one L2 once
+    int cb = blockIdx.x;
+    if (S == 16 && (gridDim.x & 15) == 0) cb = (cb & ~15) | ((cb & 7) << 1) | ((cb >> 3) & 1);
+    const int r = cb * S + col;
     const long long z = blockIdx.z;
     const int32_t* src = digits + z * n + r;
     int32_t dv[16];
