@@ -132,10 +132,34 @@ __device__ __forceinline__ void byte_transpose4(uint32_t x0, uint32_t x1, uint32
     y[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
 }
 
+// A slot's accumulator record (B ciphertexts x cols key columns) is N-tile-major: the columns of N-tile nt
+// (16, the last one w = cols - 16 (NT - 1)) of ciphertext b at 16 nt B + b w + (col - 16 nt), so the MAC
+// writes each tile's outputs as one contiguous block.  acc_record_z is the inverse, to z = b cols + col.
+__host__ __device__ __forceinline__ int acc_record_pos(int b, int col, int B, int cols) {
+    const int nt = col >> 4, NT = (cols + 15) >> 4;
+    const int w = nt == NT - 1 ? cols - 16 * (NT - 1) : 16;
+    return 16 * nt * B + b * w + (col - 16 * nt);
+}
+__host__ __device__ __forceinline__ int acc_record_z(int p, int B, int cols) {
+    const int NT = (cols + 15) >> 4, full = 16 * B * (NT - 1);
+    int nt, b, jc;
+    if (p < full) {
+        nt = p / (16 * B);
+        b = (p >> 4) % B;
+        jc = p & 15;
+    } else {
+        const int w = cols - 16 * (NT - 1);
+        nt = NT - 1;
+        b = (p - full) / w;
+        jc = (p - full) % w;
+    }
+    return b * cols + 16 * nt + jc;
+}
+
 struct MfmaArgs {
     const uint64_t* dsb;   // [2][N][B][Dp] digit words (slot-major)
     const uint8_t* key;    // [2][N][7 cols Dk] key byte planes (Dk = digits rounded up to 8), then 64 bytes of padding
-    double* acc;           // [G][2][N][B cols] accumulators (slot-major, reduced)
+    double* acc;           // [G][2][N] slot records of B cols accumulators (reduced), in acc_record_pos order
     int n, B, Dp, Dk, cols, D, gsize;
     double q[2], qinv[2], c32[2];  // the auxiliary primes, 1/s, 2^32 mod s (centred)
 };
@@ -261,13 +285,15 @@ __device__ __forceinline__ void mfma_wave(const MfmaArgs& A, int s, long long xb
         return *(const __attribute__((address_space(1))) v2u*)(dp + (uint64_t)la);
     };
     // output of a step: ciphertext 16 mt + j, columns 16 nt + 4 gq + 2 pr + {0, 1} (cols is even, so a pair
-    // is valid or not as a whole), as a byte offset in the slot's record of zc doubles; kDrop for lanes
-    // without an output
+    // is valid or not as a whole), as a byte offset in the slot's record of zc doubles (acc_record_pos: a
+    // full N-tile's 16 ciphertexts are 2 KiB of whole lines); kDrop for lanes without an output
     constexpr uint32_t kDrop = 0x80000000u;
     double* const accg = A.acc + (long long)g * 2 * A.n * zc;
     auto out_off = [&](int mt, int nt, int pr) -> uint32_t {
         const int bb = 16 * mt + j, col = 16 * nt + 4 * gq + 2 * pr;
-        return bb < A.B && col < A.cols ? (uint32_t)(bb * A.cols + col) * 8u : kDrop;
+        const bool ok = (bb < A.B) & (col < A.cols);  // both evaluated: a select, no branch
+        const uint32_t off = (uint32_t)acc_record_pos(bb, col, A.B, A.cols) * 8u;
+        return ok ? off : kDrop;
     };
     typedef double v2d __attribute__((ext_vector_type(2)));
     const int groups = nslots * MT;
@@ -456,11 +482,13 @@ __global__ __launch_bounds__(256) void k_fwd_rows_sm(const double* __restrict__ 
     }
 }
 
-// Inverse row stages of slot-major accumulators: in [G][2][N][Zc] (Zc = B cols) -> rows
-// ((g Zc + z) 2 + s) N, the layout of the fused inverse column pass and recombination.
+// Inverse row stages of slot-major accumulators: in [G][2][N] records of Zc = B cols (acc_record_pos
+// order) -> rows ((g Zc + z) 2 + s) N, z = b cols + col, the layout of the fused inverse column pass and
+// recombination.  A workgroup reads S consecutive record positions.
 template <int LOGR>
 __global__ __launch_bounds__(256) void k_inv_rows_sm(const double* __restrict__ in, double* __restrict__ out,
-                                                     int logn, int Zc, int sbase, const DevPrime* __restrict__ primes,
+                                                     int logn, int Zc, int B, int sbase,
+                                                     const DevPrime* __restrict__ primes,
                                                      const int* __restrict__ gate) {
     constexpr int R = 1 << LOGR, TPS = R / 16, S = 256 / TPS;
     static_assert(R * (S + 1) <= S * row_pitch<R>(), "the exchange tile must fit the row buffer");
@@ -493,7 +521,8 @@ __global__ __launch_bounds__(256) void k_inv_rows_sm(const double* __restrict__ 
     inv_phase_a(v, P.iv, N1 + rb, q, qinv);
     const int z = z0 + row;
     if (z >= Zc) return;
-    double* o = out + (((long long)g * Zc + z) * 2 + s) * n + (long long)rb * R;
+    const int zo = acc_record_z(z, B, Zc / B);
+    double* o = out + (((long long)g * Zc + zo) * 2 + s) * n + (long long)rb * R;
 #pragma unroll
     for (int e = 0; e < 16; e++) o[tau + TPS * e] = v[e];
 }
@@ -511,7 +540,8 @@ static void rows_sm_t(const Context& c, bool fwd, const double* din, uint64_t* d
                                                                           c.d_primes);
     } else {
         ProfScope ps(s, gate ? "ntt_inv_rows_gated" : "ntt_inv_rows");
-        k_inv_rows_sm<LOGR><<<dim3((Z + S - 1) / S, n1, 2 * G), 256, 0, s>>>(ain, aout, (int)c.logn, Z, sbase,
+        RCK_REQUIRE(B >= 1 && Z % B == 0, "bad exact keyswitch accumulator rows");
+        k_inv_rows_sm<LOGR><<<dim3((Z + S - 1) / S, n1, 2 * G), 256, 0, s>>>(ain, aout, (int)c.logn, Z, B, sbase,
                                                                              c.d_primes, gate);
     }
     RCK_CHECK_HIP(hipGetLastError());
