@@ -49,9 +49,10 @@ __device__ __forceinline__ double crt_mod_q(double a1, double a2, const SpecCons
                                             double p1m, double p12m) {
     const double d = reduce(a2 - a1, K.s2, K.s2inv);                          // |a2 - a1| < 2^51
     const double t = canon(mulmod(d, K.inv12, K.s2, K.s2inv), K.s2, K.s2inv);  // [0, s2)
-    const bool neg = t > K.half_t || (t == K.half_t && a1 > K.half_a);
+    // both comparisons evaluated (selects, no branch)
+    const bool neg = (t > K.half_t) | ((t == K.half_t) & (a1 > K.half_a));
     double r = reduce(a1, q, qinv) + mulmod(reduce(t, q, qinv), p1m, q, qinv);  // <= q/2 + 1 + 0.97q
-    if (neg) r -= p12m;
+    r -= neg ? p12m : 0.0;
     return reduce(r, q, qinv);
 }
 
@@ -635,22 +636,31 @@ __global__ __launch_bounds__(256) void k_inv_cols_crt(const double* __restrict__
     const int r = blockIdx.x * S + col;
     const long long z = blockIdx.y;
     const int jc = (int)(z % (2 * E)), comp = jc / E, j = jc % E;
+    // element e of the thread's column: a wave-uniform row base (scalar registers) plus the 32-bit lane
+    // offset lane0 + N2 e, so each load holds one address register, not two
+    const uint32_t lane0 = (uint32_t)(r + N2 * 16 * tau);
+    auto ld_lane = [](const double* base, uint32_t idx) -> double {
+        typedef const __attribute__((address_space(1))) char gchar;
+        const uint64_t v = (uint64_t)base;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+        gchar* p = (gchar*)(((uint64_t)hi << 32) | lo);
+        return *(const __attribute__((address_space(1))) double*)(p + (uint64_t)(idx * 8u));  // idx < N <= 2^17
+    };
     const int pj = prime_of(j, nq, L);
-    // both primes' columns, and the addend's, are loaded up front (in flight during the first transform)
+    // both primes' columns are loaded up front (the second in flight during the first transform); the
+    // addend's after the recombination, when the second prime's registers are free (with the scalar row
+    // bases and the branch-free recombination, 128 registers: 4 waves per SIMD instead of 2)
     const int g = (int)(z / (2LL * E * B));
     const bool fold = addr != nullptr && g == 0 && j < nq;
     double a1[16], a2[16], a3[16];
 #pragma unroll
-    for (int e = 0; e < 16; e++) a1[e] = acc[(z * 2) * n + r + (long long)N2 * (16 * tau + e)];
+    for (int e = 0; e < 16; e++) a1[e] = ld_lane(acc + (z * 2) * n, lane0 + (uint32_t)(N2 * e));
 #pragma unroll
-    for (int e = 0; e < 16; e++) a2[e] = acc[(z * 2 + 1) * n + r + (long long)N2 * (16 * tau + e)];
-    if (fold) {
-        const int b = (int)((z / (2 * E)) % B);
-        const double* dsrc = addr + (long long)b * addr_ps + ((long long)comp * nq + j) * n;
-#pragma unroll
-        for (int e = 0; e < 16; e++) a3[e] = dsrc[r + (long long)N2 * (16 * tau + e)];
-    }
+    for (int e = 0; e < 16; e++) a2[e] = ld_lane(acc + (z * 2 + 1) * n, lane0 + (uint32_t)(N2 * e));
     auto column = [&](const DevPrime& Ps, double (&v)[16]) {
+        // this transform's twiddle loads stay here (hoisted above the previous transform, each prime's
+        // would hold 30 more registers)
+        asm volatile("" ::: "memory");
         inv_phase_b<LOGR>(v, Ps.iv, 1, tau, Ps.q, Ps.qinv);
         // LDS-only barriers: the next column's loads stay in flight across this exchange
         lds_barrier();  // the previous transform's reads of lds are done
@@ -672,6 +682,11 @@ __global__ __launch_bounds__(256) void k_inv_cols_crt(const double* __restrict__
 #pragma unroll
     for (int e = 0; e < 16; e++) a1[e] = crt_mod_q(a1[e], a2[e], SK, Pj.q, Pj.qinv, spmod[2 * pj], spmod[2 * pj + 1]);
     if (fold) {
+        const int b = (int)((z / (2 * E)) % B);
+        const double* dsrc = addr + (long long)b * addr_ps + ((long long)comp * nq + j) * n;
+        asm volatile("" ::: "memory");  // keeps these loads here (hoisted, they would hold 32 more registers)
+#pragma unroll
+        for (int e = 0; e < 16; e++) a3[e] = ld_lane(dsrc, lane0 + (uint32_t)(N2 * e));
         column(Pj, a3);
         const double pm = reduce(mulmod(Pj.ninv, pmodq[j], Pj.q, Pj.qinv), Pj.q, Pj.qinv);  // N^{-1} P mod p_j
 #pragma unroll
