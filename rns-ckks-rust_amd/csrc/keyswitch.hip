@@ -127,21 +127,21 @@ __global__ __launch_bounds__(kDecBS) void k_decompose(const uint64_t* __restrict
     }
     // centring: X > floor(Q/2)  ->  X - Q, two's complement over the W words (30 W >= bits(Q): the
     // centred value lies in (-Q/2, Q/2))
+    // branch-free (selects): the first differing word from the top decides
     bool gt = false, decided = false;
 #pragma unroll
     for (int k = W - 1; k >= 0; k--) {
         const uint32_t h = hw[k];
-        if (!decided && x[k] != h) {
-            gt = x[k] > h;
-            decided = true;
-        }
+        const bool ne = x[k] != h;
+        gt = (!decided & ne) ? x[k] > h : gt;
+        decided |= ne;
     }
-    if (gt) {
+    {
         uint32_t borrow = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {
             const uint32_t t = x[k] - qw[k] - borrow;  // in (-2^31, 2^30): bit 31 is the borrow
-            x[k] = t & 0x3fffffffu;
+            x[k] = gt ? (t & 0x3fffffffu) : x[k];
             borrow = t >> 31;
         }
     }
