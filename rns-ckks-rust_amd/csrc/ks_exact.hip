@@ -238,7 +238,8 @@ __device__ __forceinline__ void mfma_tile(const v4i (&dp)[7], v4i (&kp)[7], doub
 
 // A wave's slots xbeg .. xbeg+nslots-1 (xbeg wave-uniform): for each (slot, M-tile) its N-tiles in order.
 // The products are D = key^T x digits, so lane l holds ciphertext 16 mt + l % 16 and the four key columns
-// 16 nt + 4 (l / 16) + r: 32 contiguous bytes of the [b][col] output, two 16-byte stores.
+// 16 nt + 4 (l / 16) + r: 32 contiguous bytes of the tile's block in the slot's accumulator record
+// (acc_record_pos), two 16-byte stores.
 // No memory operation of the loops is conditional, so the in-order memory counter lets every wait cover
 // exactly the loads it needs:
 //  * each step's refills load the next N-tile's key planes; the last N-tile's also load the next
